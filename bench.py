@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark: plain Ligerito prove of 2^24 GF(2^32) coefficients on one GPU.
+
+Workload: the reference's hardcoded_config_24 (initial matrix 2^18 x 2^6,
+two recursive steps), input drawn from SplitMix64 seed 2 and resident in
+HBM before the timed region, transcript seed 1234.  A step is one complete
+prove (commit, fold, commit, openings, induce, sumcheck rounds, ...).
+
+Prints one JSON line with throughput (coefficients proved per second), the
+roofline of the dominant kernel group (the first commit's Reed–Solomon
+encode) from HIP events on the library stream, and a CPU baseline: the
+oracle (plain-C restatement of the reference algorithm) timed on this
+host's cores on one full prove of the same input.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ligerito-rust_amd"))
+
+LOG_N = 24
+SEED = 2
+FS_SEED = 1234
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--log-n", type=int, default=LOG_N)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: the Ligerito prove path runs on a single GPU here "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import ligerito_hip as lh
+
+    L = args.log_n
+    N = 1 << L
+    ctx = lh.Context(0)
+    log("plan setup (twiddle/pis tables), untimed like ReedSolomonEncoding::new in hardcoded_config_24")
+    cfg = lh.ProverConfig(ctx, L, 32)
+
+    # SplitMix64 input, generated on the host then placed in HBM
+    poly = _splitmix_u32(SEED, N)
+    d_poly = torch.from_numpy(poly.view(np.int32)).to("cuda:0")
+    torch.cuda.synchronize()
+    ptr = d_poly.data_ptr()
+
+    for i in range(args.warmup):
+        lh.prover_dev(cfg, ptr, FS_SEED)
+        log(f"warmup {i + 1}/{args.warmup} done")
+
+    ctx.profile(True)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    proof = None
+    for _ in range(args.steps):
+        proof = lh.prover_dev(cfg, ptr, FS_SEED)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    phases = {p: ctx.profile_read(p) for p in ("encode0", "encode", "merkle", "fold", "induce", "sumcheck", "open")}
+    ctx.profile(False)
+    log(f"timed {args.steps} proves in {elapsed * 1e3:.2f} ms")
+
+    ok = lh.verifier(L, proof, FS_SEED)
+    if ok != 1:
+        log(f"verifier rejected the GPU proof ({ok})")
+        sys.exit(1)
+
+    # host-resident input (PCIe-inclusive drop-in semantics), reported beside
+    t2 = time.perf_counter()
+    lh.prover(cfg, poly, FS_SEED)
+    t_host = time.perf_counter() - t2
+
+    enc_ms, enc_n = phases["encode0"]
+    enc_avg_s = (enc_ms / max(enc_n, 1)) / 1e3
+    alg_bytes = 5 * N * 4  # read N*|F|, write 4N*|F|
+    achieved = alg_bytes / enc_avg_s / 1e9 if enc_avg_s > 0 else 0.0
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "encode0_traffic.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("hbm_bytes_per_encode")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = _cpu_baseline(poly, L)
+
+    value = args.steps * N / elapsed
+    line = {
+        "metric": "Ligerito prove coeffs/s at 2^24, batched vs plain GPU; served p99 ms",
+        "value": value,
+        "unit": "coeffs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain Ligerito prove (hardcoded_config_24 shape), 2^{L} GF(2^32) coefficients, "
+                        f"device-resident input, SplitMix64 seed {SEED}, transcript seed {FS_SEED}",
+            "log_n": L,
+            "field": "GF(2^32)",
+            "path": "plain single-proof GPU drop-in",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "first-commit Reed-Solomon encode (transpose+scale + FFT pass launches)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes": alg_bytes,
+            "avg_duration_us": enc_avg_s * 1e6,
+        },
+        "cpu_baseline": cpu,
+        "phases_ms_per_prove": {k: v[0] / max(args.steps, 1) for k, v in phases.items()},
+        "host_input_coeffs_per_s": N / t_host,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def _splitmix_u32(seed, n):
+    import numpy as np
+
+    # SplitMix64, low 32 bits of each draw (the oracle's generator)
+    idx = np.arange(1, n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+def _cpu_baseline(poly, L):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    O.build()
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    log(f"cpu baseline: one full 2^{L} prove with the oracle on {cores} threads")
+    t0 = time.perf_counter()
+    O.prove(poly, L, 32)
+    dt = time.perf_counter() - t0
+    return {
+        "value": (1 << L) / dt,
+        "unit": "coeffs/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"one complete 2^{L} GF(2^32) prove of the same input (oracle: plain-C restatement of the "
+                  f"reference algorithm, bit-serial field multiply, OpenMP over columns/queries), {dt:.2f} s",
+    }
+
+
+if __name__ == "__main__":
+    main()

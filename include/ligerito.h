@@ -1,0 +1,170 @@
+/*
+ * ligerito.h — C ABI of the MI355X (gfx950) Ligerito prover hot path.
+ *
+ * Drop-in boundary for the public API of the reference crate
+ * `cryptoutils` (ashutosh1206/ligerito-rust).  Each entry point names the
+ * reference item it replaces (file:line).  Conventions:
+ *
+ *  - Field elements cross the ABI as little-endian bytes of field_bits/8
+ *    bytes each (BinaryField::to_bytes, binary_field.rs:161-163):
+ *    uint32_t arrays for GF(2^32), pairs of uint64_t (low word first) for
+ *    GF(2^128).  field_bits is 32 or 128.
+ *  - Polynomials and encoded matrices use the reference layout: a ligero
+ *    matrix is column-major, column j = poly[j*m .. (j+1)*m] (ligero.rs:11-27).
+ *  - Every function returns LIG_OK (0) or a negative LIG_E_* status; the
+ *    reference's panics (assert!, unwrap, expect) map to LIG_E_ASSERT and
+ *    nothing unwinds across the ABI.  lig_last_error() describes the last
+ *    failure on the calling thread.
+ *  - Buffers are caller-owned; handles are library-owned and freed with the
+ *    matching *_free.  Calls on one lig_ctx are stream-ordered and must not
+ *    overlap from several threads.
+ */
+#ifndef LIGERITO_H
+#define LIGERITO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LIG_OK 0
+#define LIG_E_INVALID (-1)    /* bad argument (a reference assert on inputs) */
+#define LIG_E_ASSERT (-2)     /* a reference assert_eq!/unwrap would panic */
+#define LIG_E_HIP (-3)        /* HIP runtime failure */
+#define LIG_E_OOM (-4)        /* device allocation failed */
+#define LIG_E_NO_DEVICE (-5)  /* no usable GPU */
+#define LIG_E_MALFORMED (-6)  /* undecodable proof bytes */
+#define LIG_E_CAPACITY (-7)   /* caller buffer too small */
+#define LIG_E_MERKLE (-8)     /* verifier: Merkle opening check failed (verifier.rs:50,97-106,135-144) */
+#define LIG_E_LIGERO (-9)     /* verifier: ligero_verify row check failed (ligero.rs:124) */
+
+/* Description of the last error on this thread ("" if none). */
+const char *lig_last_error(void);
+/* Number of visible GPUs (0 on a host without one). */
+int lig_device_count(void);
+
+/* ------------------------------------------------------------ context -- */
+/* Device + stream + plan caches.  Replaces the reference's implicit global
+ * rayon pool (no counterpart type in the reference). */
+typedef struct lig_ctx lig_ctx;
+int lig_ctx_create(int device, lig_ctx **out);
+void lig_ctx_destroy(lig_ctx *ctx);
+/* Instrumentation (no counterpart in the reference; for benchmarks): with
+ * profiling on, the library records a HIP event pair on its stream around
+ * each phase of its work and accumulates the elapsed time per phase.
+ * Phases: "encode0" (Reed–Solomon encode of the first commit of a prove:
+ * transpose+scale and FFT pass kernels), "encode" (encodes of the other
+ * commits), "merkle" (leaf hashing + tree levels), "fold", "induce",
+ * "sumcheck", "open".  Enabling (or disabling) resets the totals. */
+int lig_ctx_profile(lig_ctx *ctx, int enable);
+int lig_ctx_profile_read(lig_ctx *ctx, const char *phase, double *total_ms, uint64_t *count);
+/* The hipStream_t all work of this context is ordered on. */
+void *lig_ctx_stream(lig_ctx *ctx);
+int lig_ctx_synchronize(lig_ctx *ctx);
+
+/* -------------------------------------------------------- Reed–Solomon -- */
+/* ReedSolomonEncoding<F> (reed_solomon.rs:7-17). */
+typedef struct lig_rs lig_rs;
+/* ReedSolomonEncoding::new (reed_solomon.rs:49-64): twiddles for the block
+ * length and pis for the message length; both powers of two,
+ * message_length < block_length.  Device tables are built here (setup). */
+int lig_rs_new(lig_ctx *ctx, int field_bits, size_t message_length, size_t block_length, lig_rs **out);
+void lig_rs_free(lig_rs *rs);
+/* message_length() / block_length() (reed_solomon.rs:66-72). */
+size_t lig_rs_message_length(const lig_rs *rs);
+size_t lig_rs_block_length(const lig_rs *rs);
+/* encode (reed_solomon.rs:90-102): systematic; host buffers, out has
+ * block_length elements. */
+int lig_rs_encode(lig_rs *rs, const void *message, void *out);
+/* encode_non_systematic (reed_solomon.rs:104-113): in place over a host
+ * buffer of block_length elements. */
+int lig_rs_encode_non_systematic(lig_rs *rs, void *message);
+/* Device variant of encode_non_systematic over n_cols columns stored
+ * column-major (each column block_length elements), as encode_cols
+ * (ligero.rs:29-38) applies it.  Synchronous. */
+int lig_rs_encode_non_systematic_cols_dev(lig_rs *rs, void *d_columns, size_t n_cols);
+/* compute_twiddles output of this plan (binary_fft.rs:30-58): block_length-1
+ * elements. */
+int lig_rs_twiddles(const lig_rs *rs, void *out);
+
+/* ------------------------------------------------------------- Merkle -- */
+/* build_merkle_tree (merkle_tree.rs:22-60) over n_leaves fixed-length
+ * leaves (host buffer, leaf i = leaves[i*leaf_len ..]).  layers_out receives
+ * every layer, leaf hashes first, 32*(2*n_leaves-1) bytes. */
+int lig_build_merkle_tree(lig_ctx *ctx, const void *leaves, size_t n_leaves, size_t leaf_len, uint8_t *layers_out);
+/* prove (merkle_tree.rs:98-111) from the layers written above; queries
+ * sorted and distinct.  Writes up to cap_nodes 32-byte nodes. */
+int lig_merkle_prove(const uint8_t *layers, size_t n_leaves, const size_t *queries, size_t n_queries, uint8_t *proof_out,
+                     size_t cap_nodes, size_t *n_nodes);
+/* verify (merkle_tree.rs:164-188): returns 1 (valid) or 0. */
+int lig_merkle_verify(const uint8_t root[32], const uint8_t *proof, size_t n_nodes, size_t depth, const void *leaves,
+                      size_t leaf_len, const size_t *indices, size_t n_queries);
+
+/* ---------------------------------------------------------- transcript -- */
+/* FS (emulated_fs.rs:6-75). */
+typedef struct lig_fs lig_fs;
+int lig_fs_new(uint32_t seed, lig_fs **out);             /* FS::new (:12-16) */
+void lig_fs_free(lig_fs *fs);
+int lig_fs_absorb(lig_fs *fs, const void *bytes, size_t len); /* absorb (:18-20) */
+int lig_fs_squeeze(lig_fs *fs, uint8_t out16[16]);      /* squeeze (:22-32), u128 LE */
+int lig_fs_get_field(lig_fs *fs, int field_bits, void *out); /* get_field (:34-54) */
+size_t lig_fs_get_query(lig_fs *fs, size_t n);          /* get_query (:56-59) */
+/* get_distinct_queries (:61-75); s <= n is required (the reference loops
+ * forever otherwise): LIG_E_INVALID. */
+int lig_fs_get_distinct_queries(lig_fs *fs, size_t n, size_t s, size_t *out);
+
+/* --------------------------------------------------------- ligero_commit -- */
+/* RecursiveLigeroWitness<F> (data_structures.rs:32-40), device-resident. */
+typedef struct lig_witness lig_witness;
+/* ligero_commit (ligero.rs:70-95): poly has m*n elements (host), rs must
+ * have message_length m and block_length 4m. */
+int lig_ligero_commit(lig_rs *rs, const void *poly, size_t m, size_t n, lig_witness **out);
+/* Same with poly already in device memory. */
+int lig_ligero_commit_dev(lig_rs *rs, const void *d_poly, size_t m, size_t n, lig_witness **out);
+int lig_witness_root(const lig_witness *w, uint8_t root[32]);    /* tree.last() */
+size_t lig_witness_num_rows(const lig_witness *w);
+size_t lig_witness_num_cols(const lig_witness *w);
+/* flat_mat in the reference's column-major layout (4m*n elements). */
+int lig_witness_flat_mat(const lig_witness *w, void *out);
+/* every Merkle layer, leaf hashes first (32*(2*num_rows-1) bytes). */
+int lig_witness_layers(const lig_witness *w, uint8_t *out);
+/* extract_row (ligero.rs:40-51). */
+int lig_witness_extract_row(const lig_witness *w, size_t row, void *out);
+void lig_witness_free(lig_witness *w);
+
+/* ---------------------------------------------------------------- config -- */
+/* ProverConfig<F> (data_structures.rs:9-21).  log_n = 20 and 24 build
+ * exactly hardcoded_config_20 / hardcoded_config_24 (config.rs:6-47,69-112);
+ * other log_n in [16, 27] follow the same shape rules. */
+typedef struct lig_prover_config lig_prover_config;
+int lig_prover_config_new(lig_ctx *ctx, int field_bits, int log_n, lig_prover_config **out);
+void lig_prover_config_free(lig_prover_config *cfg);
+
+/* ---------------------------------------------------------------- prove -- */
+/* FinalizedLigeritoProof<F> (data_structures.rs:91-103). */
+typedef struct lig_proof lig_proof;
+/* prover (prover.rs:15-199): poly has 2^log_n elements (host).  fs_seed is
+ * the transcript seed; the reference hard-codes 1234 (prover.rs:29). */
+int lig_prover(lig_prover_config *cfg, const void *poly, uint32_t fs_seed, lig_proof **out);
+/* Same with poly already in device memory (left unmodified). */
+int lig_prover_dev(lig_prover_config *cfg, const void *d_poly, uint32_t fs_seed, lig_proof **out);
+/* Canonical encoding (the reference has none): little-endian, struct field
+ * order; see INTEGRATION.md.  With buf == NULL only *len is set. */
+int lig_proof_serialize(const lig_proof *p, uint8_t *buf, size_t cap, size_t *len);
+int lig_proof_deserialize(int field_bits, const uint8_t *buf, size_t len, lig_proof **out);
+/* ProofSize::proof_size (data_structures.rs:105-194). */
+size_t lig_proof_size(const lig_proof *p);
+void lig_proof_free(lig_proof *p);
+
+/* --------------------------------------------------------------- verify -- */
+/* verifier (verifier.rs:10-164) on the configuration for 2^log_n (host
+ * CPU).  Returns 1 (accept), 0 (final sumcheck check false), or
+ * LIG_E_MERKLE / LIG_E_LIGERO / LIG_E_ASSERT where the reference panics. */
+int lig_verifier(int field_bits, int log_n, const lig_proof *p, uint32_t fs_seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LIGERITO_H */

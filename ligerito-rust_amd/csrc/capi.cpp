@@ -1,0 +1,515 @@
+// extern "C" boundary (include/ligerito.h).  Every entry point catches
+// library exceptions and returns a status; nothing unwinds across the ABI.
+#include <array>
+#include <exception>
+
+#include "engine.hpp"
+
+using namespace lig;
+
+struct lig_ctx {
+    Ctx c;
+};
+
+struct lig_rs {
+    int bits;
+    lig_ctx* ctx;
+    std::unique_ptr<RsPlan<H32>> p32;
+    std::unique_ptr<RsPlan<H128>> p128;
+};
+
+struct lig_witness {
+    int bits;
+    Witness<H32> w32;
+    Witness<H128> w128;
+};
+
+struct lig_fs {
+    Transcript t;
+    explicit lig_fs(uint32_t s) : t(s) {}
+};
+
+struct lig_prover_config {
+    int bits;
+    Config<H32> c32;
+    Config<H128> c128;
+};
+
+struct lig_proof {
+    int bits;
+    Proof<H32> p32;
+    Proof<H128> p128;
+};
+
+namespace {
+template <class Fn> int guarded(Fn&& fn) {
+    try {
+        set_error("");
+        return fn();
+    } catch (const Error& e) {
+        set_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return LIG_E_OOM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return LIG_E_ASSERT;
+    }
+}
+
+int check_bits(int bits) {
+    LIG_REQUIRE(bits == 32 || bits == 128, LIG_E_INVALID, "field_bits must be 32 or 128");
+    return 0;
+}
+
+size_t elem_bytes(int bits) { return (size_t)bits / 8; }
+
+// Upload a host buffer into a pooled device buffer.
+template <class T> PBuf<T> upload(Ctx& c, const void* host, size_t count) {
+    PBuf<T> b(&c.pool, count);
+    LIG_HIP(hipMemcpyAsync(b.p, host, count * sizeof(T), hipMemcpyHostToDevice, c.stream));
+    return b;
+}
+}  // namespace
+
+extern "C" {
+
+const char* lig_last_error(void) { return last_error(); }
+
+int lig_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int lig_ctx_create(int device, lig_ctx** out) {
+    return guarded([&] {
+        LIG_REQUIRE(out, LIG_E_INVALID, "out is NULL");
+        int n = lig_device_count();
+        LIG_REQUIRE(n > 0, LIG_E_NO_DEVICE, "no HIP device visible");
+        LIG_REQUIRE(device >= 0 && device < n, LIG_E_INVALID, "device index out of range");
+        LIG_HIP(hipSetDevice(device));
+        std::unique_ptr<lig_ctx> c(new lig_ctx());
+        c->c.device = device;
+        LIG_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        *out = c.release();
+        return LIG_OK;
+    });
+}
+
+void lig_ctx_destroy(lig_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    try {
+        ctx->c.collect();
+    } catch (...) {
+    }
+    ctx->c.pool.drain();
+    (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+}
+
+int lig_ctx_profile(lig_ctx* ctx, int enable) {
+    return guarded([&] {
+        LIG_REQUIRE(ctx, LIG_E_INVALID, "ctx is NULL");
+        ctx->c.collect();
+        ctx->c.profiling = enable != 0;
+        for (int i = 0; i < PH_COUNT; i++) {
+            ctx->c.phase_ms[i] = 0;
+            ctx->c.phase_n[i] = 0;
+        }
+        return LIG_OK;
+    });
+}
+
+int lig_ctx_profile_read(lig_ctx* ctx, const char* phase, double* total_ms, uint64_t* count) {
+    return guarded([&] {
+        LIG_REQUIRE(ctx && phase && total_ms && count, LIG_E_INVALID, "NULL argument");
+        ctx->c.collect();
+        for (int i = 0; i < PH_COUNT; i++)
+            if (strcmp(phase, phase_name(i)) == 0) {
+                *total_ms = ctx->c.phase_ms[i];
+                *count = ctx->c.phase_n[i];
+                return LIG_OK;
+            }
+        throw Error(LIG_E_INVALID, std::string("unknown phase ") + phase);
+    });
+}
+
+void* lig_ctx_stream(lig_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
+
+int lig_ctx_synchronize(lig_ctx* ctx) {
+    return guarded([&] {
+        LIG_REQUIRE(ctx, LIG_E_INVALID, "ctx is NULL");
+        ctx->c.sync();
+        return LIG_OK;
+    });
+}
+
+// ------------------------------------------------------------ Reed–Solomon --
+int lig_rs_new(lig_ctx* ctx, int field_bits, size_t message_length, size_t block_length, lig_rs** out) {
+    return guarded([&] {
+        LIG_REQUIRE(ctx && out, LIG_E_INVALID, "NULL argument");
+        check_bits(field_bits);
+        LIG_HIP(hipSetDevice(ctx->c.device));
+        std::unique_ptr<lig_rs> r(new lig_rs());
+        r->bits = field_bits;
+        r->ctx = ctx;
+        if (field_bits == 32) {
+            r->p32.reset(new RsPlan<H32>());
+            r->p32->init(&ctx->c, message_length, block_length);
+        } else {
+            r->p128.reset(new RsPlan<H128>());
+            r->p128->init(&ctx->c, message_length, block_length);
+        }
+        *out = r.release();
+        return LIG_OK;
+    });
+}
+
+void lig_rs_free(lig_rs* rs) { delete rs; }
+
+size_t lig_rs_message_length(const lig_rs* rs) { return !rs ? 0 : rs->bits == 32 ? rs->p32->m() : rs->p128->m(); }
+size_t lig_rs_block_length(const lig_rs* rs) { return !rs ? 0 : rs->bits == 32 ? rs->p32->M() : rs->p128->M(); }
+
+int lig_rs_twiddles(const lig_rs* rs, void* out) {
+    return guarded([&] {
+        LIG_REQUIRE(rs && out, LIG_E_INVALID, "NULL argument");
+        if (rs->bits == 32) memcpy(out, rs->p32->tw.data(), rs->p32->tw.size() * 4);
+        else memcpy(out, rs->p128->tw.data(), rs->p128->tw.size() * 16);
+        return LIG_OK;
+    });
+}
+
+int lig_rs_encode(lig_rs* rs, const void* message, void* out) {
+    return guarded([&] {
+        LIG_REQUIRE(rs && message && out, LIG_E_INVALID, "NULL argument");
+        LIG_HIP(hipSetDevice(rs->ctx->c.device));
+        if (rs->bits == 32) rs->p32->encode_systematic((const uint32_t*)message, (uint32_t*)out);
+        else rs->p128->encode_systematic((const u128v*)message, (u128v*)out);
+        return LIG_OK;
+    });
+}
+
+int lig_rs_encode_non_systematic_cols_dev(lig_rs* rs, void* d_columns, size_t n_cols) {
+    return guarded([&] {
+        LIG_REQUIRE(rs && d_columns && n_cols, LIG_E_INVALID, "bad argument");
+        LIG_HIP(hipSetDevice(rs->ctx->c.device));
+        if (rs->bits == 32) rs->p32->encode_non_systematic_cols((uint32_t*)d_columns, n_cols);
+        else rs->p128->encode_non_systematic_cols((u128v*)d_columns, n_cols);
+        return LIG_OK;
+    });
+}
+
+int lig_rs_encode_non_systematic(lig_rs* rs, void* message) {
+    return guarded([&] {
+        LIG_REQUIRE(rs && message, LIG_E_INVALID, "NULL argument");
+        Ctx& c = rs->ctx->c;
+        LIG_HIP(hipSetDevice(c.device));
+        size_t M = lig_rs_block_length(rs), eb = elem_bytes(rs->bits);
+        PBuf<uint8_t> d(&c.pool, M * eb);
+        LIG_HIP(hipMemcpyAsync(d.p, message, M * eb, hipMemcpyHostToDevice, c.stream));
+        if (rs->bits == 32) rs->p32->encode_non_systematic_cols((uint32_t*)d.p, 1);
+        else rs->p128->encode_non_systematic_cols((u128v*)d.p, 1);
+        LIG_HIP(hipMemcpyAsync(message, d.p, M * eb, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        return LIG_OK;
+    });
+}
+
+// ------------------------------------------------------------------ Merkle --
+int lig_build_merkle_tree(lig_ctx* ctx, const void* leaves, size_t n_leaves, size_t leaf_len, uint8_t* layers_out) {
+    return guarded([&] {
+        LIG_REQUIRE(ctx && leaves && layers_out, LIG_E_INVALID, "NULL argument");
+        LIG_REQUIRE(is_pow2(n_leaves), LIG_E_INVALID, "build_merkle_tree: leaf count must be a power of two");
+        LIG_REQUIRE(leaf_len % 4 == 0 && leaf_len > 0 && leaf_len <= 256 * 1024, LIG_E_INVALID,
+                    "leaf_len must be a positive multiple of 4 up to 256 KiB");
+        Ctx& c = ctx->c;
+        LIG_HIP(hipSetDevice(c.device));
+        // a leaf is a row of leaf_len/4 32-bit words
+        PBuf<uint32_t> d = upload<uint32_t>(c, leaves, n_leaves * leaf_len / 4);
+        PBuf<uint32_t> tree(&c.pool, (2 * n_leaves - 1) * 8);
+        launch_leaf_hash<F32>(d.p, n_leaves, leaf_len / 4, tree.p, c.stream);
+        launch_merkle_levels(tree.p, n_leaves, c.stream);
+        LIG_HIP(hipGetLastError());
+        LIG_HIP(hipMemcpyAsync(layers_out, tree.p, (2 * n_leaves - 1) * 32, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        return LIG_OK;
+    });
+}
+
+int lig_merkle_prove(const uint8_t* layers, size_t n_leaves, const size_t* queries, size_t n_queries, uint8_t* proof_out,
+                     size_t cap_nodes, size_t* n_nodes) {
+    return guarded([&] {
+        LIG_REQUIRE(layers && queries && n_nodes && n_queries, LIG_E_INVALID, "bad argument");
+        LIG_REQUIRE(is_pow2(n_leaves), LIG_E_INVALID, "leaf count must be a power of two");
+        std::vector<size_t> q(queries, queries + n_queries);
+        for (size_t i = 0; i < q.size(); i++) {
+            LIG_REQUIRE(q[i] < n_leaves, LIG_E_INVALID, "query out of range");
+            LIG_REQUIRE(i == 0 || q[i] > q[i - 1], LIG_E_INVALID, "queries must be sorted and distinct");
+        }
+        std::vector<uint64_t> nodes = merkle_proof_nodes(n_leaves, q);
+        *n_nodes = nodes.size();
+        if (!proof_out) return LIG_OK;
+        LIG_REQUIRE(nodes.size() <= cap_nodes, LIG_E_CAPACITY, "proof buffer too small");
+        for (size_t i = 0; i < nodes.size(); i++) memcpy(proof_out + 32 * i, layers + 32 * nodes[i], 32);
+        return LIG_OK;
+    });
+}
+
+int lig_merkle_verify(const uint8_t root[32], const uint8_t* proof, size_t n_nodes, size_t depth, const void* leaves,
+                      size_t leaf_len, const size_t* indices, size_t n_queries) {
+    int r = guarded([&] {
+        LIG_REQUIRE(root && leaves && indices && n_queries, LIG_E_INVALID, "bad argument");
+        std::vector<std::array<uint8_t, 32>> lh(n_queries);
+        for (size_t i = 0; i < n_queries; i++) blake3_bytes((const uint8_t*)leaves + i * leaf_len, leaf_len, lh[i].data());
+        std::vector<size_t> q(indices, indices + n_queries);
+        return merkle_verify_hashes(root, proof, n_nodes, depth, lh, q) ? 1 : 0;
+    });
+    return r < 0 ? 0 : r;
+}
+
+// -------------------------------------------------------------- transcript --
+int lig_fs_new(uint32_t seed, lig_fs** out) {
+    return guarded([&] {
+        LIG_REQUIRE(out, LIG_E_INVALID, "out is NULL");
+        *out = new lig_fs(seed);
+        return LIG_OK;
+    });
+}
+void lig_fs_free(lig_fs* fs) { delete fs; }
+int lig_fs_absorb(lig_fs* fs, const void* bytes, size_t len) {
+    return guarded([&] {
+        LIG_REQUIRE(fs && (bytes || !len), LIG_E_INVALID, "bad argument");
+        fs->t.absorb(bytes, len);
+        return LIG_OK;
+    });
+}
+int lig_fs_squeeze(lig_fs* fs, uint8_t out16[16]) {
+    return guarded([&] {
+        LIG_REQUIRE(fs && out16, LIG_E_INVALID, "bad argument");
+        uint64_t lo, hi;
+        fs->t.squeeze(&lo, &hi);
+        memcpy(out16, &lo, 8);
+        memcpy(out16 + 8, &hi, 8);
+        return LIG_OK;
+    });
+}
+int lig_fs_get_field(lig_fs* fs, int field_bits, void* out) {
+    return guarded([&] {
+        LIG_REQUIRE(fs && out, LIG_E_INVALID, "bad argument");
+        check_bits(field_bits);
+        if (field_bits == 32) {
+            uint32_t v = fs->t.get_field<H32>();
+            memcpy(out, &v, 4);
+        } else {
+            u128v v = fs->t.get_field<H128>();
+            H128::to_bytes(v, (uint8_t*)out);
+        }
+        return LIG_OK;
+    });
+}
+size_t lig_fs_get_query(lig_fs* fs, size_t n) { return (fs && n) ? fs->t.get_query(n) : 0; }
+int lig_fs_get_distinct_queries(lig_fs* fs, size_t n, size_t s, size_t* out) {
+    return guarded([&] {
+        LIG_REQUIRE(fs && out, LIG_E_INVALID, "bad argument");
+        std::vector<size_t> q = fs->t.get_distinct_queries(n, s);
+        memcpy(out, q.data(), s * sizeof(size_t));
+        return LIG_OK;
+    });
+}
+
+// ----------------------------------------------------------- ligero_commit --
+static int commit_impl(lig_rs* rs, const void* poly, size_t m, size_t n, bool on_device, lig_witness** out) {
+    return guarded([&] {
+        LIG_REQUIRE(rs && poly && out, LIG_E_INVALID, "NULL argument");
+        Ctx& c = rs->ctx->c;
+        LIG_HIP(hipSetDevice(c.device));
+        std::unique_ptr<lig_witness> w(new lig_witness());
+        w->bits = rs->bits;
+        if (rs->bits == 32) {
+            PBuf<uint32_t> tmp;
+            const uint32_t* d = (const uint32_t*)poly;
+            if (!on_device) {
+                tmp = upload<uint32_t>(c, poly, m * n);
+                d = tmp.p;
+            }
+            commit<H32>(*rs->p32, d, m, n, w->w32);
+        } else {
+            PBuf<u128v> tmp;
+            const u128v* d = (const u128v*)poly;
+            if (!on_device) {
+                tmp = upload<u128v>(c, poly, m * n);
+                d = tmp.p;
+            }
+            commit<H128>(*rs->p128, d, m, n, w->w128);
+        }
+        *out = w.release();
+        return LIG_OK;
+    });
+}
+
+int lig_ligero_commit(lig_rs* rs, const void* poly, size_t m, size_t n, lig_witness** out) {
+    return commit_impl(rs, poly, m, n, false, out);
+}
+int lig_ligero_commit_dev(lig_rs* rs, const void* d_poly, size_t m, size_t n, lig_witness** out) {
+    return commit_impl(rs, d_poly, m, n, true, out);
+}
+
+int lig_witness_root(const lig_witness* w, uint8_t root[32]) {
+    return guarded([&] {
+        LIG_REQUIRE(w && root, LIG_E_INVALID, "NULL argument");
+        memcpy(root, w->bits == 32 ? w->w32.root : w->w128.root, 32);
+        return LIG_OK;
+    });
+}
+size_t lig_witness_num_rows(const lig_witness* w) { return !w ? 0 : w->bits == 32 ? w->w32.M : w->w128.M; }
+size_t lig_witness_num_cols(const lig_witness* w) { return !w ? 0 : w->bits == 32 ? w->w32.n : w->w128.n; }
+
+}  // extern "C"
+
+template <class H> static void flat_mat_of(const Witness<H>& w, void* out) {
+    using T = typename H::T;
+    std::vector<T> rm(w.M * w.n);
+    LIG_HIP(hipMemcpyAsync(rm.data(), w.mat.p, rm.size() * sizeof(T), hipMemcpyDeviceToHost, w.ctx->stream));
+    LIG_HIP(hipStreamSynchronize(w.ctx->stream));
+    T* o = (T*)out;
+    for (size_t j = 0; j < w.n; j++)
+        for (size_t i = 0; i < w.M; i++) o[j * w.M + i] = rm[i * w.n + j];
+}
+
+extern "C" {
+
+int lig_witness_flat_mat(const lig_witness* w, void* out) {
+    return guarded([&] {
+        LIG_REQUIRE(w && out, LIG_E_INVALID, "NULL argument");
+        if (w->bits == 32) flat_mat_of<H32>(w->w32, out);
+        else flat_mat_of<H128>(w->w128, out);
+        return LIG_OK;
+    });
+}
+
+int lig_witness_layers(const lig_witness* w, uint8_t* out) {
+    return guarded([&] {
+        LIG_REQUIRE(w && out, LIG_E_INVALID, "NULL argument");
+        const uint32_t* t = w->bits == 32 ? w->w32.tree.p : w->w128.tree.p;
+        size_t M = lig_witness_num_rows(w);
+        hipStream_t s = w->bits == 32 ? w->w32.ctx->stream : w->w128.ctx->stream;
+        LIG_HIP(hipMemcpyAsync(out, t, (2 * M - 1) * 32, hipMemcpyDeviceToHost, s));
+        LIG_HIP(hipStreamSynchronize(s));
+        return LIG_OK;
+    });
+}
+
+int lig_witness_extract_row(const lig_witness* w, size_t row, void* out) {
+    return guarded([&] {
+        LIG_REQUIRE(w && out, LIG_E_INVALID, "NULL argument");
+        LIG_REQUIRE(row < lig_witness_num_rows(w), LIG_E_INVALID, "row out of range");
+        size_t n = lig_witness_num_cols(w), eb = elem_bytes(w->bits);
+        const void* src = w->bits == 32 ? (const void*)(w->w32.mat.p + row * n) : (const void*)(w->w128.mat.p + row * n);
+        hipStream_t s = w->bits == 32 ? w->w32.ctx->stream : w->w128.ctx->stream;
+        LIG_HIP(hipMemcpyAsync(out, src, n * eb, hipMemcpyDeviceToHost, s));
+        LIG_HIP(hipStreamSynchronize(s));
+        return LIG_OK;
+    });
+}
+
+void lig_witness_free(lig_witness* w) { delete w; }
+
+// ------------------------------------------------------------------ config --
+int lig_prover_config_new(lig_ctx* ctx, int field_bits, int log_n, lig_prover_config** out) {
+    return guarded([&] {
+        LIG_REQUIRE(ctx && out, LIG_E_INVALID, "NULL argument");
+        check_bits(field_bits);
+        LIG_HIP(hipSetDevice(ctx->c.device));
+        std::unique_ptr<lig_prover_config> c(new lig_prover_config());
+        c->bits = field_bits;
+        if (field_bits == 32) c->c32.init(&ctx->c, log_n);
+        else c->c128.init(&ctx->c, log_n);
+        *out = c.release();
+        return LIG_OK;
+    });
+}
+void lig_prover_config_free(lig_prover_config* cfg) { delete cfg; }
+
+// ------------------------------------------------------------------- prove --
+static int prover_impl(lig_prover_config* cfg, const void* poly, uint32_t fs_seed, bool on_device, lig_proof** out) {
+    return guarded([&] {
+        LIG_REQUIRE(cfg && poly && out, LIG_E_INVALID, "NULL argument");
+        std::unique_ptr<lig_proof> p(new lig_proof());
+        p->bits = cfg->bits;
+        if (cfg->bits == 32) {
+            Ctx& c = *cfg->c32.ctx;
+            LIG_HIP(hipSetDevice(c.device));
+            PBuf<uint32_t> tmp;
+            const uint32_t* d = (const uint32_t*)poly;
+            if (!on_device) {
+                tmp = upload<uint32_t>(c, poly, (size_t)1 << cfg->c32.log_n);
+                d = tmp.p;
+            }
+            prove<H32>(cfg->c32, d, fs_seed, p->p32);
+        } else {
+            Ctx& c = *cfg->c128.ctx;
+            LIG_HIP(hipSetDevice(c.device));
+            PBuf<u128v> tmp;
+            const u128v* d = (const u128v*)poly;
+            if (!on_device) {
+                tmp = upload<u128v>(c, poly, (size_t)1 << cfg->c128.log_n);
+                d = tmp.p;
+            }
+            prove<H128>(cfg->c128, d, fs_seed, p->p128);
+        }
+        *out = p.release();
+        return LIG_OK;
+    });
+}
+
+int lig_prover(lig_prover_config* cfg, const void* poly, uint32_t fs_seed, lig_proof** out) {
+    return prover_impl(cfg, poly, fs_seed, false, out);
+}
+int lig_prover_dev(lig_prover_config* cfg, const void* d_poly, uint32_t fs_seed, lig_proof** out) {
+    return prover_impl(cfg, d_poly, fs_seed, true, out);
+}
+
+int lig_proof_serialize(const lig_proof* p, uint8_t* buf, size_t cap, size_t* len) {
+    return guarded([&] {
+        LIG_REQUIRE(p && len, LIG_E_INVALID, "NULL argument");
+        std::vector<uint8_t> b = p->bits == 32 ? serialize<H32>(p->p32) : serialize<H128>(p->p128);
+        *len = b.size();
+        if (!buf) return LIG_OK;
+        LIG_REQUIRE(cap >= b.size(), LIG_E_CAPACITY, "buffer too small");
+        memcpy(buf, b.data(), b.size());
+        return LIG_OK;
+    });
+}
+
+int lig_proof_deserialize(int field_bits, const uint8_t* buf, size_t len, lig_proof** out) {
+    return guarded([&] {
+        LIG_REQUIRE(buf && out, LIG_E_INVALID, "NULL argument");
+        check_bits(field_bits);
+        std::unique_ptr<lig_proof> p(new lig_proof());
+        p->bits = field_bits;
+        bool ok = field_bits == 32 ? deserialize<H32>(buf, len, p->p32) : deserialize<H128>(buf, len, p->p128);
+        LIG_REQUIRE(ok, LIG_E_MALFORMED, "malformed proof bytes");
+        *out = p.release();
+        return LIG_OK;
+    });
+}
+
+size_t lig_proof_size(const lig_proof* p) { return !p ? 0 : p->bits == 32 ? proof_size<H32>(p->p32) : proof_size<H128>(p->p128); }
+void lig_proof_free(lig_proof* p) { delete p; }
+
+int lig_verifier(int field_bits, int log_n, const lig_proof* p, uint32_t fs_seed) {
+    return guarded([&] {
+        LIG_REQUIRE(p, LIG_E_INVALID, "NULL proof");
+        LIG_REQUIRE(p->bits == field_bits, LIG_E_INVALID, "proof field does not match field_bits");
+        return field_bits == 32 ? verify<H32>(log_n, p->p32, fs_seed) : verify<H128>(log_n, p->p128, fs_seed);
+    });
+}
+
+}  // extern "C"

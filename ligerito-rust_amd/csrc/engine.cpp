@@ -1,0 +1,432 @@
+// Device engine: plans, commit, GPU prover.
+#include "engine.hpp"
+
+#include <algorithm>
+
+namespace lig {
+
+// ------------------------------------------------------------------- pool --
+void* Pool::get(size_t bytes) {
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = free_.find(bytes);
+        if (it != free_.end()) {
+            void* p = it->second;
+            free_.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        drain();  // retry once with the cache returned to the device
+        LIG_HIP(hipMalloc(&p, bytes));
+    }
+    return p;
+}
+
+void Pool::put(void* p, size_t bytes) {
+    std::lock_guard<std::mutex> g(mu);
+    free_.emplace(bytes, p);
+}
+
+void Pool::drain() {
+    std::lock_guard<std::mutex> g(mu);
+    for (auto& kv : free_) (void)hipFree(kv.second);
+    free_.clear();
+}
+
+const char* phase_name(int p) {
+    static const char* names[PH_COUNT] = {"encode0", "encode", "merkle", "fold", "induce", "sumcheck", "open"};
+    return (p >= 0 && p < PH_COUNT) ? names[p] : "";
+}
+
+void Ctx::collect() {
+    if (marks.empty()) return;
+    sync();
+    for (auto& mk : marks) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, mk.a, mk.b) == hipSuccess) {
+            phase_ms[mk.phase] += ms;
+            phase_n[mk.phase] += 1;
+        }
+        (void)hipEventDestroy(mk.a);
+        (void)hipEventDestroy(mk.b);
+    }
+    marks.clear();
+}
+
+// ---------------------------------------------------------------- RS plan --
+template <class H> void RsPlan<H>::init(Ctx* c, size_t msg_len, size_t block_len) {
+    using F = typename DevField<H>::F;
+    LIG_REQUIRE(is_pow2(msg_len) && is_pow2(block_len) && msg_len < block_len, LIG_E_INVALID,
+                "ReedSolomonEncoding::new: lengths must be powers of two with message < block");
+    ctx = c;
+    log_msg = ilog2(msg_len);
+    log_block = ilog2(block_len);
+    tw = compute_twiddles<H>(log_block);
+    Vec<H> sks = eval_sk_at_vks<H>(msg_len);
+    pis = compute_pis<H>(msg_len, sks);
+    hipStream_t s = ctx->stream;
+    const size_t M = block_len, m = msg_len;
+    d_tw.alloc(M - 1);
+    LIG_HIP(hipMemcpyAsync(d_tw.p, tw.data(), (M - 1) * sizeof(T), hipMemcpyHostToDevice, s));
+    d_tw_tabs.alloc((M - 1) * F::TAB);
+    launch_build_tables<F>(d_tw.p, M - 1, d_tw_tabs.p, s);
+    d_pis.alloc(m);
+    LIG_HIP(hipMemcpyAsync(d_pis.p, pis.data(), m * sizeof(T), hipMemcpyHostToDevice, s));
+    size_t nlo = std::min<size_t>(m, 64), nhi = std::max<size_t>(m / 64, 1);
+    d_pis_lo.alloc(nlo * F::TAB);
+    launch_build_tables<F>(d_pis.p, nlo, d_pis_lo.p, s);
+    Vec<H> hi(nhi);
+    for (size_t h = 0; h < nhi; h++) hi[h] = pis[h * 64 < m ? h * 64 : 0];
+    DevBuf<T> d_hi(nhi);
+    LIG_HIP(hipMemcpyAsync(d_hi.p, hi.data(), nhi * sizeof(T), hipMemcpyHostToDevice, s));
+    d_pis_hi.alloc(nhi * F::TAB);
+    launch_build_tables<F>(d_hi.p, nhi, d_pis_hi.p, s);
+    LIG_HIP(hipGetLastError());
+    ctx->sync();
+}
+
+// reed_solomon.rs:104-113 over n columns: pis scaling, then the full FFT
+// (including the first two stages, which are replication only when the
+// tail beyond m is zero).
+template <class H> void RsPlan<H>::encode_non_systematic_cols(T* d_cols, size_t n) {
+    using F = typename DevField<H>::F;
+    hipStream_t s = ctx->stream;
+    const size_t M_ = M(), m_ = m();
+    // column-major -> row-major [M][n]
+    PBuf<T> rm(&ctx->pool, M_ * n);
+    std::vector<T> host(M_ * n), rmh(M_ * n);
+    LIG_HIP(hipMemcpyAsync(host.data(), d_cols, M_ * n * sizeof(T), hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    for (size_t j = 0; j < n; j++)
+        for (size_t i = 0; i < M_; i++) rmh[i * n + j] = host[j * M_ + i];
+    LIG_HIP(hipMemcpyAsync(rm.p, rmh.data(), M_ * n * sizeof(T), hipMemcpyHostToDevice, s));
+    launch_scale_rows<F>(rm.p, m_, n, d_pis.p, s);
+    // two leading stages with the general butterfly, then the quarter passes
+    int stages = log_block < 2 ? log_block : 2;
+    for (int st = 0; st < stages; st++) launch_fft_stage<F>(rm.p, M_, n, st, d_tw.p, 0, s);
+    if (log_block >= 2) launch_fft_quarters<F>(nullptr, rm.p, M_ / 4, n, d_tw_tabs.p, s);
+    LIG_HIP(hipGetLastError());
+    LIG_HIP(hipMemcpyAsync(rmh.data(), rm.p, M_ * n * sizeof(T), hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    for (size_t j = 0; j < n; j++)
+        for (size_t i = 0; i < M_; i++) host[j * M_ + i] = rmh[i * n + j];
+    LIG_HIP(hipMemcpyAsync(d_cols, host.data(), M_ * n * sizeof(T), hipMemcpyHostToDevice, s));
+    ctx->sync();
+}
+
+// reed_solomon.rs:19-41
+template <class H> static Vec<H> short_from_long(const Vec<H>& l, int n, int k) {
+    Vec<H> s(((size_t)1 << k) - 1);
+    size_t jump = (size_t)1 << (n - k);
+    s[0] = l[jump - 1];
+    size_t idx = 2;
+    for (int i = 1; i < k; i++) {
+        jump *= 2;
+        size_t take = (size_t)1 << i;
+        for (size_t j = 0; j < take; j++) s[idx + j - 1] = l[jump + j - 1];
+        idx += take;
+    }
+    return s;
+}
+
+// reed_solomon.rs:90-102: IFFT of the message with the short twiddles, then
+// the FFT of the zero-padded block.
+template <class H> void RsPlan<H>::encode_systematic(const T* msg, T* out) {
+    using F = typename DevField<H>::F;
+    hipStream_t s = ctx->stream;
+    const size_t M_ = M(), m_ = m();
+    Vec<H> stw = short_from_long<H>(tw, log_block, log_msg);
+    PBuf<T> buf(&ctx->pool, M_), dstw(&ctx->pool, std::max<size_t>(stw.size(), 1));
+    LIG_HIP(hipMemsetAsync(buf.p, 0, M_ * sizeof(T), s));
+    LIG_HIP(hipMemcpyAsync(buf.p, msg, m_ * sizeof(T), hipMemcpyHostToDevice, s));
+    if (!stw.empty()) LIG_HIP(hipMemcpyAsync(dstw.p, stw.data(), stw.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    // binary_fft.rs:161-178: the recursive IFFT applies the deepest stage first
+    for (int st = log_msg - 1; st >= 0; st--) launch_fft_stage<F>(buf.p, m_, 1, st, dstw.p, 1, s);
+    for (int st = 0; st < log_block; st++) launch_fft_stage<F>(buf.p, M_, 1, st, d_tw.p, 0, s);
+    LIG_HIP(hipGetLastError());
+    LIG_HIP(hipMemcpyAsync(out, buf.p, M_ * sizeof(T), hipMemcpyDeviceToHost, s));
+    ctx->sync();
+}
+
+// ----------------------------------------------------------------- commit --
+template <class H>
+void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witness<H>& w, int encode_phase) {
+    using F = typename DevField<H>::F;
+    using T = typename H::T;
+    LIG_REQUIRE(rs.m() == m && rs.M() == 4 * m, LIG_E_INVALID, "ligero_commit: plan must encode m -> 4m");
+    LIG_REQUIRE(n >= 1, LIG_E_INVALID, "ligero_commit: n >= 1");
+    LIG_REQUIRE(n * sizeof(T) <= 256 * 1024, LIG_E_INVALID, "ligero_commit: leaves above 256 KiB are not supported");
+    Ctx* ctx = rs.ctx;
+    hipStream_t s = ctx->stream;
+    const size_t M = 4 * m;
+    w.ctx = ctx;
+    w.m = m;
+    w.n = n;
+    w.M = M;
+    {
+        PBuf<T> msg(&ctx->pool, m * n);
+        w.mat = PBuf<T>(&ctx->pool, M * n);
+        PhaseTimer pt(ctx, encode_phase);
+        launch_transpose_scale<F>(d_poly, m, n, rs.d_pis_lo.p, rs.d_pis_hi.p, msg.p, s);
+        launch_fft_quarters<F>(msg.p, w.mat.p, m, n, rs.d_tw_tabs.p, s);
+        // msg returns to the pool; stream order keeps the FFT's reads first
+    }
+    w.tree = PBuf<uint32_t>(&ctx->pool, (2 * M - 1) * 8);
+    {
+        PhaseTimer pt(ctx, PH_MERKLE);
+        launch_leaf_hash<F>(w.mat.p, M, n, w.tree.p, s);
+        launch_merkle_levels(w.tree.p, M, s);
+    }
+    LIG_HIP(hipGetLastError());
+    LIG_HIP(hipMemcpyAsync(w.root, w.tree.p + (2 * M - 2) * 8, 32, hipMemcpyDeviceToHost, s));
+    ctx->sync();
+}
+
+// ----------------------------------------------------------------- config --
+template <class H> void Config<H>::init(Ctx* c, int L) {
+    LIG_REQUIRE(L >= 16 && L <= 27, LIG_E_INVALID, "prover config: log_n must be in [16, 27]");
+    ctx = c;
+    log_n = L;
+    recursive_steps = config_steps(L);
+    initial_k = 6;
+    initial_m = (size_t)1 << (L - 6);
+    initial_n = 64;
+    dims_m.clear();
+    dims_n.clear();
+    ks.clear();
+    for (int i = 0; i < recursive_steps; i++) {
+        dims_m.push_back((size_t)1 << (L - 6 - 4 * (i + 1)));
+        dims_n.push_back(16);
+        ks.push_back(4);
+    }
+    initial_rs.reset(new RsPlan<H>());
+    initial_rs->init(c, initial_m, 4 * initial_m);
+    rs.clear();
+    for (int i = 0; i < recursive_steps; i++) {
+        rs.emplace_back(new RsPlan<H>());
+        rs.back()->init(c, dims_m[i], 4 * dims_m[i]);
+    }
+}
+
+// ----------------------------------------------------------------- prover --
+namespace {
+
+template <class H> typename H::T acc_to_elem(const typename DevField<H>::F::Acc& a);
+template <> uint32_t acc_to_elem<H32>(const uint64_t& a) { return gf32_reduce64(a); }
+template <> u128v acc_to_elem<H128>(const u128v& a) { return a; }
+
+// Gathers opened rows and the batched Merkle proof of a witness
+// (prover.rs:72-76 / 114-125, merkle_tree.rs:98-111).
+template <class H> void open_rows(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out) {
+    using F = typename DevField<H>::F;
+    using T = typename H::T;
+    Ctx* ctx = w.ctx;
+    hipStream_t s = ctx->stream;
+    std::vector<uint64_t> nodes = merkle_proof_nodes(w.M, q);
+    std::vector<uint64_t> qi(q.begin(), q.end());
+    PBuf<uint64_t> d_idx(&ctx->pool, qi.size() + nodes.size());
+    LIG_HIP(hipMemcpyAsync(d_idx.p, qi.data(), qi.size() * 8, hipMemcpyHostToDevice, s));
+    if (!nodes.empty())
+        LIG_HIP(hipMemcpyAsync(d_idx.p + qi.size(), nodes.data(), nodes.size() * 8, hipMemcpyHostToDevice, s));
+    PBuf<T> d_rows(&ctx->pool, q.size() * w.n);
+    PBuf<uint32_t> d_nodes(&ctx->pool, std::max<size_t>(nodes.size(), 1) * 8);
+    {
+        PhaseTimer pt(ctx, PH_OPEN);
+        launch_gather_rows<F>(w.mat.p, w.n, d_idx.p, q.size(), d_rows.p, s);
+        launch_gather_nodes(w.tree.p, d_idx.p + qi.size(), nodes.size(), d_nodes.p, s);
+    }
+    out.n_rows = q.size();
+    out.n_cols = w.n;
+    out.rows.resize(q.size() * w.n);
+    out.nodes.resize(nodes.size() * 32);
+    LIG_HIP(hipMemcpyAsync(out.rows.data(), d_rows.p, out.rows.size() * sizeof(T), hipMemcpyDeviceToHost, s));
+    if (!nodes.empty())
+        LIG_HIP(hipMemcpyAsync(out.nodes.data(), d_nodes.p, out.nodes.size(), hipMemcpyDeviceToHost, s));
+    ctx->sync();
+}
+
+// induce_sumcheck_poly (sumcheck_polys.rs:74-160): basis on the device,
+// enforced_sum on the host from the opened rows.
+template <class H>
+void induce(Ctx* ctx, int ell, const Rlp<H>& rows, const Vec<H>& v_ch, const std::vector<size_t>& q, typename H::T alpha,
+            PBuf<typename H::T>& d_basis, typename H::T& enforced) {
+    using F = typename DevField<H>::F;
+    using T = typename H::T;
+    hipStream_t s = ctx->stream;
+    const size_t nq = q.size();
+    LIG_REQUIRE(((size_t)1 << v_ch.size()) == rows.n_cols, LIG_E_ASSERT, "opened row length != 2^k");
+    Vec<H> gr = lagrange_basis<H>(v_ch);
+    Vec<H> sks = eval_sk_at_vks<H>((size_t)1 << ell);
+    Vec<H> cvals(nq * ell), ap(nq);
+    enforced = H::zero();
+    T a = H::one();
+    for (size_t i = 0; i < nq; i++) {
+        Vec<H> sx = sks_at<H>(H::from_int(q[i]), sks, ell);
+        for (int t = 0; t < ell; t++) cvals[i * ell + t] = sx[t];
+        ap[i] = a;
+        T dot = H::zero();
+        for (size_t j = 0; j < rows.n_cols; j++) dot ^= H::mul(rows.rows[i * rows.n_cols + j], gr[j]);
+        enforced ^= H::mul(dot, a);
+        a = H::mul(a, alpha);
+    }
+    int clo = ell - 8;
+    if (clo < 8) clo = ell < 8 ? ell : 8;
+    if (clo > 10) clo = 10;
+    PBuf<T> d_c(&ctx->pool, nq * ell), d_ap(&ctx->pool, nq);
+    PBuf<T> d_P(&ctx->pool, nq << clo), d_K(&ctx->pool, nq << (ell - clo));
+    LIG_HIP(hipMemcpyAsync(d_c.p, cvals.data(), cvals.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    LIG_HIP(hipMemcpyAsync(d_ap.p, ap.data(), ap.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    d_basis = PBuf<T>(&ctx->pool, (size_t)1 << ell);
+    {
+        PhaseTimer pt(ctx, PH_INDUCE);
+        launch_induce<F>(d_c.p, d_ap.p, (int)nq, ell, d_P.p, d_K.p, d_basis.p, s);
+    }
+    LIG_HIP(hipGetLastError());
+    ctx->sync();  // host vectors above go out of scope
+}
+
+template <class H> void read_triple(Ctx* ctx, typename DevField<H>::F::Acc* d_res, typename H::T t[3]) {
+    typename DevField<H>::F::Acc r[3];
+    LIG_HIP(hipMemcpyAsync(r, d_res, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    for (int i = 0; i < 3; i++) t[i] = acc_to_elem<H>(r[i]);
+}
+
+}  // namespace
+
+template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<H>& proof) {
+    using F = typename DevField<H>::F;
+    using T = typename H::T;
+    using A = typename F::Acc;
+    Ctx* ctx = cfg.ctx;
+    hipStream_t s = ctx->stream;
+    const size_t S = 148;  // prover.rs:30
+    Transcript fs(fs_seed);
+
+    proof = Proof<H>();
+    Witness<H> w0;
+    commit<H>(*cfg.initial_rs, d_poly, cfg.initial_m, cfg.initial_n, w0, PH_ENCODE0);
+    memcpy(proof.initial_root.data(), w0.root, 32);
+    fs.absorb(w0.root, 32);
+
+    Vec<H> pe0(cfg.initial_k);
+    for (auto& r : pe0) r = fs.get_field<H>();
+
+    // partial_eval over the top initial_k variables == row combination with gr
+    const size_t N = cfg.initial_m * cfg.initial_n;
+    size_t flen = N >> cfg.initial_k;
+    PBuf<T> d_f(&ctx->pool, flen), d_f2(&ctx->pool, flen / 2);
+    {
+        Vec<H> gr = lagrange_basis<H>(pe0);
+        PBuf<T> d_gr(&ctx->pool, gr.size()), d_grt(&ctx->pool, gr.size() * F::TAB);
+        LIG_HIP(hipMemcpyAsync(d_gr.p, gr.data(), gr.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        PhaseTimer pt(ctx, PH_FOLD);
+        launch_build_tables<F>(d_gr.p, gr.size(), d_grt.p, s);
+        launch_fold<F>(d_poly, flen, cfg.initial_k, d_grt.p, d_f.p, s);
+        LIG_HIP(hipGetLastError());
+        ctx->sync();
+    }
+
+    Witness<H> wprev;
+    commit<H>(*cfg.rs[0], d_f.p, cfg.dims_m[0], cfg.dims_n[0], wprev);
+    proof.rec_roots.emplace_back();
+    memcpy(proof.rec_roots.back().data(), wprev.root, 32);
+    fs.absorb(wprev.root, 32);
+
+    std::vector<size_t> q = fs.get_distinct_queries(w0.M, S);
+    T alpha = fs.get_field<H>();
+    int nv = ilog2(flen);
+    open_rows<H>(w0, q, proof.initial);
+    PBuf<T> d_b;
+    T enforced;
+    induce<H>(ctx, nv, proof.initial, pe0, q, alpha, d_b, enforced);
+    w0 = Witness<H>();  // release the first witness
+
+    PBuf<A> d_res(&ctx->pool, 3);
+    PBuf<T> d_b2(&ctx->pool, flen / 2);
+    T t[3];
+    {
+        PhaseTimer pt(ctx, PH_SUMCHECK);
+        launch_eval013<F>(d_f.p, d_b.p, flen, d_res.p, s);
+    }
+    read_triple<H>(ctx, d_res.p, t);
+    LIG_REQUIRE(H::eq(t[0] ^ t[1], enforced), LIG_E_ASSERT, "sumcheck: s0 + s1 != enforced sum (ligerito.rs:35)");
+    proof.tr.insert(proof.tr.end(), t, t + 3);
+    fs.absorb_elems<H>(t, 3);
+
+    for (int i = 0; i < cfg.recursive_steps; i++) {
+        Vec<H> rs;
+        for (int j = 0; j < cfg.ks[i]; j++) {
+            T r = fs.get_field<H>();
+            {
+                PhaseTimer pt(ctx, PH_SUMCHECK);
+                launch_sumcheck_fold<F>(d_f.p, d_b.p, flen, r, d_f2.p, d_b2.p, d_res.p, s);
+            }
+            LIG_HIP(hipGetLastError());
+            read_triple<H>(ctx, d_res.p, t);
+            std::swap(d_f, d_f2);
+            std::swap(d_b, d_b2);
+            flen /= 2;
+            proof.tr.insert(proof.tr.end(), t, t + 3);
+            fs.absorb_elems<H>(t, 3);
+            rs.push_back(r);
+        }
+        if (i == cfg.recursive_steps - 1) {
+            proof.yr.resize(flen);
+            LIG_HIP(hipMemcpyAsync(proof.yr.data(), d_f.p, flen * sizeof(T), hipMemcpyDeviceToHost, s));
+            ctx->sync();
+            fs.absorb_elems<H>(proof.yr.data(), flen);
+            q = fs.get_distinct_queries(wprev.M, S);
+            open_rows<H>(wprev, q, proof.final_);
+            return;
+        }
+        Witness<H> wnext;
+        commit<H>(*cfg.rs[i + 1], d_f.p, cfg.dims_m[i + 1], cfg.dims_n[i + 1], wnext);
+        proof.rec_roots.emplace_back();
+        memcpy(proof.rec_roots.back().data(), wnext.root, 32);
+        fs.absorb(wnext.root, 32);
+
+        q = fs.get_distinct_queries(wprev.M, S);
+        T a = fs.get_field<H>();
+        nv = ilog2(flen);
+        proof.rec_proofs.emplace_back();
+        open_rows<H>(wprev, q, proof.rec_proofs.back());
+        PBuf<T> d_bnew;
+        T enf;
+        induce<H>(ctx, nv, proof.rec_proofs.back(), rs, q, a, d_bnew, enf);
+        // introduce_new (ligerito.rs:50-57): triple of f with the new basis alone
+        {
+            PhaseTimer pt(ctx, PH_SUMCHECK);
+            launch_eval013<F>(d_f.p, d_bnew.p, flen, d_res.p, s);
+        }
+        read_triple<H>(ctx, d_res.p, t);
+        LIG_REQUIRE(H::eq(t[0] ^ t[1], enf), LIG_E_ASSERT, "sumcheck: s0 + s1 != h (ligerito.rs:52)");
+        proof.tr.insert(proof.tr.end(), t, t + 3);
+        fs.absorb_elems<H>(t, 3);
+        // glue (ligerito.rs:59-67): the summed basis absorbs beta * b_new;
+        // folds and the 0/1/3 evaluations are linear in the basis polys.
+        T beta = fs.get_field<H>();
+        {
+            PhaseTimer pt(ctx, PH_SUMCHECK);
+            launch_glue<F>(d_b.p, d_bnew.p, flen, beta, s);
+        }
+        LIG_HIP(hipGetLastError());
+        wprev = std::move(wnext);
+    }
+    throw Error(LIG_E_ASSERT, "Prover completed without entering final step (prover.rs:198)");
+}
+
+template struct RsPlan<H32>;
+template struct RsPlan<H128>;
+template struct Config<H32>;
+template struct Config<H128>;
+template void commit<H32>(RsPlan<H32>&, const uint32_t*, size_t, size_t, Witness<H32>&, int);
+template void commit<H128>(RsPlan<H128>&, const u128v*, size_t, size_t, Witness<H128>&, int);
+template void prove<H32>(Config<H32>&, const uint32_t*, uint32_t, Proof<H32>&);
+template void prove<H128>(Config<H128>&, const u128v*, uint32_t, Proof<H128>&);
+
+}  // namespace lig

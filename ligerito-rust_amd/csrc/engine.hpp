@@ -1,0 +1,172 @@
+// Device engine of the Ligerito library: context with a caching device
+// allocator, Reed–Solomon plans, device-resident witnesses, the GPU prover
+// and the proof object.
+#pragma once
+#include <array>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "hostmath.hpp"
+#include "kernels.hpp"
+
+namespace lig {
+
+// Device-memory cache: freed blocks are kept per exact size and reused, so
+// repeated proofs of one configuration allocate nothing after the first.
+struct Pool {
+    std::multimap<size_t, void*> free_;
+    std::mutex mu;
+    void* get(size_t bytes);
+    void put(void* p, size_t bytes);
+    void drain();
+    ~Pool() { drain(); }
+};
+
+// Phases timed with HIP events when profiling is enabled.
+enum Phase { PH_ENCODE0 = 0, PH_ENCODE, PH_MERKLE, PH_FOLD, PH_INDUCE, PH_SUMCHECK, PH_OPEN, PH_COUNT };
+const char* phase_name(int p);
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Pool pool;
+    bool profiling = false;
+    struct Mark {
+        int phase;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
+    double phase_ms[PH_COUNT] = {};
+    uint64_t phase_n[PH_COUNT] = {};
+    void sync() { LIG_HIP(hipStreamSynchronize(stream)); }
+    // Resolves recorded event pairs into per-phase totals.
+    void collect();
+};
+
+// Records a HIP event pair on the context stream around a scope.
+struct PhaseTimer {
+    Ctx* ctx;
+    int phase;
+    hipEvent_t a = nullptr, b = nullptr;
+    PhaseTimer(Ctx* c, int p) : ctx(c), phase(p) {
+        if (!ctx->profiling) return;
+        LIG_HIP(hipEventCreate(&a));
+        LIG_HIP(hipEventCreate(&b));
+        LIG_HIP(hipEventRecord(a, ctx->stream));
+    }
+    ~PhaseTimer() {
+        if (!a) return;
+        if (hipEventRecord(b, ctx->stream) == hipSuccess) ctx->marks.push_back({phase, a, b});
+    }
+};
+
+// Pool-backed device buffer.
+template <class T> struct PBuf {
+    Pool* pool = nullptr;
+    T* p = nullptr;
+    size_t n = 0;
+    PBuf() = default;
+    PBuf(Pool* pl, size_t count) : pool(pl), n(count) {
+        p = count ? (T*)pool->get(count * sizeof(T)) : nullptr;
+    }
+    PBuf(const PBuf&) = delete;
+    PBuf& operator=(const PBuf&) = delete;
+    PBuf(PBuf&& o) noexcept : pool(o.pool), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    PBuf& operator=(PBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            pool = o.pool; p = o.p; n = o.n;
+            o.p = nullptr; o.n = 0;
+        }
+        return *this;
+    }
+    ~PBuf() { release(); }
+    void release() {
+        if (p) pool->put(p, n * sizeof(T));
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// Host field H paired with its device traits.
+template <class H> struct DevField;
+template <> struct DevField<H32> { using F = F32; };
+template <> struct DevField<H128> { using F = F128; };
+
+// ReedSolomonEncoding<F> (reed_solomon.rs:7-64) with device tables.
+template <class H> struct RsPlan {
+    using T = typename H::T;
+    Ctx* ctx = nullptr;
+    int log_msg = 0, log_block = 0;
+    Vec<H> tw, pis;
+    DevBuf<T> d_tw, d_tw_tabs, d_pis, d_pis_lo, d_pis_hi;
+    size_t m() const { return (size_t)1 << log_msg; }
+    size_t M() const { return (size_t)1 << log_block; }
+    void init(Ctx* c, size_t msg_len, size_t block_len);
+    // encode_non_systematic on n columns, column-major, in place (device).
+    void encode_non_systematic_cols(T* d_cols, size_t n);
+    // encode (systematic) on one host message.
+    void encode_systematic(const T* msg, T* out);
+};
+
+// RecursiveLigeroWitness<F> (data_structures.rs:32-40), device-resident:
+// the encoded matrix row-major [M][n] and the Merkle tree, all layers back
+// to back.
+template <class H> struct Witness {
+    using T = typename H::T;
+    Ctx* ctx = nullptr;
+    size_t m = 0, n = 0, M = 0;
+    PBuf<T> mat;
+    PBuf<uint32_t> tree;
+    uint8_t root[32];
+};
+
+// ligero_commit (ligero.rs:70-95) on a device poly (column-major m x n).
+template <class H> void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witness<H>& w, int encode_phase = PH_ENCODE);
+
+// ProverConfig<F> (data_structures.rs:9-21).
+template <class H> struct Config {
+    Ctx* ctx = nullptr;
+    int log_n = 0;
+    int recursive_steps = 0;
+    size_t initial_m = 0, initial_n = 0;
+    int initial_k = 0;
+    std::vector<size_t> dims_m, dims_n;
+    std::vector<int> ks;
+    std::unique_ptr<RsPlan<H>> initial_rs;
+    std::vector<std::unique_ptr<RsPlan<H>>> rs;
+    void init(Ctx* c, int L);
+};
+
+// RecursiveLigeroProof / FinalLigeroProof parts (data_structures.rs:42-66).
+template <class H> struct Rlp {
+    size_t n_rows = 0, n_cols = 0;
+    std::vector<typename H::T> rows;
+    std::vector<uint8_t> nodes;  // 32 bytes each
+};
+
+// FinalizedLigeritoProof<F> (data_structures.rs:91-103).
+template <class H> struct Proof {
+    std::array<uint8_t, 32> initial_root;
+    Rlp<H> initial;
+    std::vector<std::array<uint8_t, 32>> rec_roots;
+    std::vector<Rlp<H>> rec_proofs;
+    std::vector<typename H::T> yr;
+    Rlp<H> final_;
+    std::vector<typename H::T> tr;  // 3 per round
+};
+
+// prover (prover.rs:15-199) on a device-resident poly.
+template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<H>& out);
+// verifier (verifier.rs:10-164) on the host; 1 accept, 0 reject, <0 panic sites.
+template <class H> int verify(int L, const Proof<H>& p, uint32_t fs_seed);
+template <class H> std::vector<uint8_t> serialize(const Proof<H>& p);
+template <class H> bool deserialize(const uint8_t* b, size_t len, Proof<H>& p);
+template <class H> size_t proof_size(const Proof<H>& p);
+
+// Generated configuration shape (config.rs:6-134 for L = 20, 24).
+inline int config_steps(int L) { return L <= 23 ? 1 : 2 + (L - 24) / 4; }
+
+}  // namespace lig

@@ -1,0 +1,234 @@
+// Host utilities: error strings, SHA-256, Fiat–Shamir transcript, BLAKE3 over
+// byte strings, Merkle proof/verify logic.
+#include <algorithm>
+#include <array>
+#include <unordered_set>
+
+#include "host.hpp"
+
+namespace lig {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+// ----------------------------------------------------------------- SHA-256 --
+// FIPS 180-4.
+namespace {
+const uint32_t kSha[64] = {
+    0x428a2f98u, 0x71374491u, 0xb5c0fbcfu, 0xe9b5dba5u, 0x3956c25bu, 0x59f111f1u, 0x923f82a4u, 0xab1c5ed5u,
+    0xd807aa98u, 0x12835b01u, 0x243185beu, 0x550c7dc3u, 0x72be5d74u, 0x80deb1feu, 0x9bdc06a7u, 0xc19bf174u,
+    0xe49b69c1u, 0xefbe4786u, 0x0fc19dc6u, 0x240ca1ccu, 0x2de92c6fu, 0x4a7484aau, 0x5cb0a9dcu, 0x76f988dau,
+    0x983e5152u, 0xa831c66du, 0xb00327c8u, 0xbf597fc7u, 0xc6e00bf3u, 0xd5a79147u, 0x06ca6351u, 0x14292967u,
+    0x27b70a85u, 0x2e1b2138u, 0x4d2c6dfcu, 0x53380d13u, 0x650a7354u, 0x766a0abbu, 0x81c2c92eu, 0x92722c85u,
+    0xa2bfe8a1u, 0xa81a664bu, 0xc24b8b70u, 0xc76c51a3u, 0xd192e819u, 0xd6990624u, 0xf40e3585u, 0x106aa070u,
+    0x19a4c116u, 0x1e376c08u, 0x2748774cu, 0x34b0bcb5u, 0x391c0cb3u, 0x4ed8aa4au, 0x5b9cca4fu, 0x682e6ff3u,
+    0x748f82eeu, 0x78a5636fu, 0x84c87814u, 0x8cc70208u, 0x90befffau, 0xa4506cebu, 0xbef9a3f7u, 0xc67178f2u};
+
+inline uint32_t ror(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+void sha_block(uint32_t st[8], const uint8_t* p) {
+    uint32_t w[64];
+    for (int i = 0; i < 16; i++) w[i] = (uint32_t)p[4 * i] << 24 | (uint32_t)p[4 * i + 1] << 16 | (uint32_t)p[4 * i + 2] << 8 | p[4 * i + 3];
+    for (int i = 16; i < 64; i++)
+        w[i] = w[i - 16] + (ror(w[i - 15], 7) ^ ror(w[i - 15], 18) ^ (w[i - 15] >> 3)) + w[i - 7] +
+               (ror(w[i - 2], 17) ^ ror(w[i - 2], 19) ^ (w[i - 2] >> 10));
+    uint32_t v[8];
+    memcpy(v, st, 32);
+    for (int i = 0; i < 64; i++) {
+        uint32_t t1 = v[7] + (ror(v[4], 6) ^ ror(v[4], 11) ^ ror(v[4], 25)) + ((v[4] & v[5]) ^ (~v[4] & v[6])) + kSha[i] + w[i];
+        uint32_t t2 = (ror(v[0], 2) ^ ror(v[0], 13) ^ ror(v[0], 22)) + ((v[0] & v[1]) ^ (v[0] & v[2]) ^ (v[1] & v[2]));
+        memmove(v + 1, v, 7 * sizeof(uint32_t));
+        v[4] += t1;
+        v[0] = t1 + t2;
+    }
+    for (int i = 0; i < 8; i++) st[i] += v[i];
+}
+}  // namespace
+
+Sha256::Sha256() {
+    static const uint32_t h0[8] = {0x6a09e667u, 0xbb67ae85u, 0x3c6ef372u, 0xa54ff53au,
+                                   0x510e527fu, 0x9b05688cu, 0x1f83d9abu, 0x5be0cd19u};
+    memcpy(h, h0, 32);
+}
+
+void Sha256::update(const void* data, size_t len) {
+    const uint8_t* p = (const uint8_t*)data;
+    total += len;
+    if (blen) {
+        size_t t = std::min<size_t>(64 - blen, len);
+        memcpy(buf + blen, p, t);
+        blen += (uint32_t)t;
+        p += t;
+        len -= t;
+        if (blen == 64) {
+            sha_block(h, buf);
+            blen = 0;
+        }
+    }
+    while (len >= 64) {
+        sha_block(h, p);
+        p += 64;
+        len -= 64;
+    }
+    if (len) {
+        memcpy(buf, p, len);
+        blen = (uint32_t)len;
+    }
+}
+
+void Sha256::digest(uint8_t out[32]) const {
+    Sha256 c = *this;
+    uint64_t bits = c.total * 8;
+    uint8_t pad[72] = {0x80};
+    size_t padlen = (c.blen < 56) ? 56 - c.blen : 120 - c.blen;
+    c.update(pad, padlen);
+    uint8_t lb[8];
+    for (int i = 0; i < 8; i++) lb[i] = (uint8_t)(bits >> (56 - 8 * i));
+    c.update(lb, 8);
+    for (int i = 0; i < 8; i++) {
+        out[4 * i] = (uint8_t)(c.h[i] >> 24);
+        out[4 * i + 1] = (uint8_t)(c.h[i] >> 16);
+        out[4 * i + 2] = (uint8_t)(c.h[i] >> 8);
+        out[4 * i + 3] = (uint8_t)c.h[i];
+    }
+}
+
+// ------------------------------------------------------------- transcript --
+Transcript::Transcript(uint32_t seed) {
+    uint8_t b[4];
+    memcpy(b, &seed, 4);  // to_le_bytes
+    hasher.update(b, 4);
+}
+
+void Transcript::squeeze(uint64_t* lo, uint64_t* hi) {
+    uint8_t b[4];
+    memcpy(b, &counter, 4);
+    hasher.update(b, 4);
+    counter++;
+    uint8_t d[32];
+    hasher.digest(d);
+    memcpy(lo, d, 8);
+    memcpy(hi, d + 8, 8);
+}
+
+size_t Transcript::get_query(size_t n) {
+    uint64_t lo, hi;
+    squeeze(&lo, &hi);
+    return (size_t)(lo % (uint64_t)n);  // (u128 as usize) % n on a 64-bit target
+}
+
+std::vector<size_t> Transcript::get_distinct_queries(size_t n, size_t s) {
+    LIG_REQUIRE(s <= n, LIG_E_INVALID, "get_distinct_queries: s > n never terminates");
+    std::vector<size_t> q;
+    q.reserve(s);
+    std::vector<uint8_t> seen((n + 7) / 8, 0);
+    while (q.size() < s) {
+        size_t v = get_query(n);
+        if (!(seen[v >> 3] & (1u << (v & 7)))) {
+            seen[v >> 3] |= (uint8_t)(1u << (v & 7));
+            q.push_back(v);
+        }
+    }
+    std::sort(q.begin(), q.end());
+    return q;
+}
+
+// ------------------------------------------------------------ BLAKE3 bytes --
+void blake3_bytes(const uint8_t* data, size_t len, uint8_t out[32]) {
+    b3::Hasher h;
+    h.init();
+    uint32_t m[16];
+    size_t nblocks = len ? (len + 63) / 64 : 1;
+    for (size_t b = 0; b + 1 < nblocks; b++) {
+        memcpy(m, data + 64 * b, 64);
+        h.block(m);
+    }
+    size_t last = len - 64 * (nblocks - 1);
+    memset(m, 0, 64);
+    if (last) memcpy(m, data + 64 * (nblocks - 1), last);
+    uint32_t o[8];
+    h.finish(m, (uint32_t)last, o);
+    memcpy(out, o, 32);
+}
+
+// ------------------------------------------------------------- Merkle host --
+std::vector<uint64_t> merkle_proof_nodes(size_t n_leaves, const std::vector<size_t>& sorted_queries) {
+    // merkle_tree.rs:62-111; depth = log2(n_leaves) layers below the root.
+    std::vector<uint64_t> out;
+    std::vector<size_t> q = sorted_queries;
+    size_t cnt = q.size();
+    size_t layer_off = 0, layer_n = n_leaves;
+    while (layer_n > 1) {
+        size_t next = 0, i = 0;
+        while (i < cnt) {
+            size_t query = q[i], sib = query ^ 1;
+            next++;
+            q[next - 1] = query >> 1;
+            if (i == cnt - 1) {
+                out.push_back(layer_off + sib);
+                break;
+            }
+            if (query % 2 != 0) {
+                out.push_back(layer_off + sib);
+                i += 1;
+            } else if (q[i + 1] != sib) {
+                out.push_back(layer_off + sib);
+                i += 1;
+            } else {
+                i += 2;
+            }
+        }
+        cnt = next;
+        layer_off += layer_n;
+        layer_n >>= 1;
+    }
+    return out;
+}
+
+static void hash_pair(const uint8_t* l, const uint8_t* r, uint8_t out[32]) {
+    uint32_t m[16], o[8];
+    memcpy(m, l, 32);
+    memcpy(m + 8, r, 32);
+    b3::hash64(m, o);
+    memcpy(out, o, 32);
+}
+
+bool merkle_verify_hashes(const uint8_t root[32], const uint8_t* proof, size_t n_proof, size_t depth,
+                          std::vector<std::array<uint8_t, 32>> layer, std::vector<size_t> q) {
+    size_t cnt = q.size(), pc = 0;
+    if (cnt == 0) return false;
+    for (size_t d = 0; d < depth; d++) {
+        size_t next = 0, i = 0;
+        while (i < cnt) {
+            size_t query = q[i], sib = query ^ 1;
+            next++;
+            q[next - 1] = query >> 1;
+            uint8_t h[32];
+            if (i == cnt - 1) {
+                if (++pc > n_proof) return false;
+                if (query % 2 != 0) hash_pair(proof + 32 * (pc - 1), layer[i].data(), h);
+                else hash_pair(layer[i].data(), proof + 32 * (pc - 1), h);
+                memcpy(layer[next - 1].data(), h, 32);
+                break;
+            }
+            if (query % 2 != 0) {
+                if (++pc > n_proof) return false;
+                hash_pair(proof + 32 * (pc - 1), layer[i].data(), h);
+                i += 1;
+            } else if (q[i + 1] != sib) {
+                if (++pc > n_proof) return false;
+                hash_pair(layer[i].data(), proof + 32 * (pc - 1), h);
+                i += 1;
+            } else {
+                hash_pair(layer[i].data(), layer[i + 1].data(), h);
+                i += 2;
+            }
+            memcpy(layer[next - 1].data(), h, 32);
+        }
+        cnt = next;
+    }
+    return cnt == 1 && memcmp(layer[0].data(), root, 32) == 0;
+}
+
+}  // namespace lig

@@ -1,0 +1,586 @@
+// HIP kernels for the Ligerito prover hot path on CDNA4 (gfx950).
+//
+// Data layout in HBM: an encoded matrix is row-major [4m][n] (a Merkle leaf
+// is one contiguous row), a Merkle tree is every layer back to back
+// (layer 0 = leaf hashes, 32 bytes per node), multilinear tables are plain
+// arrays.  All arithmetic is exact binary-field arithmetic.
+#include <hip/hip_runtime.h>
+
+#include "blake3.hpp"
+#include "fields.hpp"
+#include "kernels.hpp"
+
+namespace lig {
+
+template <> struct PassShape<F32> {
+    static constexpr int K = 7;   // 128 rows per tile
+    static constexpr int C = 64;  // 64 columns (one wave covers one row)
+};
+template <> struct PassShape<F128> {
+    static constexpr int K = 4;   // 16 rows per tile (8 KiB tables)
+    static constexpr int C = 64;
+};
+
+static inline unsigned grid_for(size_t work, unsigned block, unsigned cap = 65535u * 16u) {
+    size_t g = (work + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+// ------------------------------------------------------------------ tables --
+
+template <class F>
+__global__ void k_build_tables(const typename F::T* __restrict__ lams, size_t count, typename F::T* __restrict__ tabs) {
+    size_t total = count * (size_t)F::TAB;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        size_t i = idx / F::TAB;
+        int e = (int)(idx % F::TAB);
+        tabs[idx] = Ops<F>::table_entry(lams[i], e);
+    }
+}
+
+template <class F> void launch_build_tables(const typename F::T* lams, size_t count, typename F::T* tabs, hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_build_tables<F>, dim3(grid_for(count * F::TAB, 256)), dim3(256), 0, s, lams, count, tabs);
+}
+
+// --------------------------------------------------------- transpose+scale --
+// Tile of 64 message rows x 64 columns: coalesced column-major reads, LDS
+// transpose, pis scaling with two table multiplies, coalesced row writes.
+template <class F>
+__global__ __launch_bounds__(256) void k_transpose_scale(const typename F::T* __restrict__ poly, size_t m, size_t n,
+                                                         const typename F::T* __restrict__ lo_tabs,
+                                                         const typename F::T* __restrict__ hi_tabs,
+                                                         typename F::T* __restrict__ msg) {
+    using T = typename F::T;
+    __shared__ T tile[64][65];
+    const int rt = m < 64 ? (int)m : 64;
+    const int ct = n < 64 ? (int)n : 64;
+    const size_t row_tiles = (m + 63) / 64;
+    const size_t i0 = (blockIdx.x % row_tiles) * 64;
+    const size_t c0 = (blockIdx.x / row_tiles) * 64;
+    for (int idx = threadIdx.x; idx < rt * ct; idx += blockDim.x) {
+        int r = idx % rt, c = idx / rt;  // lanes walk down a column
+        tile[c][r] = poly[(c0 + c) * m + i0 + r];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < rt * ct; idx += blockDim.x) {
+        int c = idx % ct, r = idx / ct;  // lanes walk along a row
+        size_t i = i0 + r;
+        T x = tile[c][r];
+        x = Ops<F>::mul_tab(lo_tabs + (size_t)(i & 63) * F::TAB, x);
+        x = Ops<F>::mul_tab(hi_tabs + (size_t)(i >> 6) * F::TAB, x);
+        msg[i * n + c0 + c] = x;
+    }
+}
+
+template <class F>
+void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const typename F::T* lo_tabs,
+                            const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s) {
+    size_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
+    hipLaunchKernelGGL(k_transpose_scale<F>, dim3((unsigned)tiles), dim3(256), 0, s, poly, m, n, lo_tabs, hi_tabs, msg);
+}
+
+template <class F>
+__global__ void k_scale_rows(typename F::T* __restrict__ buf, size_t rows, size_t n, const typename F::T* __restrict__ pis) {
+    size_t total = rows * n;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x)
+        buf[idx] = Ops<F>::mul(buf[idx], pis[idx / n]);
+}
+
+template <class F> void launch_scale_rows(typename F::T* buf, size_t rows, size_t n, const typename F::T* pis, hipStream_t s) {
+    if (!rows) return;
+    hipLaunchKernelGGL(k_scale_rows<F>, dim3(grid_for(rows * n, 256)), dim3(256), 0, s, buf, rows, n, pis);
+}
+
+// binary_fft.rs:180-206, one stage over all blocks; twiddle index of block b
+// at stage `stage` is 2^stage - 1 + b (heap order, binary_fft.rs:127-139).
+template <class F>
+__global__ void k_fft_stage(typename F::T* __restrict__ buf, size_t N, size_t n, int stage, const typename F::T* __restrict__ tw,
+                            int inverse) {
+    using T = typename F::T;
+    size_t half = N >> (stage + 1);
+    size_t pairs = (N / 2) * n;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < pairs; idx += (size_t)gridDim.x * blockDim.x) {
+        size_t c = idx % n, p = idx / n;
+        size_t blk = p / half, off = p % half;
+        size_t ru = blk * 2 * half + off, rw = ru + half;
+        T lam = tw[((size_t)1 << stage) - 1 + blk];
+        T u = buf[ru * n + c], w = buf[rw * n + c];
+        if (!inverse) {
+            u ^= Ops<F>::mul(lam, w);
+            w ^= u;
+        } else {
+            w ^= u;
+            u ^= Ops<F>::mul(lam, w);
+        }
+        buf[ru * n + c] = u;
+        buf[rw * n + c] = w;
+    }
+}
+
+template <class F>
+void launch_fft_stage(typename F::T* buf, size_t N, size_t n, int stage, const typename F::T* tw, int inverse, hipStream_t s) {
+    hipLaunchKernelGGL(k_fft_stage<F>, dim3(grid_for((N / 2) * n, 256)), dim3(256), 0, s, buf, N, n, stage, tw, inverse);
+}
+
+// ------------------------------------------------------------- FFT passes --
+// One pass = k consecutive stages d0..d0+k-1 of a quarter's size-m FFT
+// (global stages d0+2..).  A tile holds the 2^k rows
+//   r(t) = B*2^(ell-d0) + t*S + lo,  S = 2^(ell-d0-k),
+// that the butterflies of those stages connect, for up to C columns.  At
+// tile stage j the block of row t is B*2^j + (t >> (k-j)); its twiddle is
+// twiddle index 2^s - 1 + q*2^d + B*2^j + t'.  Tables for the 2^j blocks of
+// a stage are staged into LDS; a wave covering one row pair reads a single
+// table (conflict-free 16-entry groups).
+template <class F, int K, int C>
+__global__ __launch_bounds__(256) void k_fft_pass(const typename F::T* __restrict__ src, typename F::T* __restrict__ dst,
+                                                  size_t m, size_t n, int ell, int d0, int k, int src_shared,
+                                                  const typename F::T* __restrict__ tw_tabs) {
+    using T = typename F::T;
+    __shared__ T tile[(1 << K) * C];
+    __shared__ T tabs[(1 << (K - 1)) * F::TAB_STRIDE];
+    const size_t q = blockIdx.y;
+    const int cc = n < (size_t)C ? (int)n : C;
+    const size_t nchunks = (n + C - 1) / C;
+    const size_t tile_id = blockIdx.x / nchunks;
+    const size_t c0 = (blockIdx.x % nchunks) * (size_t)C;
+    const int sh = ell - d0 - k;
+    const size_t S = (size_t)1 << sh;
+    const size_t B = tile_id >> sh, lo = tile_id & (S - 1);
+    const size_t rbase = (B << (ell - d0)) + lo;
+    const int R = 1 << k;
+
+    for (int idx = threadIdx.x; idx < R * cc; idx += blockDim.x) {
+        int t = idx / cc, c = idx % cc;
+        size_t r = rbase + (size_t)t * S;
+        size_t srow = src_shared ? r : q * m + r;
+        tile[t * C + c] = src[srow * n + c0 + c];
+    }
+    for (int j = 0; j < k; j++) {
+        const int d = d0 + j;
+        const int s = d + 2;
+        const int nb = 1 << j;
+        const size_t tw_base = (((size_t)1 << s) - 1) + (q << d) + (B << j);
+        for (int idx = threadIdx.x; idx < nb * F::TAB; idx += blockDim.x) {
+            int b = idx / F::TAB, e = idx % F::TAB;
+            tabs[b * F::TAB_STRIDE + e] = tw_tabs[(tw_base + b) * F::TAB + e];
+        }
+        __syncthreads();
+        const int hs = k - 1 - j;
+        const int h = 1 << hs;
+        for (int idx = threadIdx.x; idx < (R / 2) * cc; idx += blockDim.x) {
+            int p = idx / cc, c = idx % cc;
+            int blk = p >> hs;
+            int tu = (blk << (hs + 1)) + (p & (h - 1));
+            int tv = tu + h;
+            T u = tile[tu * C + c], w = tile[tv * C + c];
+            u ^= Ops<F>::mul_tab(&tabs[blk * F::TAB_STRIDE], w);
+            w ^= u;
+            tile[tu * C + c] = u;
+            tile[tv * C + c] = w;
+        }
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < R * cc; idx += blockDim.x) {
+        int t = idx / cc, c = idx % cc;
+        size_t r = rbase + (size_t)t * S;
+        dst[(q * m + r) * n + c0 + c] = tile[t * C + c];
+    }
+}
+
+template <class F>
+void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw_tabs,
+                         hipStream_t s) {
+    constexpr int K = PassShape<F>::K, C = PassShape<F>::C;
+    int ell = 0;
+    while (((size_t)1 << ell) < m) ell++;
+    int npass = ell == 0 ? 1 : (ell + K - 1) / K;
+    int d0 = 0;
+    const size_t nchunks = (n + C - 1) / C;
+    for (int p = 0; p < npass; p++) {
+        int k = ell / npass + (p < ell % npass ? 1 : 0);
+        size_t tiles = (size_t)1 << (ell - k);
+        dim3 grid((unsigned)(tiles * nchunks), 4);
+        const typename F::T* src = (p == 0 && msg) ? msg : mat;
+        int shared = (p == 0 && msg) ? 1 : 0;
+        hipLaunchKernelGGL((k_fft_pass<F, K, C>), grid, dim3(256), 0, s, src, mat, m, n, ell, d0, k, shared, tw_tabs);
+        d0 += k;
+    }
+}
+
+// ------------------------------------------------------------- leaf hash ---
+// One lane per leaf: BLAKE3 over the row's little-endian bytes.
+template <class F>
+__global__ void k_leaf_hash(const typename F::T* __restrict__ mat, size_t rows, size_t n, uint32_t* __restrict__ layer0) {
+    const size_t words = n * (sizeof(typename F::T) / 4);
+    const size_t len = words * 4;
+    const size_t nblocks = (len + 63) / 64;
+    for (size_t row = blockIdx.x * (size_t)blockDim.x + threadIdx.x; row < rows; row += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t* p = reinterpret_cast<const uint32_t*>(mat) + row * words;
+        uint32_t m[16];
+        uint32_t out[8];
+        const size_t lb = (nblocks - 1) * 16;
+        const uint32_t last_len = (uint32_t)(len - (nblocks - 1) * 64);
+        if (nblocks <= 16) {
+            // single chunk: chain the blocks, ROOT on the last one
+            uint32_t cv[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) cv[i] = b3::iv(i);
+            for (size_t b = 0; b + 1 < nblocks; b++) {
+                const uint4* q4 = reinterpret_cast<const uint4*>(p + b * 16);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    uint4 v = q4[i];
+                    m[4 * i] = v.x;
+                    m[4 * i + 1] = v.y;
+                    m[4 * i + 2] = v.z;
+                    m[4 * i + 3] = v.w;
+                }
+                b3::compress_cv(cv, m, 0, 64, b == 0 ? b3::CHUNK_START : 0u, cv);
+            }
+#pragma unroll
+            for (int i = 0; i < 16; i++) m[i] = (lb + i < words) ? p[lb + i] : 0u;
+            b3::compress_cv(cv, m, 0, last_len, b3::CHUNK_END | b3::ROOT | (nblocks == 1 ? b3::CHUNK_START : 0u), out);
+        } else {
+            b3::Hasher h;
+            h.init();
+            for (size_t b = 0; b + 1 < nblocks; b++) {
+                const uint4* q4 = reinterpret_cast<const uint4*>(p + b * 16);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    uint4 v = q4[i];
+                    m[4 * i] = v.x;
+                    m[4 * i + 1] = v.y;
+                    m[4 * i + 2] = v.z;
+                    m[4 * i + 3] = v.w;
+                }
+                h.block(m);
+            }
+#pragma unroll
+            for (int i = 0; i < 16; i++) m[i] = (lb + i < words) ? p[lb + i] : 0u;
+            h.finish(m, last_len, out);
+        }
+        uint32_t* o = layer0 + row * 8;
+#pragma unroll
+        for (int i = 0; i < 8; i++) o[i] = out[i];
+    }
+}
+
+template <class F> void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, hipStream_t s) {
+    hipLaunchKernelGGL(k_leaf_hash<F>, dim3(grid_for(rows, 128)), dim3(128), 0, s, mat, rows, n, layer0);
+}
+
+// ----------------------------------------------------------- Merkle levels --
+// A workgroup turns 2^g nodes of layer `lin` into the g layers above them.
+__global__ __launch_bounds__(256) void k_merkle_subtree(uint32_t* __restrict__ tree, size_t in_off, size_t n_in, int g) {
+    __shared__ uint32_t buf[2][256][8];
+    const size_t base = blockIdx.x * ((size_t)1 << g);  // first input node of this group
+    size_t lay_off = in_off, lay_n = n_in;
+    int cur = 0;
+    for (int l = 1; l <= g; l++) {
+        const int cnt = 1 << (g - l);
+        const size_t out_off = lay_off + lay_n;
+        const size_t out_base = base >> l;
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+            uint32_t msg[16];
+            if (l == 1) {
+                const uint32_t* src = tree + (lay_off + base + 2 * (size_t)i) * 8;
+#pragma unroll
+                for (int w = 0; w < 16; w++) msg[w] = src[w];
+            } else {
+#pragma unroll
+                for (int w = 0; w < 8; w++) {
+                    msg[w] = buf[cur][2 * i][w];
+                    msg[8 + w] = buf[cur][2 * i + 1][w];
+                }
+            }
+            uint32_t out[8];
+            b3::hash64(msg, out);
+            uint32_t* dst = tree + (out_off + out_base + i) * 8;
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                dst[w] = out[w];
+                buf[cur ^ 1][i][w] = out[w];
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+        lay_off = out_off;
+        lay_n >>= 1;
+    }
+}
+
+void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s) {
+    size_t in_off = 0, nodes = n_leaves;
+    while (nodes > 1) {
+        int lg = 0;
+        while (((size_t)1 << lg) < nodes) lg++;
+        int g = lg < 9 ? lg : 9;
+        size_t groups = nodes >> g;
+        hipLaunchKernelGGL(k_merkle_subtree, dim3((unsigned)groups), dim3(256), 0, s, tree, in_off, nodes, g);
+        // advance to the top layer produced
+        for (int l = 0; l < g; l++) {
+            in_off += nodes;
+            nodes >>= 1;
+        }
+    }
+}
+
+__global__ void k_gather_nodes(const uint32_t* __restrict__ tree, const uint64_t* __restrict__ idx, size_t count,
+                               uint32_t* __restrict__ out) {
+    size_t total = count * 8;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = tree[idx[i / 8] * 8 + (i % 8)];
+}
+
+void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t count, uint32_t* out, hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_gather_nodes, dim3(grid_for(count * 8, 256)), dim3(256), 0, s, tree, node_idx, count, out);
+}
+
+// ------------------------------------------------------------------ fold ---
+template <class F>
+__global__ void k_fold(const typename F::T* __restrict__ poly, size_t m, int kk, const typename F::T* __restrict__ gr_tabs,
+                       typename F::T* __restrict__ out) {
+    using T = typename F::T;
+    const int nc = 1 << kk;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) {
+        T acc = Ops<F>::zero();
+        for (int j = 0; j < nc; j++) acc ^= Ops<F>::mul_tab(gr_tabs + (size_t)j * F::TAB, poly[(size_t)j * m + i]);
+        out[i] = acc;
+    }
+}
+
+template <class F>
+void launch_fold(const typename F::T* poly, size_t m, int kk, const typename F::T* gr_tabs, typename F::T* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_fold<F>, dim3(grid_for(m, 256, 4096)), dim3(256), 0, s, poly, m, kk, gr_tabs, out);
+}
+
+template <class F>
+__global__ void k_gather_rows(const typename F::T* __restrict__ mat, size_t n, const uint64_t* __restrict__ rows, size_t count,
+                              typename F::T* __restrict__ out) {
+    size_t total = count * n;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = mat[rows[i / n] * n + (i % n)];
+}
+
+template <class F>
+void launch_gather_rows(const typename F::T* mat, size_t n, const uint64_t* rows, size_t count, typename F::T* out,
+                        hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_gather_rows<F>, dim3(grid_for(count * n, 256)), dim3(256), 0, s, mat, n, rows, count, out);
+}
+
+// ---------------------------------------------------------------- induce ---
+// basis[X*2^clo + x] = sum_i K[i][X] * P[i][x] with
+//   P[i][x] = prod_{t < clo, bit t of x} W_t(q_i),
+//   K[i][X] = alpha^i * prod_{t >= clo, bit t-clo of X} W_t(q_i),
+// which is the sum over queries of evaluate_scaled_basis_inplace
+// (binary_fft.rs:276-301, sumcheck_polys.rs:74-160).
+template <class F>
+__global__ void k_induce_tensor(const typename F::T* __restrict__ cvals, const typename F::T* __restrict__ ap, int ell, int clo,
+                                typename F::T* __restrict__ P, typename F::T* __restrict__ Kt) {
+    using T = typename F::T;
+    const int i = blockIdx.x;
+    const T* c = cvals + (size_t)i * ell;
+    const size_t np = (size_t)1 << clo, nk = (size_t)1 << (ell - clo);
+    for (size_t x = threadIdx.x; x < np + nk; x += blockDim.x) {
+        if (x < np) {
+            T v = Ops<F>::one();
+            for (int t = 0; t < clo; t++)
+                if ((x >> t) & 1) v = Ops<F>::mul(v, c[t]);
+            P[(size_t)i * np + x] = v;
+        } else {
+            size_t X = x - np;
+            T v = ap[i];
+            for (int t = 0; t < ell - clo; t++)
+                if ((X >> t) & 1) v = Ops<F>::mul(v, c[clo + t]);
+            Kt[(size_t)i * nk + X] = v;
+        }
+    }
+}
+
+template <class F>
+__global__ __launch_bounds__(256) void k_induce_accum(const typename F::T* __restrict__ P, const typename F::T* __restrict__ Kt,
+                                                      int nq, int ell, int clo, typename F::T* __restrict__ basis) {
+    using T = typename F::T;
+    __shared__ T tab[2][F::TAB];
+    constexpr int PER = 4;  // x values per thread (np <= 1024)
+    const size_t np = (size_t)1 << clo, nk = (size_t)1 << (ell - clo);
+    const size_t X = blockIdx.x;
+    T acc[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) acc[r] = Ops<F>::zero();
+    for (int i = 0; i < nq; i++) {
+        T kv = Kt[(size_t)i * nk + X];
+        for (int e = threadIdx.x; e < F::TAB; e += blockDim.x) tab[i & 1][e] = Ops<F>::table_entry(kv, e);
+        __syncthreads();
+        const T* Pi = P + (size_t)i * np;
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            size_t x = threadIdx.x + (size_t)r * blockDim.x;
+            if (x < np) acc[r] ^= Ops<F>::mul_tab(tab[i & 1], Pi[x]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        size_t x = threadIdx.x + (size_t)r * blockDim.x;
+        if (x < np) basis[X * np + x] = acc[r];
+    }
+}
+
+template <class F>
+void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
+                   typename F::T* basis, hipStream_t s) {
+    int clo = ell - 8;
+    if (clo < 8) clo = ell < 8 ? ell : 8;
+    if (clo > 10) clo = 10;
+    hipLaunchKernelGGL(k_induce_tensor<F>, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K);
+    size_t nk = (size_t)1 << (ell - clo);
+    hipLaunchKernelGGL(k_induce_accum<F>, dim3((unsigned)nk), dim3(256), 0, s, P, K, nq, ell, clo, basis);
+}
+
+// -------------------------------------------------------------- sumcheck ---
+template <class F> __device__ inline void acc_atomic_xor(typename F::Acc* dst, typename F::Acc v);
+template <> __device__ inline void acc_atomic_xor<F32>(uint64_t* dst, uint64_t v) {
+    if (v) atomicXor(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)v);
+}
+template <> __device__ inline void acc_atomic_xor<F128>(u128v* dst, u128v v) {
+    if (v.lo) atomicXor(reinterpret_cast<unsigned long long*>(&dst->lo), (unsigned long long)v.lo);
+    if (v.hi) atomicXor(reinterpret_cast<unsigned long long*>(&dst->hi), (unsigned long long)v.hi);
+}
+
+// XOR-reduces three accumulators across the workgroup and folds them into res.
+template <class F> __device__ inline void block_xor3(typename F::Acc a0, typename F::Acc a1, typename F::Acc a2, typename F::Acc* res) {
+    using A = typename F::Acc;
+    __shared__ A red[3][256];
+    red[0][threadIdx.x] = a0;
+    red[1][threadIdx.x] = a1;
+    red[2][threadIdx.x] = a2;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] ^= red[0][threadIdx.x + w];
+            red[1][threadIdx.x] ^= red[1][threadIdx.x + w];
+            red[2][threadIdx.x] ^= red[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        acc_atomic_xor<F>(&res[0], red[0][0]);
+        acc_atomic_xor<F>(&res[1], red[1][0]);
+        acc_atomic_xor<F>(&res[2], red[2][0]);
+    }
+}
+
+template <class F> __global__ void k_zero_acc(typename F::Acc* res) {
+    using A = typename F::Acc;
+    if (threadIdx.x < 3) res[threadIdx.x] = A{};
+}
+
+template <class F>
+__global__ __launch_bounds__(256) void k_eval013(const typename F::T* __restrict__ f, const typename F::T* __restrict__ g,
+                                                 size_t len, typename F::Acc* res) {
+    using T = typename F::T;
+    using A = typename F::Acc;
+    const size_t h = len / 2;
+    A s0 = Ops<F>::acc_zero(), s1 = Ops<F>::acc_zero(), s2 = Ops<F>::acc_zero();
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < h; i += (size_t)gridDim.x * blockDim.x) {
+        T a0 = f[i], a1 = f[i + h], b0 = g[i], b1 = g[i + h];
+        Ops<F>::acc_mul(s0, a0, b0);
+        Ops<F>::acc_mul(s1, a1, b1);
+        T a2 = Ops<F>::mul2(a1) ^ a1 ^ Ops<F>::mul2(a0);  // 3*f1 + (1+3)*f0
+        T b2 = Ops<F>::mul2(b1) ^ b1 ^ Ops<F>::mul2(b0);
+        Ops<F>::acc_mul(s2, a2, b2);
+    }
+    block_xor3<F>(s0, s1, s2, res);
+}
+
+template <class F>
+void launch_eval013(const typename F::T* f, const typename F::T* g, size_t len, typename F::Acc* res, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_acc<F>, dim3(1), dim3(64), 0, s, res);
+    hipLaunchKernelGGL(k_eval013<F>, dim3(grid_for(len / 2, 256, 1024)), dim3(256), 0, s, f, g, len, res);
+}
+
+template <class F>
+__global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __restrict__ f, const typename F::T* __restrict__ b,
+                                                       size_t len, typename F::T r, typename F::T* __restrict__ f2,
+                                                       typename F::T* __restrict__ b2, typename F::Acc* res) {
+    using T = typename F::T;
+    using A = typename F::Acc;
+    __shared__ T tab[F::TAB];
+    for (int e = threadIdx.x; e < F::TAB; e += blockDim.x) tab[e] = Ops<F>::table_entry(r, e);
+    __syncthreads();
+    const size_t L2 = len / 2, h2 = L2 / 2;
+    A s0 = Ops<F>::acc_zero(), s1 = Ops<F>::acc_zero(), s2 = Ops<F>::acc_zero();
+    if (h2 == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            f2[0] = f[0] ^ Ops<F>::mul_tab(tab, f[0] ^ f[1]);
+            b2[0] = b[0] ^ Ops<F>::mul_tab(tab, b[0] ^ b[1]);
+        }
+    } else {
+        for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < h2; i += (size_t)gridDim.x * blockDim.x) {
+            // (1+r)*e[i] + r*e[i+L2] = e[i] + r*(e[i] + e[i+L2])  (multilinear_poly.rs:71-73)
+            T fa = f[i], fb = f[i + L2], fc = f[i + h2], fd = f[i + h2 + L2];
+            T ba = b[i], bb = b[i + L2], bc = b[i + h2], bd = b[i + h2 + L2];
+            T f0 = fa ^ Ops<F>::mul_tab(tab, fa ^ fb);
+            T f1 = fc ^ Ops<F>::mul_tab(tab, fc ^ fd);
+            T g0 = ba ^ Ops<F>::mul_tab(tab, ba ^ bb);
+            T g1 = bc ^ Ops<F>::mul_tab(tab, bc ^ bd);
+            f2[i] = f0;
+            f2[i + h2] = f1;
+            b2[i] = g0;
+            b2[i + h2] = g1;
+            Ops<F>::acc_mul(s0, f0, g0);
+            Ops<F>::acc_mul(s1, f1, g1);
+            T fx = Ops<F>::mul2(f1) ^ f1 ^ Ops<F>::mul2(f0);
+            T gx = Ops<F>::mul2(g1) ^ g1 ^ Ops<F>::mul2(g0);
+            Ops<F>::acc_mul(s2, fx, gx);
+        }
+    }
+    block_xor3<F>(s0, s1, s2, res);
+}
+
+template <class F>
+void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t len, typename F::T r, typename F::T* f2,
+                          typename F::T* b2, typename F::Acc* res, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_acc<F>, dim3(1), dim3(64), 0, s, res);
+    size_t h2 = len / 4;
+    hipLaunchKernelGGL(k_sumcheck_fold<F>, dim3(grid_for(h2 ? h2 : 1, 256, 1024)), dim3(256), 0, s, f, b, len, r, f2, b2, res);
+}
+
+template <class F>
+__global__ void k_glue(typename F::T* __restrict__ bsum, const typename F::T* __restrict__ bnew, size_t len, typename F::T beta) {
+    using T = typename F::T;
+    __shared__ T tab[F::TAB];
+    for (int e = threadIdx.x; e < F::TAB; e += blockDim.x) tab[e] = Ops<F>::table_entry(beta, e);
+    __syncthreads();
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x)
+        bsum[i] ^= Ops<F>::mul_tab(tab, bnew[i]);
+}
+
+template <class F> void launch_glue(typename F::T* bsum, const typename F::T* bnew, size_t len, typename F::T beta, hipStream_t s) {
+    hipLaunchKernelGGL(k_glue<F>, dim3(grid_for(len, 256, 2048)), dim3(256), 0, s, bsum, bnew, len, beta);
+}
+
+// ------------------------------------------------------- instantiations ---
+#define LIG_INST(F)                                                                                                         \
+    template void launch_build_tables<F>(const F::T*, size_t, F::T*, hipStream_t);                                          \
+    template void launch_transpose_scale<F>(const F::T*, size_t, size_t, const F::T*, const F::T*, F::T*, hipStream_t);     \
+    template void launch_scale_rows<F>(F::T*, size_t, size_t, const F::T*, hipStream_t);                                    \
+    template void launch_fft_stage<F>(F::T*, size_t, size_t, int, const F::T*, int, hipStream_t);                           \
+    template void launch_fft_quarters<F>(const F::T*, F::T*, size_t, size_t, const F::T*, hipStream_t);                     \
+    template void launch_leaf_hash<F>(const F::T*, size_t, size_t, uint32_t*, hipStream_t);                                 \
+    template void launch_fold<F>(const F::T*, size_t, int, const F::T*, F::T*, hipStream_t);                                \
+    template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, hipStream_t);                  \
+    template void launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t);                   \
+    template void launch_eval013<F>(const F::T*, const F::T*, size_t, F::Acc*, hipStream_t);                                \
+    template void launch_sumcheck_fold<F>(const F::T*, const F::T*, size_t, F::T, F::T*, F::T*, F::Acc*, hipStream_t);      \
+    template void launch_glue<F>(F::T*, const F::T*, size_t, F::T, hipStream_t);
+
+LIG_INST(F32)
+LIG_INST(F128)
+
+}  // namespace lig

@@ -1,0 +1,94 @@
+// Launchers for the Ligerito hot-path kernels (kernels.hip).  All pointers
+// are device pointers; every launcher is asynchronous on `s`.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gf.hpp"
+
+namespace lig {
+
+// Element layouts shared by host and device code.  Device operations on
+// them live in fields.hpp (Ops<F>).
+struct F32 {
+    using T = uint32_t;
+    using Acc = uint64_t;                 // unreduced carry-less products
+    static constexpr int BITS = 32;
+    static constexpr int TAB = 128;        // multiply table: 8 nibble positions x 16 values
+    static constexpr int TAB_STRIDE = 144; // LDS stride: consecutive tables start 16 banks apart
+};
+struct F128 {
+    using T = u128v;
+    using Acc = u128v;                     // products are reduced before accumulation
+    static constexpr int BITS = 128;
+    static constexpr int TAB = 512;        // 32 nibble positions x 16 values
+    static constexpr int TAB_STRIDE = 512;
+};
+
+// FFT tile shape per field: up to 2^K rows x C columns per pass.
+template <class F> struct PassShape;
+
+// ---------------------------------------------------------------- tables --
+// tabs[i * F::TAB + e] = multiply-by-lams[i] table entry e.
+template <class F> void launch_build_tables(const typename F::T* lams, size_t count, typename F::T* tabs, hipStream_t s);
+
+// ------------------------------------------------------------ Reed–Solomon --
+// msg[i*n + c] = poly[c*m + i] * pis[i]  (poly is column-major as in
+// ligero.rs:11-27; msg is row-major).  pis[i] = pis[i & 63] * pis[i & ~63];
+// lo_tabs has min(m,64) tables, hi_tabs has max(m/64,1) tables.
+template <class F>
+void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const typename F::T* lo_tabs,
+                            const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s);
+// buf[i*n + c] *= pis[i] for i < rows (general multiply).
+template <class F> void launch_scale_rows(typename F::T* buf, size_t rows, size_t n, const typename F::T* pis, hipStream_t s);
+// One additive-FFT stage on a row-major [N][n] buffer with general multiply
+// (binary_fft.rs:180-206 butterflies); inverse selects the IFFT butterfly.
+template <class F>
+void launch_fft_stage(typename F::T* buf, size_t N, size_t n, int stage, const typename F::T* tw, int inverse, hipStream_t s);
+// Stages 2..log2(4m)-1 of the length-4m FFT on the four quarters of the
+// row-major [4m][n] matrix `mat`.  If msg != nullptr, the first pass reads
+// its input rows from msg (all four quarters start equal to msg), otherwise
+// from mat itself.  tw_tabs holds one table per twiddle index.
+template <class F>
+void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw_tabs,
+                         hipStream_t s);
+
+// -------------------------------------------------------------- Merkle ----
+// layer0[row*8 .. +8] = BLAKE3(row bytes) for every row of the [rows][n] matrix.
+template <class F> void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, hipStream_t s);
+// Builds layers 1.. in place after layer 0 (contiguous node layout,
+// merkle_tree.rs:22-60).  n_leaves must be a power of two.
+void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s);
+// out[i] = node at byte offset offs[i] of the tree (32 bytes each).
+void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t count, uint32_t* out, hipStream_t s);
+
+// ------------------------------------------------------------ folds -------
+// out[i] = sum_j gr[j] * poly[j*m + i], j < 2^kk (partial_eval over the top
+// kk variables, multilinear_poly.rs:66-87).  gr_tabs: 2^kk tables.
+template <class F>
+void launch_fold(const typename F::T* poly, size_t m, int kk, const typename F::T* gr_tabs, typename F::T* out, hipStream_t s);
+template <class F>
+void launch_gather_rows(const typename F::T* mat, size_t n, const uint64_t* rows, size_t count, typename F::T* out,
+                        hipStream_t s);
+
+// ------------------------------------------------------------ induce ------
+// cvals[i*ell + t] = W_t(q_i); ap[i] = alpha^i.  Produces basis[2^ell].
+template <class F>
+void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
+                   typename F::T* basis, hipStream_t s);
+
+// ------------------------------------------------------------ sumcheck ----
+// res[0..3) = (sum f0*g0, sum f1*g1, sum f2*g2) unreduced (eval_013_product).
+template <class F>
+void launch_eval013(const typename F::T* f, const typename F::T* g, size_t len, typename F::Acc* res, hipStream_t s);
+// Folds f and b by r (length len -> len/2) into f2, b2 and writes the
+// triple of the folded pair (ligerito.rs:119-127).
+template <class F>
+void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t len, typename F::T r, typename F::T* f2,
+                          typename F::T* b2, typename F::Acc* res, hipStream_t s);
+// bsum[i] ^= beta * bnew[i]   (glue, ligerito.rs:59-67)
+template <class F>
+void launch_glue(typename F::T* bsum, const typename F::T* bnew, size_t len, typename F::T beta, hipStream_t s);
+
+}  // namespace lig

@@ -1,0 +1,413 @@
+"""Python binding of libligerito_hip.so (include/ligerito.h) mirroring the
+reference crate's API names (ashutosh1206/ligerito-rust, src/lib.rs:17-31).
+
+The library is the HIP implementation; if it is missing this module raises
+instead of falling back to anything else.  Field elements are numpy arrays:
+uint32 for GF(2^32), (n, 2) uint64 (low word first) for GF(2^128).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libligerito_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "ligerito.h")
+
+LIG_OK = 0
+ERRORS = {-1: "LIG_E_INVALID", -2: "LIG_E_ASSERT", -3: "LIG_E_HIP", -4: "LIG_E_OOM", -5: "LIG_E_NO_DEVICE",
+          -6: "LIG_E_MALFORMED", -7: "LIG_E_CAPACITY", -8: "LIG_E_MERKLE", -9: "LIG_E_LIGERO"}
+
+_lib = None
+
+
+class LigeritoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def header_symbols():
+    """Every function the C header declares."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(lig_[a-z0-9_]+)\s*\(", txt)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P, SZ, I, U32 = C.c_void_p, C.c_size_t, C.c_int, C.c_uint32
+        PP = C.POINTER(C.c_void_p)
+        sig = {
+            "lig_last_error": (C.c_char_p, []),
+            "lig_device_count": (I, []),
+            "lig_ctx_create": (I, [I, PP]),
+            "lig_ctx_destroy": (None, [P]),
+            "lig_ctx_stream": (P, [P]),
+            "lig_ctx_profile": (I, [P, I]),
+            "lig_ctx_profile_read": (I, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+            "lig_ctx_synchronize": (I, [P]),
+            "lig_rs_new": (I, [P, I, SZ, SZ, PP]),
+            "lig_rs_free": (None, [P]),
+            "lig_rs_message_length": (SZ, [P]),
+            "lig_rs_block_length": (SZ, [P]),
+            "lig_rs_encode": (I, [P, P, P]),
+            "lig_rs_encode_non_systematic": (I, [P, P]),
+            "lig_rs_encode_non_systematic_cols_dev": (I, [P, P, SZ]),
+            "lig_rs_twiddles": (I, [P, P]),
+            "lig_build_merkle_tree": (I, [P, P, SZ, SZ, P]),
+            "lig_merkle_prove": (I, [P, SZ, P, SZ, P, SZ, C.POINTER(SZ)]),
+            "lig_merkle_verify": (I, [P, P, SZ, SZ, P, SZ, P, SZ]),
+            "lig_fs_new": (I, [U32, PP]),
+            "lig_fs_free": (None, [P]),
+            "lig_fs_absorb": (I, [P, P, SZ]),
+            "lig_fs_squeeze": (I, [P, P]),
+            "lig_fs_get_field": (I, [P, I, P]),
+            "lig_fs_get_query": (SZ, [P, SZ]),
+            "lig_fs_get_distinct_queries": (I, [P, SZ, SZ, P]),
+            "lig_ligero_commit": (I, [P, P, SZ, SZ, PP]),
+            "lig_ligero_commit_dev": (I, [P, P, SZ, SZ, PP]),
+            "lig_witness_root": (I, [P, P]),
+            "lig_witness_num_rows": (SZ, [P]),
+            "lig_witness_num_cols": (SZ, [P]),
+            "lig_witness_flat_mat": (I, [P, P]),
+            "lig_witness_layers": (I, [P, P]),
+            "lig_witness_extract_row": (I, [P, SZ, P]),
+            "lig_witness_free": (None, [P]),
+            "lig_prover_config_new": (I, [P, I, I, PP]),
+            "lig_prover_config_free": (None, [P]),
+            "lig_prover": (I, [P, P, U32, PP]),
+            "lig_prover_dev": (I, [P, P, U32, PP]),
+            "lig_proof_serialize": (I, [P, P, SZ, C.POINTER(SZ)]),
+            "lig_proof_deserialize": (I, [I, P, SZ, PP]),
+            "lig_proof_size": (SZ, [P]),
+            "lig_proof_free": (None, [P]),
+            "lig_verifier": (I, [I, I, P, U32]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise LigeritoError(rc, lib().lig_last_error().decode())
+    return rc
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def as_elems(x, bits):
+    if bits == 128:
+        return np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, 2)
+    return np.ascontiguousarray(x, dtype=np.uint32)
+
+
+def empty(n, bits):
+    return np.zeros((n, 2), dtype=np.uint64) if bits == 128 else np.zeros(n, dtype=np.uint32)
+
+
+def nelems(a, bits):
+    return a.shape[0]
+
+
+# ------------------------------------------------------------------ context --
+class Context:
+    """Device, stream and caches (lig_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().lig_ctx_create(device, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return lib().lig_ctx_stream(self._h)
+
+    def synchronize(self):
+        _check(lib().lig_ctx_synchronize(self._h))
+
+    def profile(self, enable: bool = True):
+        """Enables per-phase HIP-event timing on the library stream (resets totals)."""
+        _check(lib().lig_ctx_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self, phase: str):
+        """(total_ms, count) of a phase since profiling was enabled."""
+        ms, n = C.c_double(0), C.c_uint64(0)
+        _check(lib().lig_ctx_profile_read(self._h, phase.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lig_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+# --------------------------------------------------------------- Reed–Solomon --
+class ReedSolomonEncoding:
+    """src/reed_solomon.rs:7-114"""
+
+    def __init__(self, ctx: Context, message_length: int, block_length: int, bits: int = 32):
+        h = C.c_void_p()
+        _check(lib().lig_rs_new(ctx.handle, bits, message_length, block_length, C.byref(h)))
+        self._h, self.bits, self.ctx = h, bits, ctx
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lig_rs_free(self._h)
+            self._h = None
+
+    def message_length(self):
+        return lib().lig_rs_message_length(self._h)
+
+    def block_length(self):
+        return lib().lig_rs_block_length(self._h)
+
+    def twiddles(self):
+        out = empty(self.block_length() - 1, self.bits)
+        _check(lib().lig_rs_twiddles(self._h, _p(out)))
+        return out
+
+    def encode(self, message):
+        m = as_elems(message, self.bits)
+        if m.shape[0] != self.message_length():
+            raise LigeritoError(-1, "message length mismatch (reed_solomon.rs:91)")
+        out = empty(self.block_length(), self.bits)
+        _check(lib().lig_rs_encode(self._h, _p(m), _p(out)))
+        return out
+
+    def encode_non_systematic(self, message):
+        """Returns the encoded block (the reference encodes in place)."""
+        m = as_elems(message, self.bits).copy()
+        if m.shape[0] != self.block_length():
+            raise LigeritoError(-1, "block length mismatch (reed_solomon.rs:105)")
+        _check(lib().lig_rs_encode_non_systematic(self._h, _p(m)))
+        return m
+
+
+# ------------------------------------------------------------------- Merkle --
+def build_merkle_tree(ctx: Context, leaves: np.ndarray):
+    """src/merkle_tree.rs:22-60 for equal-length leaves (rows of a 2-D uint8
+    array).  Returns the list of layers as bytes, leaves first."""
+    leaves = np.ascontiguousarray(leaves, dtype=np.uint8)
+    n, ll = leaves.shape
+    out = np.zeros(32 * (2 * n - 1), dtype=np.uint8)
+    _check(lib().lig_build_merkle_tree(ctx.handle, _p(leaves), n, ll, _p(out)))
+    layers, pos, d = [], 0, 0
+    while (n >> d) >= 1:
+        sz = 32 * (n >> d)
+        layers.append(out[pos:pos + sz].tobytes())
+        pos += sz
+        d += 1
+    return layers
+
+
+def prove(tree_layers, queries):
+    """src/merkle_tree.rs:98-111"""
+    flat = np.frombuffer(b"".join(tree_layers), dtype=np.uint8)
+    n = len(tree_layers[0]) // 32
+    q = np.ascontiguousarray(queries, dtype=np.uint64)
+    cnt = C.c_size_t(0)
+    _check(lib().lig_merkle_prove(_p(flat), n, _p(q), q.shape[0], None, 0, C.byref(cnt)))
+    out = np.zeros(32 * max(cnt.value, 1), dtype=np.uint8)
+    _check(lib().lig_merkle_prove(_p(flat), n, _p(q), q.shape[0], _p(out), cnt.value, C.byref(cnt)))
+    return [out[32 * i:32 * i + 32].tobytes() for i in range(cnt.value)]
+
+
+def verify(root: bytes, proof, depth: int, leaves, indices) -> bool:
+    """src/merkle_tree.rs:164-188 for equal-length leaves."""
+    lv = np.ascontiguousarray(leaves, dtype=np.uint8)
+    pr = np.frombuffer(b"".join(proof) + b"\0", dtype=np.uint8)
+    rt = np.frombuffer(bytes(root), dtype=np.uint8)
+    idx = np.ascontiguousarray(indices, dtype=np.uint64)
+    return bool(lib().lig_merkle_verify(_p(rt), _p(pr), len(proof), depth, _p(lv), lv.shape[1], _p(idx), idx.shape[0]))
+
+
+# --------------------------------------------------------------- transcript --
+class FS:
+    """src/emulated_fs.rs:6-75"""
+
+    def __init__(self, seed: int):
+        h = C.c_void_p()
+        _check(lib().lig_fs_new(seed, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lig_fs_free(self._h)
+            self._h = None
+
+    def absorb(self, data: bytes):
+        b = bytes(data)
+        buf = C.create_string_buffer(b, max(1, len(b)))
+        _check(lib().lig_fs_absorb(self._h, buf, len(b)))
+
+    def squeeze(self) -> int:
+        out = (C.c_uint8 * 16)()
+        _check(lib().lig_fs_squeeze(self._h, out))
+        return int.from_bytes(bytes(out), "little")
+
+    def get_field(self, bits: int = 32) -> int:
+        out = (C.c_uint8 * (bits // 8))()
+        _check(lib().lig_fs_get_field(self._h, bits, out))
+        return int.from_bytes(bytes(out), "little")
+
+    def get_query(self, n: int) -> int:
+        return int(lib().lig_fs_get_query(self._h, n))
+
+    def get_distinct_queries(self, n: int, s: int):
+        out = np.zeros(s, dtype=np.uint64)
+        _check(lib().lig_fs_get_distinct_queries(self._h, n, s, _p(out)))
+        return [int(x) for x in out]
+
+
+# ------------------------------------------------------------ ligero_commit --
+class Witness:
+    """RecursiveLigeroWitness (data_structures.rs:32-40), device-resident."""
+
+    def __init__(self, h, bits):
+        self._h, self.bits = h, bits
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lig_witness_free(self._h)
+            self._h = None
+
+    @property
+    def num_rows(self):
+        return lib().lig_witness_num_rows(self._h)
+
+    @property
+    def num_cols(self):
+        return lib().lig_witness_num_cols(self._h)
+
+    def root(self) -> bytes:
+        out = (C.c_uint8 * 32)()
+        _check(lib().lig_witness_root(self._h, out))
+        return bytes(out)
+
+    def flat_mat(self):
+        out = empty(self.num_rows * self.num_cols, self.bits)
+        _check(lib().lig_witness_flat_mat(self._h, _p(out)))
+        return out
+
+    def tree(self):
+        n = self.num_rows
+        out = np.zeros(32 * (2 * n - 1), dtype=np.uint8)
+        _check(lib().lig_witness_layers(self._h, _p(out)))
+        layers, pos, d = [], 0, 0
+        while (n >> d) >= 1:
+            sz = 32 * (n >> d)
+            layers.append(out[pos:pos + sz].tobytes())
+            pos += sz
+            d += 1
+        return layers
+
+    def extract_row(self, row):
+        out = empty(self.num_cols, self.bits)
+        _check(lib().lig_witness_extract_row(self._h, row, _p(out)))
+        return out
+
+
+def ligero_commit(poly, m: int, n: int, rs: ReedSolomonEncoding) -> Witness:
+    """src/ligero.rs:70-95"""
+    p = as_elems(poly, rs.bits)
+    h = C.c_void_p()
+    _check(lib().lig_ligero_commit(rs._h, _p(p), m, n, C.byref(h)))
+    return Witness(h, rs.bits)
+
+
+def ligero_commit_dev(d_poly_ptr: int, m: int, n: int, rs: ReedSolomonEncoding) -> Witness:
+    h = C.c_void_p()
+    _check(lib().lig_ligero_commit_dev(rs._h, C.c_void_p(d_poly_ptr), m, n, C.byref(h)))
+    return Witness(h, rs.bits)
+
+
+# ------------------------------------------------------------ prove / verify --
+class ProverConfig:
+    """ProverConfig (data_structures.rs:9-21); hardcoded_config_20/24 at log_n 20/24."""
+
+    def __init__(self, ctx: Context, log_n: int, bits: int = 32):
+        h = C.c_void_p()
+        _check(lib().lig_prover_config_new(ctx.handle, bits, log_n, C.byref(h)))
+        self._h, self.bits, self.log_n, self.ctx = h, bits, log_n, ctx
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lig_prover_config_free(self._h)
+            self._h = None
+
+
+def hardcoded_config_20(ctx, bits=32):
+    return ProverConfig(ctx, 20, bits)
+
+
+def hardcoded_config_24(ctx, bits=32):
+    return ProverConfig(ctx, 24, bits)
+
+
+class Proof:
+    """FinalizedLigeritoProof (data_structures.rs:91-103)."""
+
+    def __init__(self, h, bits):
+        self._h, self.bits = h, bits
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lig_proof_free(self._h)
+            self._h = None
+
+    def to_bytes(self) -> bytes:
+        n = C.c_size_t(0)
+        _check(lib().lig_proof_serialize(self._h, None, 0, C.byref(n)))
+        buf = np.zeros(n.value, dtype=np.uint8)
+        _check(lib().lig_proof_serialize(self._h, _p(buf), n.value, C.byref(n)))
+        return buf.tobytes()
+
+    @staticmethod
+    def from_bytes(data: bytes, bits: int = 32) -> "Proof":
+        buf = np.frombuffer(bytes(data), dtype=np.uint8).copy()
+        h = C.c_void_p()
+        _check(lib().lig_proof_deserialize(bits, _p(buf), buf.shape[0], C.byref(h)))
+        return Proof(h, bits)
+
+    def proof_size(self) -> int:
+        return int(lib().lig_proof_size(self._h))
+
+
+def prover(config: ProverConfig, poly, fs_seed: int = 1234) -> Proof:
+    """src/prover.rs:15-199 (host-resident input)."""
+    p = as_elems(poly, config.bits)
+    h = C.c_void_p()
+    _check(lib().lig_prover(config._h, _p(p), fs_seed, C.byref(h)))
+    return Proof(h, config.bits)
+
+
+def prover_dev(config: ProverConfig, d_poly_ptr: int, fs_seed: int = 1234) -> Proof:
+    """prover on a device-resident polynomial (pointer to 2^log_n elements)."""
+    h = C.c_void_p()
+    _check(lib().lig_prover_dev(config._h, C.c_void_p(d_poly_ptr), fs_seed, C.byref(h)))
+    return Proof(h, config.bits)
+
+
+def verifier(log_n: int, proof: Proof, fs_seed: int = 1234) -> int:
+    """src/verifier.rs:10-164: 1 accept, 0 reject, negative at the reference's panic sites."""
+    return int(lib().lig_verifier(proof.bits, log_n, proof._h, fs_seed))

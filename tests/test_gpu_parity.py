@@ -1,0 +1,144 @@
+"""Parity of the HIP path with the oracle (bit-exact), through the C ABI.
+
+Each case prints what it runs before it runs it, so a failing GPU step is
+named in the log."""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import ligerito_hip as lh
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if lh.lib().lig_device_count() == 0:
+        pytest.fail("no GPU visible: the HIP library cannot run (no CPU fallback exists)")
+    c = lh.Context(0)
+    yield c
+    c.close()
+
+
+def col(O, seed, n, bits):
+    return O.splitmix(seed, n, bits)
+
+
+@pytest.mark.parametrize("bits", [32, 128])
+@pytest.mark.parametrize("logm", [0, 1, 3, 6, 9])
+def test_rs_non_systematic_matches_oracle(O, ctx, bits, logm):
+    m = 1 << logm
+    M = 4 * m
+    say("rs_non_systematic", bits, m)
+    rs = lh.ReedSolomonEncoding(ctx, m, M, bits)
+    assert np.array_equal(rs.twiddles(), O.compute_twiddles(logm + 2, bits))
+    buf = O.empty(M, bits)
+    buf[:m] = col(O, 11 + logm, m, bits)
+    got = rs.encode_non_systematic(buf)
+    want = O.rs_encode_non_systematic(buf, m, bits)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("bits", [32, 128])
+def test_rs_non_systematic_nonzero_tail(O, ctx, bits):
+    # encode_non_systematic FFTs the whole block (reed_solomon.rs:104-113)
+    m, M = 64, 256
+    say("rs_non_systematic tail", bits)
+    rs = lh.ReedSolomonEncoding(ctx, m, M, bits)
+    buf = col(O, 5, M, bits)
+    assert np.array_equal(rs.encode_non_systematic(buf), O.rs_encode_non_systematic(buf, m, bits))
+
+
+@pytest.mark.parametrize("bits", [32, 128])
+def test_rs_systematic_matches_oracle(O, ctx, bits):
+    m, M = 256, 1024
+    say("rs_systematic", bits)
+    rs = lh.ReedSolomonEncoding(ctx, m, M, bits)
+    msg = col(O, 3, m, bits)
+    got = rs.encode(msg)
+    assert np.array_equal(got, O.rs_encode(msg, M, bits))
+    assert np.array_equal(got[:m], msg)  # reed_solomon.rs:121-131
+
+
+@pytest.mark.parametrize("leaf_len", [4, 32, 64, 256, 1024, 1028, 16384])
+def test_build_merkle_tree_matches_oracle(O, ctx, leaf_len):
+    n = 64 if leaf_len < 16384 else 16
+    say("merkle", leaf_len, n)
+    rng = np.random.default_rng(leaf_len)
+    leaves = rng.integers(0, 256, size=(n, leaf_len), dtype=np.uint8)
+    got = lh.build_merkle_tree(ctx, leaves)
+    want = O.build_merkle_tree([bytes(r) for r in leaves])
+    assert got == want
+
+
+@pytest.mark.parametrize("bits,m,n", [(32, 1 << 10, 64), (32, 1 << 8, 16), (32, 64, 4), (32, 16, 1),
+                                      (128, 1 << 8, 16), (128, 64, 64), (32, 1 << 9, 128)])
+def test_ligero_commit_matches_oracle(O, ctx, bits, m, n):
+    say("commit", bits, m, n)
+    rs = lh.ReedSolomonEncoding(ctx, m, 4 * m, bits)
+    poly = col(O, m + n, m * n, bits)
+    w = lh.ligero_commit(poly, m, n, rs)
+    root, mat, layers = O.ligero_commit(poly, m, n, bits, want_matrix=True, want_layers=True)
+    assert w.root() == root
+    assert np.array_equal(w.flat_mat(), mat)
+    assert w.tree() == layers
+    for r in (0, 1, 4 * m - 1):
+        assert np.array_equal(w.extract_row(r), mat.reshape(n, 4 * m, -1)[:, r].reshape(w.extract_row(r).shape))
+
+
+def test_commit_encode_kernel_shape_gf32(O, ctx):
+    # 4096 independent length-2^12 rows encoded to 2^14, 16 KiB leaves
+    m, n = 1 << 12, 4096
+    say("commit gf32 4096x2^12")
+    rs = lh.ReedSolomonEncoding(ctx, m, 4 * m, 32)
+    poly = col(O, 7, m * n, 32)
+    w = lh.ligero_commit(poly, m, n, rs)
+    root, _, _ = O.ligero_commit(poly, m, n, 32)
+    assert w.root() == root
+
+
+@pytest.mark.parametrize("bits,L", [(32, 16), (128, 16), (32, 17), (32, 20)])
+def test_prover_bytes_match_oracle(O, ctx, bits, L):
+    say("prove", bits, L)
+    cfg = lh.ProverConfig(ctx, L, bits)
+    poly = col(O, 1, 1 << L, bits)
+    p = lh.prover(cfg, poly)
+    got = p.to_bytes()
+    want = O.prove(poly, L, bits)
+    assert got == want
+    assert O.verify(got, L, bits) == 1
+    assert lh.verifier(L, p) == 1
+    if (bits, L) == (32, 20):
+        g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "oracle_proofs.json")))
+        assert hashlib.sha256(got).hexdigest() == g["L20_seed1_gf32"]["sha256"]
+
+
+def test_prover_seeded_transcripts(O, ctx):
+    cfg = lh.ProverConfig(ctx, 16, 32)
+    poly = col(O, 9, 1 << 16, 32)
+    for seed in (1, 99):
+        say("prove seed", seed)
+        got = lh.prover(cfg, poly, fs_seed=seed).to_bytes()
+        assert got == O.prove(poly, 16, 32, fs_seed=seed)
+
+
+def test_prover_2pow24_bit_exact(O, ctx):
+    # the full-size workload: GPU proof bytes == oracle proof bytes, and both
+    # verifiers accept
+    say("prove 2^24")
+    cfg = lh.ProverConfig(ctx, 24, 32)
+    poly = col(O, 2, 1 << 24, 32)
+    p = lh.prover(cfg, poly)
+    got = p.to_bytes()
+    assert lh.verifier(24, p) == 1
+    want = O.prove(poly, 24, 32)
+    assert got == want
