@@ -75,10 +75,48 @@ __global__ __launch_bounds__(256) void k_transpose_scale(const typename F::T* __
     }
 }
 
+__device__ __forceinline__ uint32_t mul_tab_lds(const uint32_t* tabs, uint32_t tb, uint32_t w);
+
+// GF(2^32) variant: the 64 low-part pis tables stay in LDS for the whole
+// workgroup, which walks several 64x64 tiles; each tile adds one table.
+__global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __restrict__ poly, size_t m, size_t n,
+                                                          const uint32_t* __restrict__ lo_tabs,
+                                                          const uint32_t* __restrict__ hi_tabs, uint32_t* __restrict__ msg) {
+    __shared__ __attribute__((aligned(256))) uint32_t tlo[64 * 128];
+    __shared__ __attribute__((aligned(256))) uint32_t thi[128];
+    __shared__ uint32_t tile[64][65];
+    const int rt = m < 64 ? (int)m : 64;
+    const int ct = n < 64 ? (int)n : 64;
+    const size_t row_tiles = (m + 63) / 64, col_tiles = (n + 63) / 64;
+    for (int idx = threadIdx.x; idx < rt * 128; idx += blockDim.x) tlo[idx] = lo_tabs[idx];
+    for (size_t tile_i = blockIdx.x; tile_i < row_tiles * col_tiles; tile_i += gridDim.x) {
+        const size_t i0 = (tile_i % row_tiles) * 64;
+        const size_t c0 = (tile_i / row_tiles) * 64;
+        if (threadIdx.x < 128) thi[threadIdx.x] = hi_tabs[(i0 >> 6) * 128 + threadIdx.x];
+        for (int idx = threadIdx.x; idx < rt * ct; idx += blockDim.x) {
+            int r = idx % rt, c = idx / rt;
+            tile[c][r] = poly[(c0 + c) * m + i0 + r];
+        }
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < rt * ct; idx += blockDim.x) {
+            int c = idx % ct, r = idx / ct;
+            uint32_t x = mul_tab_lds(tlo, (uint32_t)(r * 512), tile[c][r]);
+            x = mul_tab_lds(thi, 0u, x);
+            msg[(i0 + r) * n + c0 + c] = x;
+        }
+        __syncthreads();
+    }
+}
+
 template <class F>
 void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const typename F::T* lo_tabs,
                             const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s) {
     size_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
+    if constexpr (F::BITS == 32) {
+        unsigned g = (unsigned)(tiles < 1024 ? tiles : 1024);
+        hipLaunchKernelGGL(k_transpose_scale32, dim3(g), dim3(256), 0, s, poly, m, n, lo_tabs, hi_tabs, msg);
+        return;
+    }
     hipLaunchKernelGGL(k_transpose_scale<F>, dim3((unsigned)tiles), dim3(256), 0, s, poly, m, n, lo_tabs, hi_tabs, msg);
 }
 
@@ -190,12 +228,20 @@ __global__ __launch_bounds__(256) void k_fft_pass(const typename F::T* __restric
     }
 }
 
+static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw_tabs, hipStream_t s);
+
 template <class F>
 void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw_tabs,
                          hipStream_t s) {
     constexpr int K = PassShape<F>::K, C = PassShape<F>::C;
     int ell = 0;
     while (((size_t)1 << ell) < m) ell++;
+    if constexpr (F::BITS == 32) {
+        if (ell > 0) {
+            fft_quarters32(msg, mat, m, n, tw_tabs, s);
+            return;
+        }
+    }
     int npass = ell == 0 ? 1 : (ell + K - 1) / K;
     int d0 = 0;
     const size_t nchunks = (n + C - 1) / C;
@@ -207,6 +253,171 @@ void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m,
         int shared = (p == 0 && msg) ? 1 : 0;
         hipLaunchKernelGGL((k_fft_pass<F, K, C>), grid, dim3(256), 0, s, src, mat, m, n, ell, d0, k, shared, tw_tabs);
         d0 += k;
+    }
+}
+
+// ------------------------------------------------ GF(2^32) LDS-table helpers --
+// Tables live in LDS 512 bytes apart (256-byte aligned base).  For a table
+// starting `tb` bytes into the array, the address of the entry for nibble k
+// of w is tb | (nibble << 2) plus the immediate 64*k: v_perm_b32 builds it in
+// one instruction from tb and a byte of the pre-masked nibble words.
+__device__ __forceinline__ uint32_t lds_ld32(const uint32_t* base, uint32_t byte_off) {
+    return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+__device__ __forceinline__ uint32_t mul_tab_lds(const uint32_t* tabs, uint32_t tb, uint32_t w) {
+    const uint32_t qa = (w << 2) & 0x3C3C3C3Cu;  // byte j: nibble 2j, times 4
+    const uint32_t qb = (w >> 2) & 0x3C3C3C3Cu;  // byte j: nibble 2j+1, times 4
+    uint32_t r0 = lds_ld32(tabs + 0, __builtin_amdgcn_perm(tb, qa, 0x07060500u));
+    uint32_t r1 = lds_ld32(tabs + 16, __builtin_amdgcn_perm(tb, qb, 0x07060500u));
+    uint32_t r2 = lds_ld32(tabs + 32, __builtin_amdgcn_perm(tb, qa, 0x07060501u));
+    uint32_t r3 = lds_ld32(tabs + 48, __builtin_amdgcn_perm(tb, qb, 0x07060501u));
+    uint32_t r4 = lds_ld32(tabs + 64, __builtin_amdgcn_perm(tb, qa, 0x07060502u));
+    uint32_t r5 = lds_ld32(tabs + 80, __builtin_amdgcn_perm(tb, qb, 0x07060502u));
+    uint32_t r6 = lds_ld32(tabs + 96, __builtin_amdgcn_perm(tb, qa, 0x07060503u));
+    uint32_t r7 = lds_ld32(tabs + 112, __builtin_amdgcn_perm(tb, qb, 0x07060503u));
+    return (r0 ^ r1 ^ r2) ^ (r3 ^ r4 ^ r5) ^ (r6 ^ r7);
+}
+
+// --------------------------------------------- GF(2^32) register-radix pass --
+// One launch = stages d0..d0+K-1 of quarter q's size-m FFT for every tile.
+// A tile is the 2^K rows r(t) = B*2^(ell-d0) + t*S + lo (S = 2^(ell-d0-K))
+// of up to 64 columns.  Phase 1 keeps a chain of 2^K1 rows (t = a*2^K2 + g)
+// of one column in registers and runs the first K1 stages; one LDS exchange
+// regroups the tile into chains of 2^K2 consecutive rows for the last K2
+// stages.  All 2^K - 1 twiddle tables of the tile are staged once: the
+// twiddle of block B*2^j + t' at stage j is (by GF(2)-linearity in the block
+// index) lambda(base_j) + lambda(t'), so each entry is the XOR of the tile's
+// base table and a table shared by every tile.
+template <int K1, int K2>
+__global__ __launch_bounds__(512) void k_fft_pass32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t m,
+                                                    size_t n, int ell, int d0, int q, int src_shared,
+                                                    const uint32_t* __restrict__ tw_tabs) {
+    constexpr int K = K1 + K2, R = 1 << K, NT = R - 1;
+    constexpr int L1 = 1 << K1, L2 = 1 << K2;
+    __shared__ __attribute__((aligned(256))) uint32_t tabs[NT * 128];
+    __shared__ uint32_t xch[K2 > 0 ? R * 64 : 1];
+    const int cc = n < 64 ? (int)n : 64;
+    const size_t nchunks = (n + 63) / 64;
+    const size_t tile_id = blockIdx.x / nchunks;
+    const size_t c0 = (blockIdx.x % nchunks) * 64;
+    const int sh = ell - d0 - K;
+    const size_t S = (size_t)1 << sh;
+    const size_t B = tile_id >> sh, lo = tile_id & (S - 1);
+    const size_t rbase = (B << (ell - d0)) + lo;
+    const int tid = threadIdx.x;
+    const int c = tid & (cc - 1);
+    const int g = tid / cc;  // < L2
+
+#pragma unroll 4
+    for (int idx = tid; idx < NT * 128; idx += blockDim.x) {
+        const int ti = idx >> 7, e = idx & 127;
+        const int j = 31 - __clz(ti + 1);
+        const int tp = ti + 1 - (1 << j);
+        const int d = d0 + j;
+        const size_t first = ((size_t)1 << (d + 2)) - 1;
+        const size_t ib = first + ((size_t)q << d) + (B << j);
+        tabs[idx] = tw_tabs[ib * 128 + e] ^ tw_tabs[(first + tp) * 128 + e];
+    }
+
+    uint32_t v[L1];
+    const size_t qoff = src_shared ? 0 : (size_t)q * m;
+    const uint32_t* sp = src + c0 + c;
+#pragma unroll
+    for (int a = 0; a < L1; a++) v[a] = sp[(qoff + rbase + ((size_t)(a * L2 + g)) * S) * n];
+    __syncthreads();
+
+#pragma unroll
+    for (int j = 0; j < K1; j++) {
+        const int h = L1 >> (j + 1);
+#pragma unroll
+        for (int a = 0; a < L1; a++) {
+            if (a & h) continue;
+            const uint32_t tb = (uint32_t)(((1 << j) - 1 + (a >> (K1 - j))) * 512);
+            const uint32_t u = v[a] ^ mul_tab_lds(tabs, tb, v[a + h]);
+            v[a] = u;
+            v[a + h] ^= u;
+        }
+    }
+
+    uint32_t* dp = dst + c0 + c;
+    const size_t doff = (size_t)q * m + rbase;
+    if constexpr (K2 == 0) {
+#pragma unroll
+        for (int a = 0; a < L1; a++) dp[(doff + (size_t)a * S) * n] = v[a];
+    } else {
+#pragma unroll
+        for (int a = 0; a < L1; a++) xch[(a * L2 + g) * 64 + c] = v[a];
+        __syncthreads();
+        constexpr int NC2 = L1 / L2;
+#pragma unroll
+        for (int i = 0; i < NC2; i++) {
+            const int hi = g + L2 * i;
+#pragma unroll
+            for (int e = 0; e < L2; e++) v[i * L2 + e] = xch[(hi * L2 + e) * 64 + c];
+        }
+#pragma unroll
+        for (int j = K1; j < K; j++) {
+            const int h = L2 >> (j - K1 + 1);
+#pragma unroll
+            for (int i = 0; i < NC2; i++) {
+                const int hi = g + L2 * i;
+#pragma unroll
+                for (int e = 0; e < L2; e++) {
+                    if (e & h) continue;
+                    const uint32_t tb = (uint32_t)(((1 << j) - 1 + (hi << (j - K1)) + (e >> (K - j))) * 512);
+                    const uint32_t u = v[i * L2 + e] ^ mul_tab_lds(tabs, tb, v[i * L2 + e + h]);
+                    v[i * L2 + e] = u;
+                    v[i * L2 + e + h] ^= u;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NC2; i++) {
+            const int hi = g + L2 * i;
+#pragma unroll
+            for (int e = 0; e < L2; e++) dp[(doff + (size_t)(hi * L2 + e) * S) * n] = v[i * L2 + e];
+        }
+    }
+}
+
+static void launch_pass32(int K, dim3 grid, int block, hipStream_t s, const uint32_t* src, uint32_t* dst, size_t m, size_t n,
+                          int ell, int d0, int q, int shared, const uint32_t* tabs) {
+#define LIG_P32(A, B_)                                                                                                   \
+    hipLaunchKernelGGL((k_fft_pass32<A, B_>), grid, dim3(block), 0, s, src, dst, m, n, ell, d0, q, shared, tabs);       \
+    break;
+    switch (K) {
+        case 1: LIG_P32(1, 0)
+        case 2: LIG_P32(1, 1)
+        case 3: LIG_P32(2, 1)
+        case 4: LIG_P32(2, 2)
+        case 5: LIG_P32(3, 2)
+        case 6: LIG_P32(3, 3)
+        default: break;
+    }
+#undef LIG_P32
+}
+
+// GF(2^32) encode passes, quarter by quarter so that a quarter (m x n x 4
+// bytes) stays resident in the last-level cache between its passes.
+static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw_tabs, hipStream_t s) {
+    int ell = 0;
+    while (((size_t)1 << ell) < m) ell++;
+    const int KMAX = 6;
+    const int npass = (ell + KMAX - 1) / KMAX;
+    const size_t nchunks = (n + 63) / 64;
+    const int cc = n < 64 ? (int)n : 64;
+    for (int q = 0; q < 4; q++) {
+        int d0 = 0;
+        for (int p = 0; p < npass; p++) {
+            const int K = ell / npass + (p < ell % npass ? 1 : 0);
+            const int K2 = K / 2;
+            const size_t tiles = (size_t)1 << (ell - K);
+            const uint32_t* src = (p == 0 && msg) ? msg : mat;
+            const int shared = (p == 0 && msg) ? 1 : 0;
+            launch_pass32(K, dim3((unsigned)(tiles * nchunks)), cc << K2, s, src, mat, m, n, ell, d0, q, shared, tw_tabs);
+            d0 += K;
+        }
     }
 }
 
@@ -353,8 +564,28 @@ __global__ void k_fold(const typename F::T* __restrict__ poly, size_t m, int kk,
     }
 }
 
+__global__ __launch_bounds__(256) void k_fold32(const uint32_t* __restrict__ poly, size_t m, int kk,
+                                                 const uint32_t* __restrict__ gr_tabs, uint32_t* __restrict__ out) {
+    __shared__ __attribute__((aligned(256))) uint32_t tabs[64 * 128];
+    const int nc = 1 << kk;
+    for (int idx = threadIdx.x; idx < nc * 128; idx += blockDim.x) tabs[idx] = gr_tabs[idx];
+    __syncthreads();
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t acc = 0;
+#pragma unroll 8
+        for (int j = 0; j < nc; j++) acc ^= mul_tab_lds(tabs, (uint32_t)(j * 512), poly[(size_t)j * m + i]);
+        out[i] = acc;
+    }
+}
+
 template <class F>
 void launch_fold(const typename F::T* poly, size_t m, int kk, const typename F::T* gr_tabs, typename F::T* out, hipStream_t s) {
+    if constexpr (F::BITS == 32) {
+        if (kk <= 6) {
+            hipLaunchKernelGGL(k_fold32, dim3(grid_for(m, 256, 1024)), dim3(256), 0, s, poly, m, kk, gr_tabs, out);
+            return;
+        }
+    }
     hipLaunchKernelGGL(k_fold<F>, dim3(grid_for(m, 256, 4096)), dim3(256), 0, s, poly, m, kk, gr_tabs, out);
 }
 
@@ -431,6 +662,27 @@ __global__ __launch_bounds__(256) void k_induce_accum(const typename F::T* __res
     }
 }
 
+// GF(2^32) variant: all query tables of a workgroup are built once in LDS
+// (up to 160 queries), then each lane streams P over the queries.
+constexpr int INDUCE_MAXQ = 160;
+__global__ __launch_bounds__(256) void k_induce_accum32(const uint32_t* __restrict__ P, const uint32_t* __restrict__ Kt, int nq,
+                                                        int ell, int clo, uint32_t* __restrict__ basis) {
+    __shared__ __attribute__((aligned(256))) uint32_t tabs[INDUCE_MAXQ * 128];
+    const size_t np = (size_t)1 << clo, nk = (size_t)1 << (ell - clo);
+    const size_t X = blockIdx.x;
+    for (int idx = threadIdx.x; idx < nq * 128; idx += blockDim.x) {
+        const int i = idx >> 7, e = idx & 127;
+        tabs[idx] = gf32_table_entry(Kt[(size_t)i * nk + X], e >> 4, (uint32_t)(e & 15));
+    }
+    __syncthreads();
+    for (size_t x = threadIdx.x; x < np; x += blockDim.x) {
+        uint32_t acc = 0;
+#pragma unroll 4
+        for (int i = 0; i < nq; i++) acc ^= mul_tab_lds(tabs, (uint32_t)(i * 512), P[(size_t)i * np + x]);
+        basis[X * np + x] = acc;
+    }
+}
+
 template <class F>
 void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
                    typename F::T* basis, hipStream_t s) {
@@ -439,6 +691,12 @@ void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, 
     if (clo > 10) clo = 10;
     hipLaunchKernelGGL(k_induce_tensor<F>, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K);
     size_t nk = (size_t)1 << (ell - clo);
+    if constexpr (F::BITS == 32) {
+        if (nq <= INDUCE_MAXQ) {
+            hipLaunchKernelGGL(k_induce_accum32, dim3((unsigned)nk), dim3(256), 0, s, P, K, nq, ell, clo, basis);
+            return;
+        }
+    }
     hipLaunchKernelGGL(k_induce_accum<F>, dim3((unsigned)nk), dim3(256), 0, s, P, K, nq, ell, clo, basis);
 }
 
