@@ -87,6 +87,7 @@ __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __res
     __shared__ uint32_t tile[64][65];
     const int rt = m < 64 ? (int)m : 64;
     const int ct = n < 64 ? (int)n : 64;
+    const int lrt = 31 - __clz(rt), lct = 31 - __clz(ct);
     const size_t row_tiles = (m + 63) / 64, col_tiles = (n + 63) / 64;
     for (int idx = threadIdx.x; idx < rt * 128; idx += blockDim.x) tlo[idx] = lo_tabs[idx];
     for (size_t tile_i = blockIdx.x; tile_i < row_tiles * col_tiles; tile_i += gridDim.x) {
@@ -94,12 +95,12 @@ __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __res
         const size_t c0 = (tile_i / row_tiles) * 64;
         if (threadIdx.x < 128) thi[threadIdx.x] = hi_tabs[(i0 >> 6) * 128 + threadIdx.x];
         for (int idx = threadIdx.x; idx < rt * ct; idx += blockDim.x) {
-            int r = idx % rt, c = idx / rt;
+            int r = idx & (rt - 1), c = idx >> lrt;
             tile[c][r] = poly[(c0 + c) * m + i0 + r];
         }
         __syncthreads();
         for (int idx = threadIdx.x; idx < rt * ct; idx += blockDim.x) {
-            int c = idx % ct, r = idx / ct;
+            int c = idx & (ct - 1), r = idx >> lct;
             uint32_t x = mul_tab_lds(tlo, (uint32_t)(r * 512), tile[c][r]);
             x = mul_tab_lds(thi, 0u, x);
             msg[(i0 + r) * n + c0 + c] = x;
@@ -291,100 +292,112 @@ __device__ __forceinline__ uint32_t mul_tab_lds(const uint32_t* tabs, uint32_t t
 // base table and a table shared by every tile.
 template <int K1, int K2>
 __global__ __launch_bounds__(512) void k_fft_pass32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t m,
-                                                    size_t n, int ell, int d0, int q, int src_shared,
+                                                    size_t n, int ell, int d0, int q_arg, int src_shared, int tiles_per_wg,
                                                     const uint32_t* __restrict__ tw_tabs) {
     constexpr int K = K1 + K2, R = 1 << K, NT = R - 1;
     constexpr int L1 = 1 << K1, L2 = 1 << K2;
     __shared__ __attribute__((aligned(256))) uint32_t tabs[NT * 128];
     __shared__ uint32_t xch[K2 > 0 ? R * 64 : 1];
+    const size_t q = q_arg >= 0 ? (size_t)q_arg : (size_t)blockIdx.y;
     const int cc = n < 64 ? (int)n : 64;
-    const size_t nchunks = (n + 63) / 64;
-    const size_t tile_id = blockIdx.x / nchunks;
-    const size_t c0 = (blockIdx.x % nchunks) * 64;
+    const int lgcc = 31 - __clz(cc);
+    const size_t nchunks = (n + 63) >> 6;
     const int sh = ell - d0 - K;
     const size_t S = (size_t)1 << sh;
-    const size_t B = tile_id >> sh, lo = tile_id & (S - 1);
-    const size_t rbase = (B << (ell - d0)) + lo;
     const int tid = threadIdx.x;
     const int c = tid & (cc - 1);
-    const int g = tid / cc;  // < L2
+    const int g = tid >> lgcc;  // < L2
+    const size_t qoff = src_shared ? 0 : q * m;
+    // tiles of one workgroup share B (consecutive lo), hence their tables
+    const size_t first_job = (size_t)blockIdx.x * tiles_per_wg;
+    const size_t B = (first_job / nchunks) >> sh;
 
-#pragma unroll 4
-    for (int idx = tid; idx < NT * 128; idx += blockDim.x) {
-        const int ti = idx >> 7, e = idx & 127;
-        const int j = 31 - __clz(ti + 1);
-        const int tp = ti + 1 - (1 << j);
-        const int d = d0 + j;
-        const size_t first = ((size_t)1 << (d + 2)) - 1;
-        const size_t ib = first + ((size_t)q << d) + (B << j);
-        tabs[idx] = tw_tabs[ib * 128 + e] ^ tw_tabs[(first + tp) * 128 + e];
-    }
-
-    uint32_t v[L1];
-    const size_t qoff = src_shared ? 0 : (size_t)q * m;
-    const uint32_t* sp = src + c0 + c;
+    for (int tj = 0; tj < tiles_per_wg; tj++) {
+        const size_t job = first_job + tj;
+        const size_t tile_id = job / nchunks;
+        const size_t c0 = (job % nchunks) << 6;
+        const size_t lo = tile_id & (S - 1);
+        const size_t rbase = (B << (ell - d0)) + lo;
+        const uint32_t* sp = src + (qoff + rbase) * n + c0 + c;
+        const size_t rs = S * n;  // element stride between tile rows
+        uint32_t v[L1];
 #pragma unroll
-    for (int a = 0; a < L1; a++) v[a] = sp[(qoff + rbase + ((size_t)(a * L2 + g)) * S) * n];
-    __syncthreads();
-
+        for (int a = 0; a < L1; a++) v[a] = sp[(size_t)(a * L2 + g) * rs];
+        if (tj == 0) {
+            // twiddle tables of the tile: stage j, block t' -> base_j ^ local_j[t']
 #pragma unroll
-    for (int j = 0; j < K1; j++) {
-        const int h = L1 >> (j + 1);
-#pragma unroll
-        for (int a = 0; a < L1; a++) {
-            if (a & h) continue;
-            const uint32_t tb = (uint32_t)(((1 << j) - 1 + (a >> (K1 - j))) * 512);
-            const uint32_t u = v[a] ^ mul_tab_lds(tabs, tb, v[a + h]);
-            v[a] = u;
-            v[a + h] ^= u;
+            for (int j = 0; j < K; j++) {
+                const int d = d0 + j;
+                const size_t first = ((size_t)1 << (d + 2)) - 1;
+                const uint32_t* base = tw_tabs + (first + (q << d) + (B << j)) * 128;
+                const uint32_t* local = tw_tabs + first * 128;
+                const int cnt = (1 << j) * 128;
+                for (int idx = tid; idx < cnt; idx += blockDim.x)
+                    tabs[((1 << j) - 1) * 128 + idx] = base[idx & 127] ^ local[idx];
+            }
+        } else {
+            __syncthreads();  // previous tile's exchange reads are done
         }
-    }
-
-    uint32_t* dp = dst + c0 + c;
-    const size_t doff = (size_t)q * m + rbase;
-    if constexpr (K2 == 0) {
-#pragma unroll
-        for (int a = 0; a < L1; a++) dp[(doff + (size_t)a * S) * n] = v[a];
-    } else {
-#pragma unroll
-        for (int a = 0; a < L1; a++) xch[(a * L2 + g) * 64 + c] = v[a];
         __syncthreads();
-        constexpr int NC2 = L1 / L2;
+
 #pragma unroll
-        for (int i = 0; i < NC2; i++) {
-            const int hi = g + L2 * i;
+        for (int j = 0; j < K1; j++) {
+            const int h = L1 >> (j + 1);
 #pragma unroll
-            for (int e = 0; e < L2; e++) v[i * L2 + e] = xch[(hi * L2 + e) * 64 + c];
+            for (int a = 0; a < L1; a++) {
+                if (a & h) continue;
+                const uint32_t tb = (uint32_t)(((1 << j) - 1 + (a >> (K1 - j))) * 512);
+                const uint32_t u = v[a] ^ mul_tab_lds(tabs, tb, v[a + h]);
+                v[a] = u;
+                v[a + h] ^= u;
+            }
         }
+
+        uint32_t* dp = dst + (q * m + rbase) * n + c0 + c;
+        if constexpr (K2 == 0) {
 #pragma unroll
-        for (int j = K1; j < K; j++) {
-            const int h = L2 >> (j - K1 + 1);
+            for (int a = 0; a < L1; a++) dp[(size_t)a * rs] = v[a];
+        } else {
+#pragma unroll
+            for (int a = 0; a < L1; a++) xch[(a * L2 + g) * 64 + c] = v[a];
+            __syncthreads();
+            constexpr int NC2 = L1 / L2;
 #pragma unroll
             for (int i = 0; i < NC2; i++) {
                 const int hi = g + L2 * i;
 #pragma unroll
-                for (int e = 0; e < L2; e++) {
-                    if (e & h) continue;
-                    const uint32_t tb = (uint32_t)(((1 << j) - 1 + (hi << (j - K1)) + (e >> (K - j))) * 512);
-                    const uint32_t u = v[i * L2 + e] ^ mul_tab_lds(tabs, tb, v[i * L2 + e + h]);
-                    v[i * L2 + e] = u;
-                    v[i * L2 + e + h] ^= u;
+                for (int e = 0; e < L2; e++) v[i * L2 + e] = xch[(hi * L2 + e) * 64 + c];
+            }
+#pragma unroll
+            for (int j = K1; j < K; j++) {
+                const int h = L2 >> (j - K1 + 1);
+#pragma unroll
+                for (int i = 0; i < NC2; i++) {
+                    const int hi = g + L2 * i;
+#pragma unroll
+                    for (int e = 0; e < L2; e++) {
+                        if (e & h) continue;
+                        const uint32_t tb = (uint32_t)(((1 << j) - 1 + (hi << (j - K1)) + (e >> (K - j))) * 512);
+                        const uint32_t u = v[i * L2 + e] ^ mul_tab_lds(tabs, tb, v[i * L2 + e + h]);
+                        v[i * L2 + e] = u;
+                        v[i * L2 + e + h] ^= u;
+                    }
                 }
             }
-        }
 #pragma unroll
-        for (int i = 0; i < NC2; i++) {
-            const int hi = g + L2 * i;
+            for (int i = 0; i < NC2; i++) {
+                const int hi = g + L2 * i;
 #pragma unroll
-            for (int e = 0; e < L2; e++) dp[(doff + (size_t)(hi * L2 + e) * S) * n] = v[i * L2 + e];
+                for (int e = 0; e < L2; e++) dp[(size_t)(hi * L2 + e) * rs] = v[i * L2 + e];
+            }
         }
     }
 }
 
 static void launch_pass32(int K, dim3 grid, int block, hipStream_t s, const uint32_t* src, uint32_t* dst, size_t m, size_t n,
-                          int ell, int d0, int q, int shared, const uint32_t* tabs) {
-#define LIG_P32(A, B_)                                                                                                   \
-    hipLaunchKernelGGL((k_fft_pass32<A, B_>), grid, dim3(block), 0, s, src, dst, m, n, ell, d0, q, shared, tabs);       \
+                          int ell, int d0, int q, int shared, int tpw, const uint32_t* tabs) {
+#define LIG_P32(A, B_)                                                                                                    \
+    hipLaunchKernelGGL((k_fft_pass32<A, B_>), grid, dim3(block), 0, s, src, dst, m, n, ell, d0, q, shared, tpw, tabs);   \
     break;
     switch (K) {
         case 1: LIG_P32(1, 0)
@@ -407,15 +420,24 @@ static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t 
     const int npass = (ell + KMAX - 1) / KMAX;
     const size_t nchunks = (n + 63) / 64;
     const int cc = n < 64 ? (int)n : 64;
-    for (int q = 0; q < 4; q++) {
+    // Large quarters run one at a time (a quarter stays cache-resident
+    // between its passes); small ones run all four per launch.
+    const bool by_quarter = m * n * 4 >= ((size_t)16 << 20);
+    for (int qi = 0; qi < (by_quarter ? 4 : 1); qi++) {
         int d0 = 0;
         for (int p = 0; p < npass; p++) {
             const int K = ell / npass + (p < ell % npass ? 1 : 0);
             const int K2 = K / 2;
             const size_t tiles = (size_t)1 << (ell - K);
+            const size_t S = (size_t)1 << (ell - d0 - K);
+            // tiles sharing B (hence tables) are grouped per workgroup
+            size_t jobs = tiles * nchunks;
+            int tpw = 1;
+            while (tpw < 4 && (size_t)(tpw * 2) <= S * nchunks && jobs / (tpw * 2) >= 2048) tpw *= 2;
             const uint32_t* src = (p == 0 && msg) ? msg : mat;
             const int shared = (p == 0 && msg) ? 1 : 0;
-            launch_pass32(K, dim3((unsigned)(tiles * nchunks)), cc << K2, s, src, mat, m, n, ell, d0, q, shared, tw_tabs);
+            dim3 grid((unsigned)(jobs / tpw), by_quarter ? 1 : 4);
+            launch_pass32(K, grid, cc << K2, s, src, mat, m, n, ell, d0, by_quarter ? qi : -1, shared, tpw, tw_tabs);
             d0 += K;
         }
     }
@@ -664,22 +686,31 @@ __global__ __launch_bounds__(256) void k_induce_accum(const typename F::T* __res
 
 // GF(2^32) variant: all query tables of a workgroup are built once in LDS
 // (up to 160 queries), then each lane streams P over the queries.
-constexpr int INDUCE_MAXQ = 160;
+constexpr int INDUCE_QCH = 32;  // queries per workgroup
+__global__ void k_zero_u32(uint32_t* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+// GF(2^32) variant: a workgroup owns one X and a chunk of up to 32 queries;
+// their tables are built once in LDS and the partial sums are XORed into
+// basis atomically (XOR is order-independent).
 __global__ __launch_bounds__(256) void k_induce_accum32(const uint32_t* __restrict__ P, const uint32_t* __restrict__ Kt, int nq,
                                                         int ell, int clo, uint32_t* __restrict__ basis) {
-    __shared__ __attribute__((aligned(256))) uint32_t tabs[INDUCE_MAXQ * 128];
+    __shared__ __attribute__((aligned(256))) uint32_t tabs[INDUCE_QCH * 128];
     const size_t np = (size_t)1 << clo, nk = (size_t)1 << (ell - clo);
     const size_t X = blockIdx.x;
-    for (int idx = threadIdx.x; idx < nq * 128; idx += blockDim.x) {
+    const int q0 = blockIdx.y * INDUCE_QCH;
+    const int qn = nq - q0 < INDUCE_QCH ? nq - q0 : INDUCE_QCH;
+    for (int idx = threadIdx.x; idx < qn * 128; idx += blockDim.x) {
         const int i = idx >> 7, e = idx & 127;
-        tabs[idx] = gf32_table_entry(Kt[(size_t)i * nk + X], e >> 4, (uint32_t)(e & 15));
+        tabs[idx] = gf32_table_entry(Kt[(size_t)(q0 + i) * nk + X], e >> 4, (uint32_t)(e & 15));
     }
     __syncthreads();
     for (size_t x = threadIdx.x; x < np; x += blockDim.x) {
         uint32_t acc = 0;
-#pragma unroll 4
-        for (int i = 0; i < nq; i++) acc ^= mul_tab_lds(tabs, (uint32_t)(i * 512), P[(size_t)i * np + x]);
-        basis[X * np + x] = acc;
+        const uint32_t* Pq = P + (size_t)q0 * np + x;
+#pragma unroll 8
+        for (int i = 0; i < qn; i++) acc ^= mul_tab_lds(tabs, (uint32_t)(i * 512), Pq[(size_t)i * np]);
+        atomicXor(&basis[X * np + x], acc);
     }
 }
 
@@ -692,10 +723,11 @@ void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, 
     hipLaunchKernelGGL(k_induce_tensor<F>, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K);
     size_t nk = (size_t)1 << (ell - clo);
     if constexpr (F::BITS == 32) {
-        if (nq <= INDUCE_MAXQ) {
-            hipLaunchKernelGGL(k_induce_accum32, dim3((unsigned)nk), dim3(256), 0, s, P, K, nq, ell, clo, basis);
-            return;
-        }
+        size_t total = (size_t)1 << ell;
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(total, 256, 2048)), dim3(256), 0, s, basis, total);
+        dim3 grid((unsigned)nk, (unsigned)((nq + INDUCE_QCH - 1) / INDUCE_QCH));
+        hipLaunchKernelGGL(k_induce_accum32, grid, dim3(256), 0, s, P, K, nq, ell, clo, basis);
+        return;
     }
     hipLaunchKernelGGL(k_induce_accum<F>, dim3((unsigned)nk), dim3(256), 0, s, P, K, nq, ell, clo, basis);
 }
