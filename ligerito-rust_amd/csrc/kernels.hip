@@ -94,9 +94,23 @@ __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __res
         const size_t i0 = (tile_i % row_tiles) * 64;
         const size_t c0 = (tile_i / row_tiles) * 64;
         if (threadIdx.x < 128) thi[threadIdx.x] = hi_tabs[(i0 >> 6) * 128 + threadIdx.x];
-        for (int idx = threadIdx.x; idx < rt * ct; idx += blockDim.x) {
-            int r = idx & (rt - 1), c = idx >> lrt;
-            tile[c][r] = poly[(c0 + c) * m + i0 + r];
+        if (rt == 64 && ct == 64) {
+            uint32_t tv[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int idx = threadIdx.x + 256 * k;
+                tv[k] = poly[(c0 + (idx >> 6)) * m + i0 + (idx & 63)];
+            }
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int idx = threadIdx.x + 256 * k;
+                tile[idx >> 6][idx & 63] = tv[k];
+            }
+        } else {
+            for (int idx = threadIdx.x; idx < rt * ct; idx += blockDim.x) {
+                int r = idx & (rt - 1), c = idx >> lrt;
+                tile[c][r] = poly[(c0 + c) * m + i0 + r];
+            }
         }
         __syncthreads();
         for (int idx = threadIdx.x; idx < rt * ct; idx += blockDim.x) {
@@ -324,16 +338,32 @@ __global__ __launch_bounds__(512) void k_fft_pass32(const uint32_t* __restrict__
 #pragma unroll
         for (int a = 0; a < L1; a++) v[a] = sp[(size_t)(a * L2 + g) * rs];
         if (tj == 0) {
-            // twiddle tables of the tile: stage j, block t' -> base_j ^ local_j[t']
+            // Twiddle tables of the tile.  Stage j, block t' < 2^j:
+            //   T = T(base_j) ^ XOR_{i in bits(t')} T(2^i)
+            // (twiddles are GF(2)-linear in the block index).  Work item
+            // (e, grp) owns entry e of the tables with t' = grp mod 4; all its
+            // loads are issued before any LDS store.
+            for (int item = tid; item < 512; item += blockDim.x) {
+                const int e = item & 127, grp = item >> 7;
+                uint32_t bv[K], tb[K * (K - 1) / 2 + 1];
 #pragma unroll
-            for (int j = 0; j < K; j++) {
-                const int d = d0 + j;
-                const size_t first = ((size_t)1 << (d + 2)) - 1;
-                const uint32_t* base = tw_tabs + (first + (q << d) + (B << j)) * 128;
-                const uint32_t* local = tw_tabs + first * 128;
-                const int cnt = (1 << j) * 128;
-                for (int idx = tid; idx < cnt; idx += blockDim.x)
-                    tabs[((1 << j) - 1) * 128 + idx] = base[idx & 127] ^ local[idx];
+                for (int j = 0; j < K; j++) {
+                    const int d = d0 + j;
+                    const size_t first = ((size_t)1 << (d + 2)) - 1;
+                    bv[j] = tw_tabs[(first + (q << d) + (B << j)) * 128 + e];
+#pragma unroll
+                    for (int i = 0; i < j; i++) tb[j * (j - 1) / 2 + i] = tw_tabs[(first + ((size_t)1 << i)) * 128 + e];
+                }
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    for (int tp = grp; tp < (1 << j); tp += 4) {
+                        uint32_t val = bv[j];
+#pragma unroll
+                        for (int i = 0; i < j; i++)
+                            if ((tp >> i) & 1) val ^= tb[j * (j - 1) / 2 + i];
+                        tabs[((1 << j) - 1 + tp) * 128 + e] = val;
+                    }
+                }
             }
         } else {
             __syncthreads();  // previous tile's exchange reads are done
@@ -696,13 +726,16 @@ __global__ void k_zero_u32(uint32_t* p, size_t n) {
 __global__ __launch_bounds__(256) void k_induce_accum32(const uint32_t* __restrict__ P, const uint32_t* __restrict__ Kt, int nq,
                                                         int ell, int clo, uint32_t* __restrict__ basis) {
     __shared__ __attribute__((aligned(256))) uint32_t tabs[INDUCE_QCH * 128];
+    __shared__ uint32_t kv[INDUCE_QCH];
     const size_t np = (size_t)1 << clo, nk = (size_t)1 << (ell - clo);
     const size_t X = blockIdx.x;
     const int q0 = blockIdx.y * INDUCE_QCH;
     const int qn = nq - q0 < INDUCE_QCH ? nq - q0 : INDUCE_QCH;
+    if ((int)threadIdx.x < qn) kv[threadIdx.x] = Kt[(size_t)(q0 + threadIdx.x) * nk + X];
+    __syncthreads();
     for (int idx = threadIdx.x; idx < qn * 128; idx += blockDim.x) {
         const int i = idx >> 7, e = idx & 127;
-        tabs[idx] = gf32_table_entry(Kt[(size_t)(q0 + i) * nk + X], e >> 4, (uint32_t)(e & 15));
+        tabs[idx] = gf32_table_entry(kv[i], e >> 4, (uint32_t)(e & 15));
     }
     __syncthreads();
     for (size_t x = threadIdx.x; x < np; x += blockDim.x) {
