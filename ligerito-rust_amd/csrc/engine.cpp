@@ -182,8 +182,10 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
         launch_merkle_levels(w.tree.p, M, s);
     }
     LIG_HIP(hipGetLastError());
-    LIG_HIP(hipMemcpyAsync(w.root, w.tree.p + (2 * M - 2) * 8, 32, hipMemcpyDeviceToHost, s));
+    uint8_t* pin = ctx->pinned(32);
+    LIG_HIP(hipMemcpyAsync(pin, w.tree.p + (2 * M - 2) * 8, 32, hipMemcpyDeviceToHost, s));
     ctx->sync();
+    memcpy(w.root, pin, 32);
 }
 
 // ----------------------------------------------------------------- config --
@@ -219,81 +221,91 @@ template <class H> typename H::T acc_to_elem(const typename DevField<H>::F::Acc&
 template <> uint32_t acc_to_elem<H32>(const uint64_t& a) { return gf32_reduce64(a); }
 template <> u128v acc_to_elem<H128>(const u128v& a) { return a; }
 
-// Gathers opened rows and the batched Merkle proof of a witness
-// (prover.rs:72-76 / 114-125, merkle_tree.rs:98-111).
-template <class H> void open_rows(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out) {
+// Openings (prover.rs:72-76 / 114-125, merkle_tree.rs:98-111) whose copies
+// to pinned host memory are in flight on the context stream.
+template <class H> struct PendingOpen {
+    Rlp<H>* out = nullptr;
+    PBuf<uint64_t> d_q, d_nodes_idx;
+    PBuf<typename H::T> d_rows;
+    PBuf<uint32_t> d_nodes;
+    size_t n_nodes = 0;
+    uint8_t* h_rows = nullptr;
+    uint8_t* h_nodes = nullptr;
+};
+
+template <class H>
+void open_launch(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, PendingOpen<H>& po, size_t pin_off) {
     using F = typename DevField<H>::F;
     using T = typename H::T;
     Ctx* ctx = w.ctx;
     hipStream_t s = ctx->stream;
     std::vector<uint64_t> nodes = merkle_proof_nodes(w.M, q);
-    std::vector<uint64_t> qi(q.begin(), q.end());
-    PBuf<uint64_t> d_idx(&ctx->pool, qi.size() + nodes.size());
-    LIG_HIP(hipMemcpyAsync(d_idx.p, qi.data(), qi.size() * 8, hipMemcpyHostToDevice, s));
-    if (!nodes.empty())
-        LIG_HIP(hipMemcpyAsync(d_idx.p + qi.size(), nodes.data(), nodes.size() * 8, hipMemcpyHostToDevice, s));
-    PBuf<T> d_rows(&ctx->pool, q.size() * w.n);
-    PBuf<uint32_t> d_nodes(&ctx->pool, std::max<size_t>(nodes.size(), 1) * 8);
+    po.out = &out;
+    po.n_nodes = nodes.size();
+    const size_t rows_bytes = q.size() * w.n * sizeof(T), nodes_bytes = nodes.size() * 32;
+    const size_t idx_bytes = (q.size() + nodes.size()) * 8;
+    uint8_t* pin = ctx->pinned(pin_off + idx_bytes + rows_bytes + nodes_bytes + 256);
+    uint64_t* h_idx = (uint64_t*)(pin + pin_off);
+    for (size_t i = 0; i < q.size(); i++) h_idx[i] = q[i];
+    memcpy(h_idx + q.size(), nodes.data(), nodes.size() * 8);
+    po.d_q = PBuf<uint64_t>(&ctx->pool, q.size() + std::max<size_t>(nodes.size(), 1));
+    LIG_HIP(hipMemcpyAsync(po.d_q.p, h_idx, idx_bytes, hipMemcpyHostToDevice, s));
+    po.d_rows = PBuf<T>(&ctx->pool, q.size() * w.n);
+    po.d_nodes = PBuf<uint32_t>(&ctx->pool, std::max<size_t>(nodes.size(), 1) * 8);
     {
         PhaseTimer pt(ctx, PH_OPEN);
-        launch_gather_rows<F>(w.mat.p, w.n, d_idx.p, q.size(), d_rows.p, s);
-        launch_gather_nodes(w.tree.p, d_idx.p + qi.size(), nodes.size(), d_nodes.p, s);
+        launch_gather_rows<F>(w.mat.p, w.n, po.d_q.p, q.size(), po.d_rows.p, s);
+        launch_gather_nodes(w.tree.p, po.d_q.p + q.size(), nodes.size(), po.d_nodes.p, s);
     }
+    po.h_rows = pin + pin_off + idx_bytes;
+    po.h_nodes = po.h_rows + rows_bytes;
+    LIG_HIP(hipMemcpyAsync(po.h_rows, po.d_rows.p, rows_bytes, hipMemcpyDeviceToHost, s));
+    if (nodes_bytes) LIG_HIP(hipMemcpyAsync(po.h_nodes, po.d_nodes.p, nodes_bytes, hipMemcpyDeviceToHost, s));
     out.n_rows = q.size();
     out.n_cols = w.n;
-    out.rows.resize(q.size() * w.n);
-    out.nodes.resize(nodes.size() * 32);
-    LIG_HIP(hipMemcpyAsync(out.rows.data(), d_rows.p, out.rows.size() * sizeof(T), hipMemcpyDeviceToHost, s));
-    if (!nodes.empty())
-        LIG_HIP(hipMemcpyAsync(out.nodes.data(), d_nodes.p, out.nodes.size(), hipMemcpyDeviceToHost, s));
-    ctx->sync();
 }
 
-// induce_sumcheck_poly (sumcheck_polys.rs:74-160): basis on the device,
-// enforced_sum on the host from the opened rows.
+// Call after the stream has been synchronized.
+template <class H> void open_finish(PendingOpen<H>& po) {
+    Rlp<H>& out = *po.out;
+    out.rows.resize(out.n_rows * out.n_cols);
+    memcpy(out.rows.data(), po.h_rows, out.rows.size() * sizeof(typename H::T));
+    out.nodes.assign(po.h_nodes, po.h_nodes + po.n_nodes * 32);
+}
+
+// induce_sumcheck_poly (sumcheck_polys.rs:74-160) entirely on the device:
+// basis polynomial into d_basis, enforced sum (unreduced) into d_enf.
 template <class H>
-void induce(Ctx* ctx, int ell, const Rlp<H>& rows, const Vec<H>& v_ch, const std::vector<size_t>& q, typename H::T alpha,
-            PBuf<typename H::T>& d_basis, typename H::T& enforced) {
+void induce_launch(Ctx* ctx, int ell, const PendingOpen<H>& po, size_t n_cols, const Vec<H>& gr, const std::vector<size_t>& q,
+                   typename H::T alpha, PBuf<typename H::T>& d_basis, typename DevField<H>::F::Acc* d_enf, size_t pin_off) {
     using F = typename DevField<H>::F;
     using T = typename H::T;
     hipStream_t s = ctx->stream;
     const size_t nq = q.size();
-    LIG_REQUIRE(((size_t)1 << v_ch.size()) == rows.n_cols, LIG_E_ASSERT, "opened row length != 2^k");
-    Vec<H> gr = lagrange_basis<H>(v_ch);
+    LIG_REQUIRE(gr.size() == n_cols, LIG_E_ASSERT, "opened row length != 2^k");
     Vec<H> sks = eval_sk_at_vks<H>((size_t)1 << ell);
-    Vec<H> cvals(nq * ell), ap(nq);
-    enforced = H::zero();
-    T a = H::one();
-    for (size_t i = 0; i < nq; i++) {
-        Vec<H> sx = sks_at<H>(H::from_int(q[i]), sks, ell);
-        for (int t = 0; t < ell; t++) cvals[i * ell + t] = sx[t];
-        ap[i] = a;
-        T dot = H::zero();
-        for (size_t j = 0; j < rows.n_cols; j++) dot ^= H::mul(rows.rows[i * rows.n_cols + j], gr[j]);
-        enforced ^= H::mul(dot, a);
-        a = H::mul(a, alpha);
-    }
+    // one pinned upload: sks | gr | alpha
+    const size_t nvals = sks.size() + gr.size() + 1;
+    LIG_REQUIRE(nvals * sizeof(T) <= 16384 - 256, LIG_E_INVALID, "induce constants exceed the staging region");
+    T* h = (T*)ctx->pinned(pin_off + nvals * sizeof(T));
+    h = (T*)((uint8_t*)h + pin_off);
+    memcpy(h, sks.data(), sks.size() * sizeof(T));
+    memcpy(h + sks.size(), gr.data(), gr.size() * sizeof(T));
+    h[nvals - 1] = alpha;
     int clo = ell - 8;
     if (clo < 8) clo = ell < 8 ? ell : 8;
     if (clo > 10) clo = 10;
-    PBuf<T> d_c(&ctx->pool, nq * ell), d_ap(&ctx->pool, nq);
+    PBuf<T> d_vals(&ctx->pool, nvals), d_c(&ctx->pool, nq * ell), d_ap(&ctx->pool, nq);
     PBuf<T> d_P(&ctx->pool, nq << clo), d_K(&ctx->pool, nq << (ell - clo));
-    LIG_HIP(hipMemcpyAsync(d_c.p, cvals.data(), cvals.size() * sizeof(T), hipMemcpyHostToDevice, s));
-    LIG_HIP(hipMemcpyAsync(d_ap.p, ap.data(), ap.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    LIG_HIP(hipMemcpyAsync(d_vals.p, h, nvals * sizeof(T), hipMemcpyHostToDevice, s));
     d_basis = PBuf<T>(&ctx->pool, (size_t)1 << ell);
     {
         PhaseTimer pt(ctx, PH_INDUCE);
+        launch_induce_prep<F>(po.d_q.p, d_vals.p, d_vals.p + nvals - 1, (int)nq, ell, d_c.p, d_ap.p, s);
+        launch_enforced<F>(po.d_rows.p, (int)nq, (int)n_cols, d_vals.p + sks.size(), d_ap.p, d_enf, s);
         launch_induce<F>(d_c.p, d_ap.p, (int)nq, ell, d_P.p, d_K.p, d_basis.p, s);
     }
     LIG_HIP(hipGetLastError());
-    ctx->sync();  // host vectors above go out of scope
-}
-
-template <class H> void read_triple(Ctx* ctx, typename DevField<H>::F::Acc* d_res, typename H::T t[3]) {
-    typename DevField<H>::F::Acc r[3];
-    LIG_HIP(hipMemcpyAsync(r, d_res, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
-    ctx->sync();
-    for (int i = 0; i < 3; i++) t[i] = acc_to_elem<H>(r[i]);
 }
 
 }  // namespace
@@ -308,6 +320,11 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     Transcript fs(fs_seed);
 
     proof = Proof<H>();
+    // Pinned staging regions (disjoint, so an in-flight copy never reads a
+    // region the host is refilling): [0,256) results, [256,16K) uploads of
+    // induce constants and fold weights, [16K,..) openings and yr.
+    constexpr size_t PIN_RES = 0, PIN_UP = 256, PIN_OPEN = 16384;
+    ctx->pinned((size_t)4 << 20);
     Witness<H> w0;
     commit<H>(*cfg.initial_rs, d_poly, cfg.initial_m, cfg.initial_n, w0, PH_ENCODE0);
     memcpy(proof.initial_root.data(), w0.root, 32);
@@ -315,20 +332,21 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
 
     Vec<H> pe0(cfg.initial_k);
     for (auto& r : pe0) r = fs.get_field<H>();
+    Vec<H> gr0 = lagrange_basis<H>(pe0);
 
     // partial_eval over the top initial_k variables == row combination with gr
     const size_t N = cfg.initial_m * cfg.initial_n;
     size_t flen = N >> cfg.initial_k;
     PBuf<T> d_f(&ctx->pool, flen), d_f2(&ctx->pool, flen / 2);
     {
-        Vec<H> gr = lagrange_basis<H>(pe0);
-        PBuf<T> d_gr(&ctx->pool, gr.size()), d_grt(&ctx->pool, gr.size() * F::TAB);
-        LIG_HIP(hipMemcpyAsync(d_gr.p, gr.data(), gr.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        T* h = (T*)(ctx->pinned(PIN_UP + gr0.size() * sizeof(T)) + PIN_UP);
+        memcpy(h, gr0.data(), gr0.size() * sizeof(T));
+        PBuf<T> d_gr(&ctx->pool, gr0.size()), d_grt(&ctx->pool, gr0.size() * F::TAB);
+        LIG_HIP(hipMemcpyAsync(d_gr.p, h, gr0.size() * sizeof(T), hipMemcpyHostToDevice, s));
         PhaseTimer pt(ctx, PH_FOLD);
-        launch_build_tables<F>(d_gr.p, gr.size(), d_grt.p, s);
+        launch_build_tables<F>(d_gr.p, gr0.size(), d_grt.p, s);
         launch_fold<F>(d_poly, flen, cfg.initial_k, d_grt.p, d_f.p, s);
         LIG_HIP(hipGetLastError());
-        ctx->sync();
     }
 
     Witness<H> wprev;
@@ -340,23 +358,35 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     std::vector<size_t> q = fs.get_distinct_queries(w0.M, S);
     T alpha = fs.get_field<H>();
     int nv = ilog2(flen);
-    open_rows<H>(w0, q, proof.initial);
-    PBuf<T> d_b;
-    T enforced;
-    induce<H>(ctx, nv, proof.initial, pe0, q, alpha, d_b, enforced);
-    w0 = Witness<H>();  // release the first witness
 
-    PBuf<A> d_res(&ctx->pool, 3);
-    PBuf<T> d_b2(&ctx->pool, flen / 2);
+    PBuf<A> d_res(&ctx->pool, 4);  // triple + enforced sum
+    A* h_res = (A*)(ctx->pinned(PIN_RES + 4 * sizeof(A)) + PIN_RES);
+    PBuf<T> d_b, d_b2(&ctx->pool, flen / 2);
     T t[3];
     {
-        PhaseTimer pt(ctx, PH_SUMCHECK);
-        launch_eval013<F>(d_f.p, d_b.p, flen, d_res.p, s);
+        PendingOpen<H> po;
+        open_launch<H>(w0, q, proof.initial, po, PIN_OPEN);
+        induce_launch<H>(ctx, nv, po, w0.n, gr0, q, alpha, d_b, d_res.p + 3, PIN_UP);
+        {
+            PhaseTimer pt(ctx, PH_SUMCHECK);
+            launch_eval013<F>(d_f.p, d_b.p, flen, d_res.p, s);
+        }
+        LIG_HIP(hipMemcpyAsync(h_res, d_res.p, 4 * sizeof(A), hipMemcpyDeviceToHost, s));
+        ctx->sync();
+        open_finish<H>(po);
     }
-    read_triple<H>(ctx, d_res.p, t);
+    for (int i = 0; i < 3; i++) t[i] = acc_to_elem<H>(h_res[i]);
+    T enforced = acc_to_elem<H>(h_res[3]);
+    w0 = Witness<H>();  // release the first witness
     LIG_REQUIRE(H::eq(t[0] ^ t[1], enforced), LIG_E_ASSERT, "sumcheck: s0 + s1 != enforced sum (ligerito.rs:35)");
     proof.tr.insert(proof.tr.end(), t, t + 3);
     fs.absorb_elems<H>(t, 3);
+
+    auto read_triple = [&]() {
+        LIG_HIP(hipMemcpyAsync(h_res, d_res.p, 3 * sizeof(A), hipMemcpyDeviceToHost, s));
+        ctx->sync();
+        for (int k = 0; k < 3; k++) t[k] = acc_to_elem<H>(h_res[k]);
+    };
 
     for (int i = 0; i < cfg.recursive_steps; i++) {
         Vec<H> rs;
@@ -367,7 +397,7 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
                 launch_sumcheck_fold<F>(d_f.p, d_b.p, flen, r, d_f2.p, d_b2.p, d_res.p, s);
             }
             LIG_HIP(hipGetLastError());
-            read_triple<H>(ctx, d_res.p, t);
+            read_triple();
             std::swap(d_f, d_f2);
             std::swap(d_b, d_b2);
             flen /= 2;
@@ -376,12 +406,16 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             rs.push_back(r);
         }
         if (i == cfg.recursive_steps - 1) {
-            proof.yr.resize(flen);
-            LIG_HIP(hipMemcpyAsync(proof.yr.data(), d_f.p, flen * sizeof(T), hipMemcpyDeviceToHost, s));
+            T* hy = (T*)(ctx->pinned(PIN_OPEN + flen * sizeof(T)) + PIN_OPEN);
+            LIG_HIP(hipMemcpyAsync(hy, d_f.p, flen * sizeof(T), hipMemcpyDeviceToHost, s));
             ctx->sync();
+            proof.yr.assign(hy, hy + flen);
             fs.absorb_elems<H>(proof.yr.data(), flen);
             q = fs.get_distinct_queries(wprev.M, S);
-            open_rows<H>(wprev, q, proof.final_);
+            PendingOpen<H> po;
+            open_launch<H>(wprev, q, proof.final_, po, PIN_OPEN);
+            ctx->sync();
+            open_finish<H>(po);
             return;
         }
         Witness<H> wnext;
@@ -394,16 +428,23 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         T a = fs.get_field<H>();
         nv = ilog2(flen);
         proof.rec_proofs.emplace_back();
-        open_rows<H>(wprev, q, proof.rec_proofs.back());
         PBuf<T> d_bnew;
-        T enf;
-        induce<H>(ctx, nv, proof.rec_proofs.back(), rs, q, a, d_bnew, enf);
-        // introduce_new (ligerito.rs:50-57): triple of f with the new basis alone
         {
-            PhaseTimer pt(ctx, PH_SUMCHECK);
-            launch_eval013<F>(d_f.p, d_bnew.p, flen, d_res.p, s);
+            PendingOpen<H> po;
+            open_launch<H>(wprev, q, proof.rec_proofs.back(), po, PIN_OPEN);
+            Vec<H> gr = lagrange_basis<H>(rs);
+            induce_launch<H>(ctx, nv, po, wprev.n, gr, q, a, d_bnew, d_res.p + 3, PIN_UP);
+            // introduce_new (ligerito.rs:50-57): triple of f with the new basis alone
+            {
+                PhaseTimer pt(ctx, PH_SUMCHECK);
+                launch_eval013<F>(d_f.p, d_bnew.p, flen, d_res.p, s);
+            }
+            LIG_HIP(hipMemcpyAsync(h_res, d_res.p, 4 * sizeof(A), hipMemcpyDeviceToHost, s));
+            ctx->sync();
+            open_finish<H>(po);
         }
-        read_triple<H>(ctx, d_res.p, t);
+        for (int k = 0; k < 3; k++) t[k] = acc_to_elem<H>(h_res[k]);
+        T enf = acc_to_elem<H>(h_res[3]);
         LIG_REQUIRE(H::eq(t[0] ^ t[1], enf), LIG_E_ASSERT, "sumcheck: s0 + s1 != h (ligerito.rs:52)");
         proof.tr.insert(proof.tr.end(), t, t + 3);
         fs.absorb_elems<H>(t, 3);

@@ -2,6 +2,7 @@
 // allocator, Reed–Solomon plans, device-resident witnesses, the GPU prover
 // and the proof object.
 #pragma once
+#include <algorithm>
 #include <array>
 #include <map>
 #include <memory>
@@ -40,9 +41,27 @@ struct Ctx {
     std::vector<Mark> marks;
     double phase_ms[PH_COUNT] = {};
     uint64_t phase_n[PH_COUNT] = {};
+    // Pinned host staging for small transfers (grown on demand; contents
+    // are only valid until the next call that uses it).
+    uint8_t* pin = nullptr;
+    size_t pin_cap = 0;
+    uint8_t* pinned(size_t bytes) {
+        if (bytes > pin_cap) {
+            sync();
+            if (pin) (void)hipHostFree(pin);
+            pin = nullptr;
+            size_t cap = std::max<size_t>(bytes, (size_t)1 << 20);
+            LIG_HIP(hipHostMalloc((void**)&pin, cap, hipHostMallocDefault));
+            pin_cap = cap;
+        }
+        return pin;
+    }
     void sync() { LIG_HIP(hipStreamSynchronize(stream)); }
     // Resolves recorded event pairs into per-phase totals.
     void collect();
+    ~Ctx() {
+        if (pin) (void)hipHostFree(pin);
+    }
 };
 
 // Records a HIP event pair on the context stream around a scope.

@@ -18,6 +18,7 @@ template <> struct Ops<F32> {
 
     __device__ static T zero() { return 0u; }
     __device__ static T one() { return 1u; }
+    __device__ static T from_u64(uint64_t v) { return (uint32_t)v; }
     __device__ static T table_entry(T lam, int e) { return gf32_table_entry(lam, e >> 4, (uint32_t)(e & 15)); }
     __device__ static T mul_tab(const T* __restrict__ t, T w) {
         T r0 = t[w & 15u] ^ t[16 + ((w >> 4) & 15u)];
@@ -39,6 +40,7 @@ template <> struct Ops<F128> {
 
     __device__ static T zero() { return {0ull, 0ull}; }
     __device__ static T one() { return {1ull, 0ull}; }
+    __device__ static T from_u64(uint64_t v) { return {v, 0ull}; }
     __device__ static T table_entry(T lam, int e) { return gf128_table_entry(lam, e >> 4, (uint32_t)(e & 15)); }
     __device__ static T mul_tab(const T* __restrict__ t, T w) {
         T r = {0ull, 0ull};

@@ -1,5 +1,9 @@
 // Host utilities: error strings, SHA-256, Fiat–Shamir transcript, BLAKE3 over
 // byte strings, Merkle proof/verify logic.
+#include <cpuid.h>
+#include <stdlib.h>
+#include <immintrin.h>
+
 #include <algorithm>
 #include <array>
 #include <unordered_set>
@@ -44,6 +48,63 @@ void sha_block(uint32_t st[8], const uint8_t* p) {
     }
     for (int i = 0; i < 8; i++) st[i] += v[i];
 }
+// SHA-256 block function with the x86 SHA extensions (same FIPS 180-4
+// compression, four rounds per sha256rnds2 pair).
+__attribute__((target("sha,sse4.1,ssse3"))) void sha_blocks_ni(uint32_t st[8], const uint8_t* p, size_t nblocks) {
+    const __m128i MASK = _mm_set_epi64x(0x0c0d0e0f08090a0bULL, 0x0405060700010203ULL);
+    __m128i tmp = _mm_loadu_si128((const __m128i*)&st[0]);
+    __m128i s1 = _mm_loadu_si128((const __m128i*)&st[4]);
+    tmp = _mm_shuffle_epi32(tmp, 0xB1);
+    s1 = _mm_shuffle_epi32(s1, 0x1B);
+    __m128i s0 = _mm_alignr_epi8(tmp, s1, 8);
+    s1 = _mm_blend_epi16(s1, tmp, 0xF0);
+    while (nblocks--) {
+        const __m128i save0 = s0, save1 = s1;
+        __m128i w[4];
+        for (int g = 0; g < 16; g++) {
+            __m128i wg;
+            if (g < 4) {
+                wg = _mm_shuffle_epi8(_mm_loadu_si128((const __m128i*)(p + 16 * g)), MASK);
+            } else {
+                __m128i t = _mm_sha256msg1_epu32(w[(g - 4) & 3], w[(g - 3) & 3]);
+                t = _mm_add_epi32(t, _mm_alignr_epi8(w[(g - 1) & 3], w[(g - 2) & 3], 4));
+                wg = _mm_sha256msg2_epu32(t, w[(g - 1) & 3]);
+            }
+            w[g & 3] = wg;
+            __m128i m = _mm_add_epi32(wg, _mm_loadu_si128((const __m128i*)&kSha[4 * g]));
+            s1 = _mm_sha256rnds2_epu32(s1, s0, m);
+            m = _mm_shuffle_epi32(m, 0x0E);
+            s0 = _mm_sha256rnds2_epu32(s0, s1, m);
+        }
+        s0 = _mm_add_epi32(s0, save0);
+        s1 = _mm_add_epi32(s1, save1);
+        p += 64;
+    }
+    tmp = _mm_shuffle_epi32(s0, 0x1B);
+    s1 = _mm_shuffle_epi32(s1, 0xB1);
+    s0 = _mm_blend_epi16(tmp, s1, 0xF0);
+    s1 = _mm_alignr_epi8(s1, tmp, 8);
+    _mm_storeu_si128((__m128i*)&st[0], s0);
+    _mm_storeu_si128((__m128i*)&st[4], s1);
+}
+
+bool have_sha_ni() {
+    static int cached = -1;
+    if (cached < 0) {
+        unsigned a = 0, b = 0, c = 0, d = 0;
+        cached = (__get_cpuid_count(7, 0, &a, &b, &c, &d) && (b & (1u << 29))) ? 1 : 0;
+        if (getenv("LIG_NO_SHA_NI")) cached = 0;
+    }
+    return cached == 1;
+}
+
+void compress(uint32_t st[8], const uint8_t* p, size_t nblocks) {
+    if (have_sha_ni()) {
+        sha_blocks_ni(st, p, nblocks);
+        return;
+    }
+    for (size_t i = 0; i < nblocks; i++) sha_block(st, p + 64 * i);
+}
 }  // namespace
 
 Sha256::Sha256() {
@@ -62,14 +123,14 @@ void Sha256::update(const void* data, size_t len) {
         p += t;
         len -= t;
         if (blen == 64) {
-            sha_block(h, buf);
+            compress(h, buf, 1);
             blen = 0;
         }
     }
-    while (len >= 64) {
-        sha_block(h, p);
-        p += 64;
-        len -= 64;
+    if (len >= 64) {
+        compress(h, p, len / 64);
+        p += 64 * (len / 64);
+        len %= 64;
     }
     if (len) {
         memcpy(buf, p, len);

@@ -765,6 +765,63 @@ void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, 
     hipLaunchKernelGGL(k_induce_accum<F>, dim3((unsigned)nk), dim3(256), 0, s, P, K, nq, ell, clo, basis);
 }
 
+template <class F>
+__global__ void k_induce_prep(const uint64_t* __restrict__ q, const typename F::T* __restrict__ sks,
+                              const typename F::T* __restrict__ alpha, int nq, int ell, typename F::T* __restrict__ cvals,
+                              typename F::T* __restrict__ ap) {
+    using T = typename F::T;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    T x = Ops<F>::from_u64(q[i]);
+    T* c = cvals + (size_t)i * ell;
+    c[0] = x;
+    for (int t = 1; t < ell; t++) {
+        x = Ops<F>::mul(x, x) ^ Ops<F>::mul(sks[t - 1], x);  // next_s (binary_fft.rs:100-107)
+        c[t] = x;
+    }
+    // alpha^i by square-and-multiply
+    T r = Ops<F>::one(), b = alpha[0];
+    for (int e = i; e; e >>= 1) {
+        if (e & 1) r = Ops<F>::mul(r, b);
+        b = Ops<F>::mul(b, b);
+    }
+    ap[i] = r;
+}
+
+template <class F>
+void launch_induce_prep(const uint64_t* q, const typename F::T* sks, const typename F::T* alpha, int nq, int ell,
+                        typename F::T* cvals, typename F::T* ap, hipStream_t s) {
+    hipLaunchKernelGGL(k_induce_prep<F>, dim3((nq + 63) / 64), dim3(64), 0, s, q, sks, alpha, nq, ell, cvals, ap);
+}
+
+template <class F>
+__global__ __launch_bounds__(256) void k_enforced(const typename F::T* __restrict__ rows, int nrows, int ncols,
+                                                  const typename F::T* __restrict__ gr, const typename F::T* __restrict__ ap,
+                                                  typename F::Acc* __restrict__ out) {
+    using T = typename F::T;
+    using A = typename F::Acc;
+    __shared__ A red[256];
+    A acc = Ops<F>::acc_zero();
+    for (int i = threadIdx.x; i < nrows; i += blockDim.x) {
+        T dot = Ops<F>::zero();
+        for (int j = 0; j < ncols; j++) dot ^= Ops<F>::mul(rows[(size_t)i * ncols + j], gr[j]);
+        Ops<F>::acc_mul(acc, dot, ap[i]);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] ^= red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+template <class F>
+void launch_enforced(const typename F::T* rows, int nrows, int ncols, const typename F::T* gr, const typename F::T* ap,
+                     typename F::Acc* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_enforced<F>, dim3(1), dim3(256), 0, s, rows, nrows, ncols, gr, ap, out);
+}
+
 // -------------------------------------------------------------- sumcheck ---
 template <class F> __device__ inline void acc_atomic_xor(typename F::Acc* dst, typename F::Acc v);
 template <> __device__ inline void acc_atomic_xor<F32>(uint64_t* dst, uint64_t v) {
@@ -901,7 +958,9 @@ template <class F> void launch_glue(typename F::T* bsum, const typename F::T* bn
     template void launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t);                   \
     template void launch_eval013<F>(const F::T*, const F::T*, size_t, F::Acc*, hipStream_t);                                \
     template void launch_sumcheck_fold<F>(const F::T*, const F::T*, size_t, F::T, F::T*, F::T*, F::Acc*, hipStream_t);      \
-    template void launch_glue<F>(F::T*, const F::T*, size_t, F::T, hipStream_t);
+    template void launch_glue<F>(F::T*, const F::T*, size_t, F::T, hipStream_t);                                           \
+    template void launch_induce_prep<F>(const uint64_t*, const F::T*, const F::T*, int, int, F::T*, F::T*, hipStream_t);   \
+    template void launch_enforced<F>(const F::T*, int, int, const F::T*, const F::T*, F::Acc*, hipStream_t);
 
 LIG_INST(F32)
 LIG_INST(F128)

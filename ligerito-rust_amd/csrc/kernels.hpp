@@ -78,6 +78,17 @@ template <class F>
 void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
                    typename F::T* basis, hipStream_t s);
 
+// cvals[i*ell + t] = W_t(q_i) (the sks_at_x recursion, binary_fft.rs:289-292)
+// and ap[i] = alpha^i for the nq queries.
+template <class F>
+void launch_induce_prep(const uint64_t* q, const typename F::T* sks, const typename F::T* alpha, int nq, int ell,
+                        typename F::T* cvals, typename F::T* ap, hipStream_t s);
+// out[0] = sum_i ap[i] * (row_i . gr) (unreduced), the enforced sum of
+// sumcheck_polys.rs:249-256.
+template <class F>
+void launch_enforced(const typename F::T* rows, int nrows, int ncols, const typename F::T* gr, const typename F::T* ap,
+                     typename F::Acc* out, hipStream_t s);
+
 // ------------------------------------------------------------ sumcheck ----
 // res[0..3) = (sum f0*g0, sum f1*g1, sum f2*g2) unreduced (eval_013_product).
 template <class F>
