@@ -359,20 +359,31 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     T alpha = fs.get_field<H>();
     int nv = ilog2(flen);
 
-    PBuf<A> d_res(&ctx->pool, 4);  // triple + enforced sum
+    // two triple accumulators used alternately (each kernel clears the other)
+    // and the enforced sum
+    PBuf<A> d_acc(&ctx->pool, 8);
+    A* res_cur = d_acc.p;
+    A* res_next = d_acc.p + 3;
+    A* d_enf = d_acc.p + 6;
+    launch_zero_acc<F>(res_cur, s);
     A* h_res = (A*)(ctx->pinned(PIN_RES + 4 * sizeof(A)) + PIN_RES);
+    auto fetch = [&](bool with_enf) {
+        LIG_HIP(hipMemcpyAsync(h_res, res_cur, 3 * sizeof(A), hipMemcpyDeviceToHost, s));
+        if (with_enf) LIG_HIP(hipMemcpyAsync(h_res + 3, d_enf, sizeof(A), hipMemcpyDeviceToHost, s));
+    };
     PBuf<T> d_b, d_b2(&ctx->pool, flen / 2);
     T t[3];
     {
         PendingOpen<H> po;
         open_launch<H>(w0, q, proof.initial, po, PIN_OPEN);
-        induce_launch<H>(ctx, nv, po, w0.n, gr0, q, alpha, d_b, d_res.p + 3, PIN_UP);
+        induce_launch<H>(ctx, nv, po, w0.n, gr0, q, alpha, d_b, d_enf, PIN_UP);
         {
             PhaseTimer pt(ctx, PH_SUMCHECK);
-            launch_eval013<F>(d_f.p, d_b.p, flen, d_res.p, s);
+            launch_eval013<F>(d_f.p, d_b.p, flen, res_cur, res_next, s);
         }
-        LIG_HIP(hipMemcpyAsync(h_res, d_res.p, 4 * sizeof(A), hipMemcpyDeviceToHost, s));
+        fetch(true);
         ctx->sync();
+        std::swap(res_cur, res_next);
         open_finish<H>(po);
     }
     for (int i = 0; i < 3; i++) t[i] = acc_to_elem<H>(h_res[i]);
@@ -383,8 +394,9 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     fs.absorb_elems<H>(t, 3);
 
     auto read_triple = [&]() {
-        LIG_HIP(hipMemcpyAsync(h_res, d_res.p, 3 * sizeof(A), hipMemcpyDeviceToHost, s));
+        fetch(false);
         ctx->sync();
+        std::swap(res_cur, res_next);
         for (int k = 0; k < 3; k++) t[k] = acc_to_elem<H>(h_res[k]);
     };
 
@@ -394,7 +406,7 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             T r = fs.get_field<H>();
             {
                 PhaseTimer pt(ctx, PH_SUMCHECK);
-                launch_sumcheck_fold<F>(d_f.p, d_b.p, flen, r, d_f2.p, d_b2.p, d_res.p, s);
+                launch_sumcheck_fold<F>(d_f.p, d_b.p, flen, r, d_f2.p, d_b2.p, res_cur, res_next, s);
             }
             LIG_HIP(hipGetLastError());
             read_triple();
@@ -433,14 +445,15 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             PendingOpen<H> po;
             open_launch<H>(wprev, q, proof.rec_proofs.back(), po, PIN_OPEN);
             Vec<H> gr = lagrange_basis<H>(rs);
-            induce_launch<H>(ctx, nv, po, wprev.n, gr, q, a, d_bnew, d_res.p + 3, PIN_UP);
+            induce_launch<H>(ctx, nv, po, wprev.n, gr, q, a, d_bnew, d_enf, PIN_UP);
             // introduce_new (ligerito.rs:50-57): triple of f with the new basis alone
             {
                 PhaseTimer pt(ctx, PH_SUMCHECK);
-                launch_eval013<F>(d_f.p, d_bnew.p, flen, d_res.p, s);
+                launch_eval013<F>(d_f.p, d_bnew.p, flen, res_cur, res_next, s);
             }
-            LIG_HIP(hipMemcpyAsync(h_res, d_res.p, 4 * sizeof(A), hipMemcpyDeviceToHost, s));
+            fetch(true);
             ctx->sync();
+            std::swap(res_cur, res_next);
             open_finish<H>(po);
         }
         for (int k = 0; k < 3; k++) t[k] = acc_to_elem<H>(h_res[k]);

@@ -75,6 +75,26 @@ __global__ __launch_bounds__(256) void k_transpose_scale(const typename F::T* __
     }
 }
 
+// Largest log2 row length the GF(2^32) induce kernels hold tables for.
+constexpr int PREP_MAXELL = 26;
+
+// Copies n words (n <= PER * blockDim.x) from global memory into LDS with
+// every load of a thread issued before its stores.
+template <int PER>
+__device__ __forceinline__ void stage_words(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, int n) {
+    uint32_t v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = threadIdx.x + k * blockDim.x;
+        v[k] = i < n ? src[i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = threadIdx.x + k * blockDim.x;
+        if (i < n) dst[i] = v[k];
+    }
+}
+
 __device__ __forceinline__ uint32_t mul_tab_lds(const uint32_t* tabs, uint32_t tb, uint32_t w);
 
 // GF(2^32) variant: the 64 low-part pis tables stay in LDS for the whole
@@ -89,7 +109,7 @@ __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __res
     const int ct = n < 64 ? (int)n : 64;
     const int lrt = 31 - __clz(rt), lct = 31 - __clz(ct);
     const size_t row_tiles = (m + 63) / 64, col_tiles = (n + 63) / 64;
-    for (int idx = threadIdx.x; idx < rt * 128; idx += blockDim.x) tlo[idx] = lo_tabs[idx];
+    stage_words<32>(tlo, lo_tabs, rt * 128);
     for (size_t tile_i = blockIdx.x; tile_i < row_tiles * col_tiles; tile_i += gridDim.x) {
         const size_t i0 = (tile_i % row_tiles) * 64;
         const size_t c0 = (tile_i / row_tiles) * 64;
@@ -620,7 +640,7 @@ __global__ __launch_bounds__(256) void k_fold32(const uint32_t* __restrict__ pol
                                                  const uint32_t* __restrict__ gr_tabs, uint32_t* __restrict__ out) {
     __shared__ __attribute__((aligned(256))) uint32_t tabs[64 * 128];
     const int nc = 1 << kk;
-    for (int idx = threadIdx.x; idx < nc * 128; idx += blockDim.x) tabs[idx] = gr_tabs[idx];
+    stage_words<32>(tabs, gr_tabs, nc * 128);
     __syncthreads();
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) {
         uint32_t acc = 0;
@@ -747,14 +767,55 @@ __global__ __launch_bounds__(256) void k_induce_accum32(const uint32_t* __restri
     }
 }
 
+// GF(2^32) variant: P and K by doubling, P[x + 2^t] = P[x] * c_t, with
+// the per-query tables of every c_t in LDS.
+constexpr int TENSOR_MAXK = 4096;
+__global__ __launch_bounds__(256) void k_induce_tensor32(const uint32_t* __restrict__ cvals, const uint32_t* __restrict__ ap,
+                                                         int ell, int clo, uint32_t* __restrict__ P, uint32_t* __restrict__ Kt) {
+    __shared__ __attribute__((aligned(256))) uint32_t tabs[PREP_MAXELL * 128];
+    __shared__ uint32_t pb[1024];
+    __shared__ uint32_t kb[TENSOR_MAXK];
+    const int i = blockIdx.x;
+    const uint32_t* c = cvals + (size_t)i * ell;
+    for (int idx = threadIdx.x; idx < ell * 128; idx += blockDim.x) {
+        const int e = idx & 127;
+        tabs[idx] = gf32_table_entry(c[idx >> 7], e >> 4, (uint32_t)(e & 15));
+    }
+    if (threadIdx.x == 0) {
+        pb[0] = 1u;
+        kb[0] = ap[i];
+    }
+    __syncthreads();
+    for (int t = 0; t < clo; t++) {
+        for (int x = threadIdx.x; x < (1 << t); x += blockDim.x) pb[x + (1 << t)] = mul_tab_lds(tabs, (uint32_t)(t * 512), pb[x]);
+        __syncthreads();
+    }
+    const int hk = ell - clo;
+    for (int t = 0; t < hk; t++) {
+        for (int x = threadIdx.x; x < (1 << t); x += blockDim.x)
+            kb[x + (1 << t)] = mul_tab_lds(tabs, (uint32_t)((clo + t) * 512), kb[x]);
+        __syncthreads();
+    }
+    const size_t np = (size_t)1 << clo, nk = (size_t)1 << hk;
+    for (size_t x = threadIdx.x; x < np; x += blockDim.x) P[(size_t)i * np + x] = pb[x];
+    for (size_t x = threadIdx.x; x < nk; x += blockDim.x) Kt[(size_t)i * nk + x] = kb[x];
+}
+
 template <class F>
 void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
                    typename F::T* basis, hipStream_t s) {
     int clo = ell - 8;
     if (clo < 8) clo = ell < 8 ? ell : 8;
     if (clo > 10) clo = 10;
-    hipLaunchKernelGGL(k_induce_tensor<F>, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K);
     size_t nk = (size_t)1 << (ell - clo);
+    if constexpr (F::BITS == 32) {
+        if (ell <= PREP_MAXELL && nk <= (size_t)TENSOR_MAXK && clo <= 10)
+            hipLaunchKernelGGL(k_induce_tensor32, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K);
+        else
+            hipLaunchKernelGGL(k_induce_tensor<F>, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K);
+    } else {
+        hipLaunchKernelGGL(k_induce_tensor<F>, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K);
+    }
     if constexpr (F::BITS == 32) {
         size_t total = (size_t)1 << ell;
         hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(total, 256, 2048)), dim3(256), 0, s, basis, total);
@@ -788,24 +849,77 @@ __global__ void k_induce_prep(const uint64_t* __restrict__ q, const typename F::
     ap[i] = r;
 }
 
+// GF(2^32) variant: squaring and multiplication by each sks[t] and each
+// alpha^(2^k) are table multiplies (squaring is GF(2)-linear).
+__global__ __launch_bounds__(256) void k_induce_prep32(const uint64_t* __restrict__ q, const uint32_t* __restrict__ sks,
+                                                       const uint32_t* __restrict__ alpha, int nq, int ell,
+                                                       uint32_t* __restrict__ cvals, uint32_t* __restrict__ ap) {
+    // table 0: squaring; 1..ell-1: times sks[t-1]; ell..ell+7: times alpha^(2^k)
+    __shared__ __attribute__((aligned(256))) uint32_t tabs[(PREP_MAXELL + 8) * 128];
+    __shared__ uint32_t apow[8];
+    if (threadIdx.x == 0) {
+        uint32_t a = alpha[0];
+        for (int k = 0; k < 8; k++) {
+            apow[k] = a;
+            a = gf32_reduce64(clmul32_portable(a, a));
+        }
+    }
+    __syncthreads();
+    const int ntab = ell + 8;
+    for (int idx = threadIdx.x; idx < ntab * 128; idx += blockDim.x) {
+        const int tbl = idx >> 7, e = idx & 127, k = e >> 4;
+        const uint32_t v = (uint32_t)(e & 15);
+        uint32_t val;
+        if (tbl == 0) {
+            const uint32_t y = v << (4 * k);
+            val = gf32_reduce64(clmul32_portable(y, y));
+        } else if (tbl < ell) {
+            val = gf32_table_entry(sks[tbl - 1], k, v);
+        } else {
+            val = gf32_table_entry(apow[tbl - ell], k, v);
+        }
+        tabs[idx] = val;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nq; i += blockDim.x) {
+        uint32_t x = (uint32_t)q[i];
+        uint32_t* c = cvals + (size_t)i * ell;
+        c[0] = x;
+        for (int t = 1; t < ell; t++) {
+            x = mul_tab_lds(tabs, 0u, x) ^ mul_tab_lds(tabs, (uint32_t)(t * 512), x);  // next_s
+            c[t] = x;
+        }
+        uint32_t r = 1u;
+        for (int k = 0; k < 8; k++)
+            if ((i >> k) & 1) r = mul_tab_lds(tabs, (uint32_t)((ell + k) * 512), r);
+        ap[i] = r;
+    }
+}
+
 template <class F>
 void launch_induce_prep(const uint64_t* q, const typename F::T* sks, const typename F::T* alpha, int nq, int ell,
                         typename F::T* cvals, typename F::T* ap, hipStream_t s) {
+    if constexpr (F::BITS == 32) {
+        if (nq <= 256 && ell <= PREP_MAXELL) {
+            hipLaunchKernelGGL(k_induce_prep32, dim3(1), dim3(256), 0, s, q, sks, alpha, nq, ell, cvals, ap);
+            return;
+        }
+    }
     hipLaunchKernelGGL(k_induce_prep<F>, dim3((nq + 63) / 64), dim3(64), 0, s, q, sks, alpha, nq, ell, cvals, ap);
 }
 
 template <class F>
-__global__ __launch_bounds__(256) void k_enforced(const typename F::T* __restrict__ rows, int nrows, int ncols,
-                                                  const typename F::T* __restrict__ gr, const typename F::T* __restrict__ ap,
-                                                  typename F::Acc* __restrict__ out) {
+__global__ __launch_bounds__(1024) void k_enforced(const typename F::T* __restrict__ rows, int nrows, int ncols,
+                                                   const typename F::T* __restrict__ gr, const typename F::T* __restrict__ ap,
+                                                   typename F::Acc* __restrict__ out) {
     using T = typename F::T;
     using A = typename F::Acc;
-    __shared__ A red[256];
+    __shared__ A red[1024];
+    const int lg = 31 - __clz(ncols);
     A acc = Ops<F>::acc_zero();
-    for (int i = threadIdx.x; i < nrows; i += blockDim.x) {
-        T dot = Ops<F>::zero();
-        for (int j = 0; j < ncols; j++) dot ^= Ops<F>::mul(rows[(size_t)i * ncols + j], gr[j]);
-        Ops<F>::acc_mul(acc, dot, ap[i]);
+    for (int p = threadIdx.x; p < nrows * ncols; p += blockDim.x) {
+        const T prod = Ops<F>::mul(rows[p], gr[p & (ncols - 1)]);
+        Ops<F>::acc_mul(acc, prod, ap[p >> lg]);
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -819,7 +933,7 @@ __global__ __launch_bounds__(256) void k_enforced(const typename F::T* __restric
 template <class F>
 void launch_enforced(const typename F::T* rows, int nrows, int ncols, const typename F::T* gr, const typename F::T* ap,
                      typename F::Acc* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_enforced<F>, dim3(1), dim3(256), 0, s, rows, nrows, ncols, gr, ap, out);
+    hipLaunchKernelGGL(k_enforced<F>, dim3(1), dim3(1024), 0, s, rows, nrows, ncols, gr, ap, out);
 }
 
 // -------------------------------------------------------------- sumcheck ---
@@ -862,9 +976,10 @@ template <class F> __global__ void k_zero_acc(typename F::Acc* res) {
 
 template <class F>
 __global__ __launch_bounds__(256) void k_eval013(const typename F::T* __restrict__ f, const typename F::T* __restrict__ g,
-                                                 size_t len, typename F::Acc* res) {
+                                                 size_t len, typename F::Acc* res, typename F::Acc* res_clear) {
     using T = typename F::T;
     using A = typename F::Acc;
+    if (blockIdx.x == 0 && threadIdx.x < 3) res_clear[threadIdx.x] = A{};
     const size_t h = len / 2;
     A s0 = Ops<F>::acc_zero(), s1 = Ops<F>::acc_zero(), s2 = Ops<F>::acc_zero();
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < h; i += (size_t)gridDim.x * blockDim.x) {
@@ -879,17 +994,23 @@ __global__ __launch_bounds__(256) void k_eval013(const typename F::T* __restrict
 }
 
 template <class F>
-void launch_eval013(const typename F::T* f, const typename F::T* g, size_t len, typename F::Acc* res, hipStream_t s) {
+void launch_eval013(const typename F::T* f, const typename F::T* g, size_t len, typename F::Acc* res,
+                    typename F::Acc* res_clear, hipStream_t s) {
+    hipLaunchKernelGGL(k_eval013<F>, dim3(grid_for(len / 2, 256, 1024)), dim3(256), 0, s, f, g, len, res, res_clear);
+}
+
+template <class F> void launch_zero_acc(typename F::Acc* res, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_acc<F>, dim3(1), dim3(64), 0, s, res);
-    hipLaunchKernelGGL(k_eval013<F>, dim3(grid_for(len / 2, 256, 1024)), dim3(256), 0, s, f, g, len, res);
 }
 
 template <class F>
 __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __restrict__ f, const typename F::T* __restrict__ b,
                                                        size_t len, typename F::T r, typename F::T* __restrict__ f2,
-                                                       typename F::T* __restrict__ b2, typename F::Acc* res) {
+                                                       typename F::T* __restrict__ b2, typename F::Acc* res,
+                                                       typename F::Acc* res_clear) {
     using T = typename F::T;
     using A = typename F::Acc;
+    if (blockIdx.x == 0 && threadIdx.x < 3) res_clear[threadIdx.x] = A{};
     __shared__ T tab[F::TAB];
     for (int e = threadIdx.x; e < F::TAB; e += blockDim.x) tab[e] = Ops<F>::table_entry(r, e);
     __syncthreads();
@@ -925,10 +1046,10 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
 
 template <class F>
 void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t len, typename F::T r, typename F::T* f2,
-                          typename F::T* b2, typename F::Acc* res, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_acc<F>, dim3(1), dim3(64), 0, s, res);
+                          typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear, hipStream_t s) {
     size_t h2 = len / 4;
-    hipLaunchKernelGGL(k_sumcheck_fold<F>, dim3(grid_for(h2 ? h2 : 1, 256, 1024)), dim3(256), 0, s, f, b, len, r, f2, b2, res);
+    hipLaunchKernelGGL(k_sumcheck_fold<F>, dim3(grid_for(h2 ? h2 : 1, 256, 1024)), dim3(256), 0, s, f, b, len, r, f2, b2, res,
+                       res_clear);
 }
 
 template <class F>
@@ -956,8 +1077,10 @@ template <class F> void launch_glue(typename F::T* bsum, const typename F::T* bn
     template void launch_fold<F>(const F::T*, size_t, int, const F::T*, F::T*, hipStream_t);                                \
     template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, hipStream_t);                  \
     template void launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t);                   \
-    template void launch_eval013<F>(const F::T*, const F::T*, size_t, F::Acc*, hipStream_t);                                \
-    template void launch_sumcheck_fold<F>(const F::T*, const F::T*, size_t, F::T, F::T*, F::T*, F::Acc*, hipStream_t);      \
+    template void launch_eval013<F>(const F::T*, const F::T*, size_t, F::Acc*, F::Acc*, hipStream_t);                       \
+    template void launch_zero_acc<F>(F::Acc*, hipStream_t);                                                                 \
+    template void launch_sumcheck_fold<F>(const F::T*, const F::T*, size_t, F::T, F::T*, F::T*, F::Acc*, F::Acc*,         \
+                                          hipStream_t);                                                                     \
     template void launch_glue<F>(F::T*, const F::T*, size_t, F::T, hipStream_t);                                           \
     template void launch_induce_prep<F>(const uint64_t*, const F::T*, const F::T*, int, int, F::T*, F::T*, hipStream_t);   \
     template void launch_enforced<F>(const F::T*, int, int, const F::T*, const F::T*, F::Acc*, hipStream_t);
