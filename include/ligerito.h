@@ -57,7 +57,9 @@ void lig_ctx_destroy(lig_ctx *ctx);
  * Phases: "encode0" (Reed–Solomon encode of the first commit of a prove:
  * transpose+scale and FFT pass kernels), "encode" (encodes of the other
  * commits), "merkle" (leaf hashing + tree levels), "fold", "induce",
- * "sumcheck", "open".  Enabling (or disabling) resets the totals. */
+ * "sumcheck", "open", and "pass64" (each radix-2^6 FFT pass launch of the
+ * first commit on its own, over GF(2^32)).  Enabling (or disabling) resets
+ * the totals. */
 int lig_ctx_profile(lig_ctx *ctx, int enable);
 int lig_ctx_profile_read(lig_ctx *ctx, const char *phase, double *total_ms, uint64_t *count);
 /* The hipStream_t all work of this context is ordered on. */

@@ -38,7 +38,7 @@ void Pool::drain() {
 }
 
 const char* phase_name(int p) {
-    static const char* names[PH_COUNT] = {"encode0", "encode", "merkle", "fold", "induce", "sumcheck", "open"};
+    static const char* names[PH_COUNT] = {"encode0", "encode", "merkle", "fold", "induce", "sumcheck", "open", "pass64"};
     return (p >= 0 && p < PH_COUNT) ? names[p] : "";
 }
 
@@ -170,9 +170,29 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
     {
         PBuf<T> msg(&ctx->pool, m * n);
         w.mat = PBuf<T>(&ctx->pool, M * n);
+        // times the radix-2^6 pass launches of the first commit one by one
+        struct Pass64Timer final : PassObserver {
+            Ctx* ctx;
+            hipEvent_t a = nullptr;
+            explicit Pass64Timer(Ctx* c) : ctx(c) {}
+            void before(int radix_log, size_t) override {
+                if (radix_log != 6) return;
+                LIG_HIP(hipEventCreate(&a));
+                LIG_HIP(hipEventRecord(a, ctx->stream));
+            }
+            void after(int radix_log, size_t) override {
+                if (radix_log != 6 || !a) return;
+                hipEvent_t b = nullptr;
+                LIG_HIP(hipEventCreate(&b));
+                LIG_HIP(hipEventRecord(b, ctx->stream));
+                ctx->marks.push_back({PH_PASS64, a, b});
+                a = nullptr;
+            }
+        } p64(ctx);
+        const bool time_passes = ctx->profiling && encode_phase == PH_ENCODE0;
         PhaseTimer pt(ctx, encode_phase);
         launch_transpose_scale<F>(d_poly, m, n, rs.d_pis_lo.p, rs.d_pis_hi.p, msg.p, s);
-        launch_fft_quarters<F>(msg.p, w.mat.p, m, n, rs.d_tw_tabs.p, s);
+        launch_fft_quarters<F>(msg.p, w.mat.p, m, n, rs.d_tw_tabs.p, s, time_passes ? &p64 : nullptr);
         // msg returns to the pool; stream order keeps the FFT's reads first
     }
     w.tree = PBuf<uint32_t>(&ctx->pool, (2 * M - 1) * 8);

@@ -26,7 +26,8 @@ struct Pool {
 };
 
 // Phases timed with HIP events when profiling is enabled.
-enum Phase { PH_ENCODE0 = 0, PH_ENCODE, PH_MERKLE, PH_FOLD, PH_INDUCE, PH_SUMCHECK, PH_OPEN, PH_COUNT };
+// PH_PASS64 times each radix-2^6 FFT pass launch of the first commit alone.
+enum Phase { PH_ENCODE0 = 0, PH_ENCODE, PH_MERKLE, PH_FOLD, PH_INDUCE, PH_SUMCHECK, PH_OPEN, PH_PASS64, PH_COUNT };
 const char* phase_name(int p);
 
 struct Ctx {

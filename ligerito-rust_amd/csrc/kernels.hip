@@ -263,17 +263,18 @@ __global__ __launch_bounds__(256) void k_fft_pass(const typename F::T* __restric
     }
 }
 
-static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw_tabs, hipStream_t s);
+static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw_tabs, hipStream_t s,
+                           PassObserver* obs);
 
 template <class F>
 void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw_tabs,
-                         hipStream_t s) {
+                         hipStream_t s, PassObserver* obs) {
     constexpr int K = PassShape<F>::K, C = PassShape<F>::C;
     int ell = 0;
     while (((size_t)1 << ell) < m) ell++;
     if constexpr (F::BITS == 32) {
         if (ell > 0) {
-            fft_quarters32(msg, mat, m, n, tw_tabs, s);
+            fft_quarters32(msg, mat, m, n, tw_tabs, s, obs);
             return;
         }
     }
@@ -463,7 +464,8 @@ static void launch_pass32(int K, dim3 grid, int block, hipStream_t s, const uint
 
 // GF(2^32) encode passes, quarter by quarter so that a quarter (m x n x 4
 // bytes) stays resident in the last-level cache between its passes.
-static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw_tabs, hipStream_t s) {
+static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw_tabs, hipStream_t s,
+                           PassObserver* obs) {
     int ell = 0;
     while (((size_t)1 << ell) < m) ell++;
     const int KMAX = 6;
@@ -487,7 +489,10 @@ static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t 
             const uint32_t* src = (p == 0 && msg) ? msg : mat;
             const int shared = (p == 0 && msg) ? 1 : 0;
             dim3 grid((unsigned)(jobs / tpw), by_quarter ? 1 : 4);
+            const size_t elems = m * n * (by_quarter ? 1 : 4);
+            if (obs) obs->before(K, elems);
             launch_pass32(K, grid, cc << K2, s, src, mat, m, n, ell, d0, by_quarter ? qi : -1, shared, tpw, tw_tabs);
+            if (obs) obs->after(K, elems);
             d0 += K;
         }
     }
@@ -1072,7 +1077,7 @@ template <class F> void launch_glue(typename F::T* bsum, const typename F::T* bn
     template void launch_transpose_scale<F>(const F::T*, size_t, size_t, const F::T*, const F::T*, F::T*, hipStream_t);     \
     template void launch_scale_rows<F>(F::T*, size_t, size_t, const F::T*, hipStream_t);                                    \
     template void launch_fft_stage<F>(F::T*, size_t, size_t, int, const F::T*, int, hipStream_t);                           \
-    template void launch_fft_quarters<F>(const F::T*, F::T*, size_t, size_t, const F::T*, hipStream_t);                     \
+    template void launch_fft_quarters<F>(const F::T*, F::T*, size_t, size_t, const F::T*, hipStream_t, PassObserver*);                     \
     template void launch_leaf_hash<F>(const F::T*, size_t, size_t, uint32_t*, hipStream_t);                                 \
     template void launch_fold<F>(const F::T*, size_t, int, const F::T*, F::T*, hipStream_t);                                \
     template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, hipStream_t);                  \

@@ -50,9 +50,19 @@ void launch_fft_stage(typename F::T* buf, size_t N, size_t n, int stage, const t
 // row-major [4m][n] matrix `mat`.  If msg != nullptr, the first pass reads
 // its input rows from msg (all four quarters start equal to msg), otherwise
 // from mat itself.  tw_tabs holds one table per twiddle index.
+// Sees each GF(2^32) FFT pass launch of launch_fft_quarters: log2 of the
+// pass radix and the elements it reads (and writes), before and after it.
+struct PassObserver {
+    virtual void before(int radix_log, size_t elems) = 0;
+    virtual void after(int radix_log, size_t elems) = 0;
+
+protected:
+    ~PassObserver() = default;
+};
+
 template <class F>
 void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw_tabs,
-                         hipStream_t s);
+                         hipStream_t s, PassObserver* obs = nullptr);
 
 // -------------------------------------------------------------- Merkle ----
 // layer0[row*8 .. +8] = BLAKE3(row bytes) for every row of the [rows][n] matrix.
