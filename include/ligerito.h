@@ -152,8 +152,12 @@ typedef struct lig_proof lig_proof;
 int lig_prover(lig_prover_config *cfg, const void *poly, uint32_t fs_seed, lig_proof **out);
 /* Same with poly already in device memory (left unmodified). */
 int lig_prover_dev(lig_prover_config *cfg, const void *d_poly, uint32_t fs_seed, lig_proof **out);
-/* Canonical encoding (the reference has none): little-endian, struct field
- * order; see INTEGRATION.md.  With buf == NULL only *len is set. */
+/* Canonical encoding (the reference has none), little-endian in struct
+ * field order: "LGRT" | u32 version 1 | u32 field_bytes | initial root[32] |
+ * RLP(initial) | u64 n + n roots[32] | u64 n + n RLPs | u64 len + yr |
+ * RLP(final) | u64 n + n (s0,s1,s2) triples, where RLP = u64 n_rows |
+ * u64 n_cols | rows (row-major) | u64 n_nodes | nodes[32].  With buf == NULL
+ * only *len is set. */
 int lig_proof_serialize(const lig_proof *p, uint8_t *buf, size_t cap, size_t *len);
 int lig_proof_deserialize(int field_bits, const uint8_t *buf, size_t len, lig_proof **out);
 /* ProofSize::proof_size (data_structures.rs:105-194). */
