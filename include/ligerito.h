@@ -85,7 +85,8 @@ int lig_rs_encode(lig_rs *rs, const void *message, void *out);
 int lig_rs_encode_non_systematic(lig_rs *rs, void *message);
 /* Device variant of encode_non_systematic over n_cols columns stored
  * column-major (each column block_length elements), as encode_cols
- * (ligero.rs:29-38) applies it.  Synchronous. */
+ * (ligero.rs:29-38) applies it.  n_cols must be a power of two
+ * (LIG_E_INVALID otherwise).  Synchronous. */
 int lig_rs_encode_non_systematic_cols_dev(lig_rs *rs, void *d_columns, size_t n_cols);
 /* compute_twiddles output of this plan (binary_fft.rs:30-58): block_length-1
  * elements. */
@@ -121,7 +122,9 @@ int lig_fs_get_distinct_queries(lig_fs *fs, size_t n, size_t s, size_t *out);
 /* RecursiveLigeroWitness<F> (data_structures.rs:32-40), device-resident. */
 typedef struct lig_witness lig_witness;
 /* ligero_commit (ligero.rs:70-95): poly has m*n elements (host), rs must
- * have message_length m and block_length 4m. */
+ * have message_length m and block_length 4m.  n must be a power of two
+ * (every configuration of the reference uses one); other n give
+ * LIG_E_INVALID. */
 int lig_ligero_commit(lig_rs *rs, const void *poly, size_t m, size_t n, lig_witness **out);
 /* Same with poly already in device memory. */
 int lig_ligero_commit_dev(lig_rs *rs, const void *d_poly, size_t m, size_t n, lig_witness **out);

@@ -72,8 +72,12 @@ template <class H> void RsPlan<H>::init(Ctx* c, size_t msg_len, size_t block_len
     const size_t M = block_len, m = msg_len;
     d_tw.alloc(M - 1);
     LIG_HIP(hipMemcpyAsync(d_tw.p, tw.data(), (M - 1) * sizeof(T), hipMemcpyHostToDevice, s));
-    d_tw_tabs.alloc((M - 1) * F::TAB);
-    launch_build_tables<F>(d_tw.p, M - 1, d_tw_tabs.p, s);
+    // per-twiddle multiply tables: only the generic passes read them (the
+    // GF(2^32) passes build theirs from d_tw in LDS)
+    if (F::BITS != 32 || m == 1) {
+        d_tw_tabs.alloc((M - 1) * F::TAB);
+        launch_build_tables<F>(d_tw.p, M - 1, d_tw_tabs.p, s);
+    }
     d_pis.alloc(m);
     LIG_HIP(hipMemcpyAsync(d_pis.p, pis.data(), m * sizeof(T), hipMemcpyHostToDevice, s));
     size_t nlo = std::min<size_t>(m, 64), nhi = std::max<size_t>(m / 64, 1);
@@ -94,6 +98,7 @@ template <class H> void RsPlan<H>::init(Ctx* c, size_t msg_len, size_t block_len
 // tail beyond m is zero).
 template <class H> void RsPlan<H>::encode_non_systematic_cols(T* d_cols, size_t n) {
     using F = typename DevField<H>::F;
+    LIG_REQUIRE(is_pow2(n), LIG_E_INVALID, "encode_non_systematic: the column count must be a power of two");
     hipStream_t s = ctx->stream;
     const size_t M_ = M(), m_ = m();
     // column-major -> row-major [M][n]
@@ -108,7 +113,7 @@ template <class H> void RsPlan<H>::encode_non_systematic_cols(T* d_cols, size_t 
     // two leading stages with the general butterfly, then the quarter passes
     int stages = log_block < 2 ? log_block : 2;
     for (int st = 0; st < stages; st++) launch_fft_stage<F>(rm.p, M_, n, st, d_tw.p, 0, s);
-    if (log_block >= 2) launch_fft_quarters<F>(nullptr, rm.p, M_ / 4, n, d_tw_tabs.p, s);
+    if (log_block >= 2) launch_fft_quarters<F>(nullptr, rm.p, M_ / 4, n, d_tw.p, d_tw_tabs.p, s);
     LIG_HIP(hipGetLastError());
     LIG_HIP(hipMemcpyAsync(rmh.data(), rm.p, M_ * n * sizeof(T), hipMemcpyDeviceToHost, s));
     ctx->sync();
@@ -158,7 +163,7 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
     using F = typename DevField<H>::F;
     using T = typename H::T;
     LIG_REQUIRE(rs.m() == m && rs.M() == 4 * m, LIG_E_INVALID, "ligero_commit: plan must encode m -> 4m");
-    LIG_REQUIRE(n >= 1, LIG_E_INVALID, "ligero_commit: n >= 1");
+    LIG_REQUIRE(is_pow2(n), LIG_E_INVALID, "ligero_commit: the column count n must be a power of two");
     LIG_REQUIRE(n * sizeof(T) <= 256 * 1024, LIG_E_INVALID, "ligero_commit: leaves above 256 KiB are not supported");
     Ctx* ctx = rs.ctx;
     hipStream_t s = ctx->stream;
@@ -192,7 +197,7 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
         const bool time_passes = ctx->profiling && encode_phase == PH_ENCODE0;
         PhaseTimer pt(ctx, encode_phase);
         launch_transpose_scale<F>(d_poly, m, n, rs.d_pis_lo.p, rs.d_pis_hi.p, msg.p, s);
-        launch_fft_quarters<F>(msg.p, w.mat.p, m, n, rs.d_tw_tabs.p, s, time_passes ? &p64 : nullptr);
+        launch_fft_quarters<F>(msg.p, w.mat.p, m, n, rs.d_tw.p, rs.d_tw_tabs.p, s, time_passes ? &p64 : nullptr);
         // msg returns to the pool; stream order keeps the FFT's reads first
     }
     w.tree = PBuf<uint32_t>(&ctx->pool, (2 * M - 1) * 8);

@@ -19,7 +19,8 @@ template <> struct Ops<F32> {
     __device__ static T zero() { return 0u; }
     __device__ static T one() { return 1u; }
     __device__ static T from_u64(uint64_t v) { return (uint32_t)v; }
-    __device__ static T table_entry(T lam, int e) { return gf32_table_entry(lam, e >> 4, (uint32_t)(e & 15)); }
+    __device__ static T table_entry(T lam, int e) { return gf32_tab_word(lam, e); }
+    // table layout of gf32_tab_word: nibble k, value v at word k * 16 + v
     __device__ static T mul_tab(const T* __restrict__ t, T w) {
         T r0 = t[w & 15u] ^ t[16 + ((w >> 4) & 15u)];
         T r1 = t[32 + ((w >> 8) & 15u)] ^ t[48 + ((w >> 12) & 15u)];

@@ -57,6 +57,11 @@ LIG_UNROLL
     return gf32_reduce64(p << (4 * k));
 }
 
+// Word e of the 128-word multiply-by-lambda table the kernels use: entry v
+// (0..15) of nibble position k (0..7) sits at word k * 16 + v, so the 16
+// entries of one position fall in 16 distinct LDS banks.
+LIG_HD uint32_t gf32_tab_word(uint32_t lambda, int e) { return gf32_table_entry(lambda, e >> 4, (uint32_t)(e & 15)); }
+
 // ------------------------------------------------------------ GF(2^128) ---
 
 struct u128v {

@@ -6,6 +6,8 @@
 // arrays.  All arithmetic is exact binary-field arithmetic.
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "blake3.hpp"
 #include "fields.hpp"
 #include "kernels.hpp"
@@ -263,18 +265,19 @@ __global__ __launch_bounds__(256) void k_fft_pass(const typename F::T* __restric
     }
 }
 
-static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw_tabs, hipStream_t s,
+static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw, hipStream_t s,
                            PassObserver* obs);
 
 template <class F>
-void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw_tabs,
-                         hipStream_t s, PassObserver* obs) {
+void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw,
+                         const typename F::T* tw_tabs, hipStream_t s, PassObserver* obs) {
     constexpr int K = PassShape<F>::K, C = PassShape<F>::C;
+    if (n == 0 || (n & (n - 1)) != 0) throw std::invalid_argument("column count must be a power of two");
     int ell = 0;
     while (((size_t)1 << ell) < m) ell++;
     if constexpr (F::BITS == 32) {
         if (ell > 0) {
-            fft_quarters32(msg, mat, m, n, tw_tabs, s, obs);
+            fft_quarters32(msg, mat, m, n, tw, s, obs);
             return;
         }
     }
@@ -301,18 +304,55 @@ __device__ __forceinline__ uint32_t lds_ld32(const uint32_t* base, uint32_t byte
     return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(base) + byte_off);
 }
 
+// a ^ b ^ c in one instruction (v_bitop3_b32, truth table 0x96)
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// One GF(2^32) table lookup address per nibble: qa = (w << 2) & 0x3C3C3C3C
+// holds nibble 2j * 4 in byte j, qb = (w >> 2) & 0x3C3C3C3C nibble 2j+1 * 4;
+// a byte permute puts byte j under the table base tb (a multiple of 256),
+// and the nibble position's 16-word offset rides in the LDS instruction.
+#define LIG_TAB_PERM(tb, q, j) __builtin_amdgcn_perm((tb), (q), 0x07060500u + (j))
+
 __device__ __forceinline__ uint32_t mul_tab_lds(const uint32_t* tabs, uint32_t tb, uint32_t w) {
-    const uint32_t qa = (w << 2) & 0x3C3C3C3Cu;  // byte j: nibble 2j, times 4
-    const uint32_t qb = (w >> 2) & 0x3C3C3C3Cu;  // byte j: nibble 2j+1, times 4
-    uint32_t r0 = lds_ld32(tabs + 0, __builtin_amdgcn_perm(tb, qa, 0x07060500u));
-    uint32_t r1 = lds_ld32(tabs + 16, __builtin_amdgcn_perm(tb, qb, 0x07060500u));
-    uint32_t r2 = lds_ld32(tabs + 32, __builtin_amdgcn_perm(tb, qa, 0x07060501u));
-    uint32_t r3 = lds_ld32(tabs + 48, __builtin_amdgcn_perm(tb, qb, 0x07060501u));
-    uint32_t r4 = lds_ld32(tabs + 64, __builtin_amdgcn_perm(tb, qa, 0x07060502u));
-    uint32_t r5 = lds_ld32(tabs + 80, __builtin_amdgcn_perm(tb, qb, 0x07060502u));
-    uint32_t r6 = lds_ld32(tabs + 96, __builtin_amdgcn_perm(tb, qa, 0x07060503u));
-    uint32_t r7 = lds_ld32(tabs + 112, __builtin_amdgcn_perm(tb, qb, 0x07060503u));
-    return (r0 ^ r1 ^ r2) ^ (r3 ^ r4 ^ r5) ^ (r6 ^ r7);
+    const uint32_t qa = (w << 2) & 0x3C3C3C3Cu, qb = (w >> 2) & 0x3C3C3C3Cu;
+    uint32_t r0 = lds_ld32(tabs + 0, LIG_TAB_PERM(tb, qa, 0));
+    uint32_t r1 = lds_ld32(tabs + 16, LIG_TAB_PERM(tb, qb, 0));
+    uint32_t r2 = lds_ld32(tabs + 32, LIG_TAB_PERM(tb, qa, 1));
+    uint32_t r3 = lds_ld32(tabs + 48, LIG_TAB_PERM(tb, qb, 1));
+    uint32_t r4 = lds_ld32(tabs + 64, LIG_TAB_PERM(tb, qa, 2));
+    uint32_t r5 = lds_ld32(tabs + 80, LIG_TAB_PERM(tb, qb, 2));
+    uint32_t r6 = lds_ld32(tabs + 96, LIG_TAB_PERM(tb, qa, 3));
+    uint32_t r7 = lds_ld32(tabs + 112, LIG_TAB_PERM(tb, qb, 3));
+    return xor3(xor3(r0, r1, r2), xor3(r3, r4, r5), r6 ^ r7);
+}
+
+// NM independent butterflies u[i] ^= w[i] * (table at byte offset tb[i])
+// with every LDS read issued before the first XOR, so one wave keeps 8*NM
+// reads in flight.  Returns the updated u in place.
+template <int NM>
+__device__ __forceinline__ void mul_tab_lds_n(const uint32_t* tabs, const uint32_t (&tb)[NM], const uint32_t (&w)[NM],
+                                              uint32_t (&u)[NM]) {
+    uint32_t r[NM][8];
+#pragma unroll
+    for (int i = 0; i < NM; i++) {
+        const uint32_t qa = (w[i] << 2) & 0x3C3C3C3Cu, qb = (w[i] >> 2) & 0x3C3C3C3Cu;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            r[i][2 * j] = LIG_TAB_PERM(tb[i], qa, j);
+            r[i][2 * j + 1] = LIG_TAB_PERM(tb[i], qb, j);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < NM; i++)
+#pragma unroll
+        for (int k = 0; k < 8; k++) r[i][k] = lds_ld32(tabs + 16 * k, r[i][k]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < NM; i++)
+        u[i] = xor3(xor3(r[i][0], r[i][1], r[i][2]), xor3(r[i][3], r[i][4], r[i][5]), xor3(r[i][6], r[i][7], u[i]));
 }
 
 // --------------------------------------------- GF(2^32) register-radix pass --
@@ -325,66 +365,92 @@ __device__ __forceinline__ uint32_t mul_tab_lds(const uint32_t* tabs, uint32_t t
 // twiddle of block B*2^j + t' at stage j is (by GF(2)-linearity in the block
 // index) lambda(base_j) + lambda(t'), so each entry is the XOR of the tile's
 // base table and a table shared by every tile.
+// Global word access at a 32-bit byte offset from a wave-uniform base (the
+// base stays in scalar registers, the offset in one vector register).
+__device__ __forceinline__ uint32_t gld_off(const uint32_t* base, uint32_t byte_off) {
+    return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+__device__ __forceinline__ void gst_off(uint32_t* base, uint32_t byte_off, uint32_t v) {
+    *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
+// Raw buffer access (gfx9 resource word 3 = 0x00020000): base and the
+// wave-uniform offset live in scalar registers, the lane offset in one vector
+// register, and accesses past num_bytes read 0 / are dropped.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t num_bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)num_bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, uint32_t v) {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)voff, (int)soff, 0);
+}
+
+// Six waves per SIMD: three 512-thread workgroups per CU, the most their LDS
+// (48.6 KB each at K = 6) allows.  Offsets inside one quarter are 32-bit
+// (the launcher requires a quarter below 2^31 bytes).  The loop-invariant
+// values that do not fit in the 80-register budget live in scratch; they
+// are reloaded once per tile.
 template <int K1, int K2>
-__global__ __launch_bounds__(512) void k_fft_pass32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t m,
-                                                    size_t n, int ell, int d0, int q_arg, int src_shared, int tiles_per_wg,
-                                                    const uint32_t* __restrict__ tw_tabs) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k_fft_pass32(
+    const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t m, size_t n, int ell, int d0, int q_arg,
+    int src_shared, int tiles_per_wg, const uint32_t* __restrict__ tw) {
     constexpr int K = K1 + K2, R = 1 << K, NT = R - 1;
     constexpr int L1 = 1 << K1, L2 = 1 << K2;
     __shared__ __attribute__((aligned(256))) uint32_t tabs[NT * 128];
     __shared__ uint32_t xch[K2 > 0 ? R * 64 : 1];
-    const size_t q = q_arg >= 0 ? (size_t)q_arg : (size_t)blockIdx.y;
+    const uint32_t q = q_arg >= 0 ? (uint32_t)q_arg : blockIdx.y;
     const int cc = n < 64 ? (int)n : 64;
     const int lgcc = 31 - __clz(cc);
-    const size_t nchunks = (n + 63) >> 6;
+    const uint32_t nchunks = (uint32_t)((n + 63) >> 6);  // a power of two
+    const int lgnc = 31 - __clz(nchunks);
     const int sh = ell - d0 - K;
-    const size_t S = (size_t)1 << sh;
+    const uint32_t S = 1u << sh;
     const int tid = threadIdx.x;
-    const int c = tid & (cc - 1);
-    const int g = tid >> lgcc;  // < L2
-    const size_t qoff = src_shared ? 0 : q * m;
+    const uint32_t c = tid & (cc - 1);
+    const uint32_t g = tid >> lgcc;  // < L2
+    const uint32_t rs4 = (S * (uint32_t)n) << 2;  // byte stride between tile rows
+    const uint32_t lane4 = (g * (S * (uint32_t)n) + c) << 2;
+    const uint32_t qbytes = (uint32_t)(m * n * 4);
+    const __amdgpu_buffer_rsrc_t rsrc = buf_rsrc(src + (src_shared ? (size_t)0 : (size_t)q * m * n), qbytes);
+    const __amdgpu_buffer_rsrc_t rdst = buf_rsrc(dst + (size_t)q * m * n, qbytes);
     // tiles of one workgroup share B (consecutive lo), hence their tables
-    const size_t first_job = (size_t)blockIdx.x * tiles_per_wg;
-    const size_t B = (first_job / nchunks) >> sh;
+    const uint32_t first_job = blockIdx.x * (uint32_t)tiles_per_wg;
+    const uint32_t B = (first_job >> lgnc) >> sh;
 
     for (int tj = 0; tj < tiles_per_wg; tj++) {
-        const size_t job = first_job + tj;
-        const size_t tile_id = job / nchunks;
-        const size_t c0 = (job % nchunks) << 6;
-        const size_t lo = tile_id & (S - 1);
-        const size_t rbase = (B << (ell - d0)) + lo;
-        const uint32_t* sp = src + (qoff + rbase) * n + c0 + c;
-        const size_t rs = S * n;  // element stride between tile rows
+        const uint32_t job = first_job + tj;
+        const uint32_t tile_id = job >> lgnc;
+        const uint32_t c0 = (job & (nchunks - 1)) << 6;
+        const uint32_t lo = tile_id & (S - 1);
+        const uint32_t rbase = (B << (ell - d0)) + lo;
+        const uint32_t toff = (rbase * (uint32_t)n + c0) << 2;  // tile's first element, bytes
         uint32_t v[L1];
 #pragma unroll
-        for (int a = 0; a < L1; a++) v[a] = sp[(size_t)(a * L2 + g) * rs];
+        for (int a = 0; a < L1; a++) v[a] = buf_ld(rsrc, lane4, toff + (uint32_t)(a * L2) * rs4);
         if (tj == 0) {
-            // Twiddle tables of the tile.  Stage j, block t' < 2^j:
-            //   T = T(base_j) ^ XOR_{i in bits(t')} T(2^i)
-            // (twiddles are GF(2)-linear in the block index).  Work item
-            // (e, grp) owns entry e of the tables with t' = grp mod 4; all its
-            // loads are issued before any LDS store.
-            for (int item = tid; item < 512; item += blockDim.x) {
-                const int e = item & 127, grp = item >> 7;
-                uint32_t bv[K], tb[K * (K - 1) / 2 + 1];
-#pragma unroll
-                for (int j = 0; j < K; j++) {
-                    const int d = d0 + j;
-                    const size_t first = ((size_t)1 << (d + 2)) - 1;
-                    bv[j] = tw_tabs[(first + (q << d) + (B << j)) * 128 + e];
-#pragma unroll
-                    for (int i = 0; i < j; i++) tb[j * (j - 1) / 2 + i] = tw_tabs[(first + ((size_t)1 << i)) * 128 + e];
-                }
-#pragma unroll
-                for (int j = 0; j < K; j++) {
-                    for (int tp = grp; tp < (1 << j); tp += 4) {
-                        uint32_t val = bv[j];
-#pragma unroll
-                        for (int i = 0; i < j; i++)
-                            if ((tp >> i) & 1) val ^= tb[j * (j - 1) / 2 + i];
-                        tabs[((1 << j) - 1 + tp) * 128 + e] = val;
-                    }
-                }
+            // Twiddle tables of the tile: table (j, t') multiplies by the
+            // heap twiddle of layer d0+j, block B*2^j + t'.  Work item
+            // (table, nibble position k) builds entries v = 0..15 from
+            // c_b = lambda * x^(4k+b): entry v = XOR of c_b over the bits of v.
+            for (int idx = tid; idx < NT * 8; idx += blockDim.x) {
+                const int tt = idx >> 3, k = idx & 7;
+                const int j = 31 - __clz(tt + 1);
+                const uint32_t tp = (uint32_t)(tt + 1) - (1u << j);
+                const int d = d0 + j;
+                const size_t first = ((size_t)1 << (d + 2)) - 1;
+                const uint32_t lam = tw[first + ((size_t)q << d) + ((size_t)B << j) + tp];
+                const uint32_t c0 = gf32_reduce64((uint64_t)lam << (4 * k));
+                const uint32_t c1 = (c0 << 1) ^ ((0u - (c0 >> 31)) & GF32_POLY);
+                const uint32_t c2 = (c1 << 1) ^ ((0u - (c1 >> 31)) & GF32_POLY);
+                const uint32_t c3 = (c2 << 1) ^ ((0u - (c2 >> 31)) & GF32_POLY);
+                const uint32_t c01 = c0 ^ c1, c23 = c2 ^ c3;
+                uint4* tdst4 = reinterpret_cast<uint4*>(tabs + tt * 128 + k * 16);
+                tdst4[0] = make_uint4(0u, c0, c1, c01);
+                tdst4[1] = make_uint4(c2, c2 ^ c0, c2 ^ c1, c2 ^ c01);
+                tdst4[2] = make_uint4(c3, c3 ^ c0, c3 ^ c1, c3 ^ c01);
+                tdst4[3] = make_uint4(c23, c23 ^ c0, c23 ^ c1, c23 ^ c01);
             }
         } else {
             __syncthreads();  // previous tile's exchange reads are done
@@ -394,20 +460,28 @@ __global__ __launch_bounds__(512) void k_fft_pass32(const uint32_t* __restrict__
 #pragma unroll
         for (int j = 0; j < K1; j++) {
             const int h = L1 >> (j + 1);
+            uint32_t tbs[L1 / 2], ws[L1 / 2], us[L1 / 2];
+            int b = 0;
 #pragma unroll
             for (int a = 0; a < L1; a++) {
                 if (a & h) continue;
-                const uint32_t tb = (uint32_t)(((1 << j) - 1 + (a >> (K1 - j))) * 512);
-                const uint32_t u = v[a] ^ mul_tab_lds(tabs, tb, v[a + h]);
-                v[a] = u;
-                v[a + h] ^= u;
+                tbs[b] = (uint32_t)(((1 << j) - 1 + (a >> (K1 - j))) * 512);
+                us[b] = v[a];
+                ws[b++] = v[a + h];
+            }
+            mul_tab_lds_n<L1 / 2>(tabs, tbs, ws, us);
+            b = 0;
+#pragma unroll
+            for (int a = 0; a < L1; a++) {
+                if (a & h) continue;
+                v[a] = us[b];
+                v[a + h] ^= us[b++];
             }
         }
 
-        uint32_t* dp = dst + (q * m + rbase) * n + c0 + c;
         if constexpr (K2 == 0) {
 #pragma unroll
-            for (int a = 0; a < L1; a++) dp[(size_t)a * rs] = v[a];
+            for (int a = 0; a < L1; a++) buf_st(rdst, lane4, toff + (uint32_t)(a * L2) * rs4, v[a]);
         } else {
 #pragma unroll
             for (int a = 0; a < L1; a++) xch[(a * L2 + g) * 64 + c] = v[a];
@@ -415,40 +489,55 @@ __global__ __launch_bounds__(512) void k_fft_pass32(const uint32_t* __restrict__
             constexpr int NC2 = L1 / L2;
 #pragma unroll
             for (int i = 0; i < NC2; i++) {
-                const int hi = g + L2 * i;
+                const uint32_t hi = g + L2 * i;
 #pragma unroll
                 for (int e = 0; e < L2; e++) v[i * L2 + e] = xch[(hi * L2 + e) * 64 + c];
             }
 #pragma unroll
             for (int j = K1; j < K; j++) {
                 const int h = L2 >> (j - K1 + 1);
+                uint32_t tbs[L1 / 2], ws[L1 / 2], us[L1 / 2];
+                int b = 0;
 #pragma unroll
                 for (int i = 0; i < NC2; i++) {
-                    const int hi = g + L2 * i;
+                    const uint32_t hi = g + L2 * i;
 #pragma unroll
                     for (int e = 0; e < L2; e++) {
                         if (e & h) continue;
-                        const uint32_t tb = (uint32_t)(((1 << j) - 1 + (hi << (j - K1)) + (e >> (K - j))) * 512);
-                        const uint32_t u = v[i * L2 + e] ^ mul_tab_lds(tabs, tb, v[i * L2 + e + h]);
-                        v[i * L2 + e] = u;
-                        v[i * L2 + e + h] ^= u;
+                        tbs[b] = (uint32_t)(((1 << j) - 1 + (hi << (j - K1)) + (e >> (K - j))) * 512);
+                        us[b] = v[i * L2 + e];
+                        ws[b++] = v[i * L2 + e + h];
+                    }
+                }
+                mul_tab_lds_n<L1 / 2>(tabs, tbs, ws, us);
+                b = 0;
+#pragma unroll
+                for (int i = 0; i < NC2; i++) {
+#pragma unroll
+                    for (int e = 0; e < L2; e++) {
+                        if (e & h) continue;
+                        v[i * L2 + e] = us[b];
+                        v[i * L2 + e + h] ^= us[b++];
                     }
                 }
             }
+            // row hi*L2+e of the tile sits (hi*L2+e)*S rows from its first
+            // row, hi = g + L2*i: lane part g*L2 rows + column c
+            const uint32_t lane_st = g * (uint32_t)L2 * rs4 + (c << 2);
 #pragma unroll
             for (int i = 0; i < NC2; i++) {
-                const int hi = g + L2 * i;
 #pragma unroll
-                for (int e = 0; e < L2; e++) dp[(size_t)(hi * L2 + e) * rs] = v[i * L2 + e];
+                for (int e = 0; e < L2; e++)
+                    buf_st(rdst, lane_st, toff + (uint32_t)(L2 * L2 * i + e) * rs4, v[i * L2 + e]);
             }
         }
     }
 }
 
 static void launch_pass32(int K, dim3 grid, int block, hipStream_t s, const uint32_t* src, uint32_t* dst, size_t m, size_t n,
-                          int ell, int d0, int q, int shared, int tpw, const uint32_t* tabs) {
+                          int ell, int d0, int q, int shared, int tpw, const uint32_t* tw) {
 #define LIG_P32(A, B_)                                                                                                    \
-    hipLaunchKernelGGL((k_fft_pass32<A, B_>), grid, dim3(block), 0, s, src, dst, m, n, ell, d0, q, shared, tpw, tabs);   \
+    hipLaunchKernelGGL((k_fft_pass32<A, B_>), grid, dim3(block), 0, s, src, dst, m, n, ell, d0, q, shared, tpw, tw);     \
     break;
     switch (K) {
         case 1: LIG_P32(1, 0)
@@ -464,8 +553,10 @@ static void launch_pass32(int K, dim3 grid, int block, hipStream_t s, const uint
 
 // GF(2^32) encode passes, quarter by quarter so that a quarter (m x n x 4
 // bytes) stays resident in the last-level cache between its passes.
-static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw_tabs, hipStream_t s,
+static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw, hipStream_t s,
                            PassObserver* obs) {
+    if (m * n * sizeof(uint32_t) >= ((size_t)1 << 31)) throw std::invalid_argument("FFT quarter of 2 GiB or more");
+    if (n == 0 || (n & (n - 1)) != 0) throw std::invalid_argument("column count must be a power of two");
     int ell = 0;
     while (((size_t)1 << ell) < m) ell++;
     const int KMAX = 6;
@@ -485,13 +576,13 @@ static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t 
             // tiles sharing B (hence tables) are grouped per workgroup
             size_t jobs = tiles * nchunks;
             int tpw = 1;
-            while (tpw < 4 && (size_t)(tpw * 2) <= S * nchunks && jobs / (tpw * 2) >= 2048) tpw *= 2;
+            while (tpw < 4 && (size_t)(tpw * 2) <= S * nchunks && jobs / (tpw * 2) >= 512) tpw *= 2;
             const uint32_t* src = (p == 0 && msg) ? msg : mat;
             const int shared = (p == 0 && msg) ? 1 : 0;
             dim3 grid((unsigned)(jobs / tpw), by_quarter ? 1 : 4);
             const size_t elems = m * n * (by_quarter ? 1 : 4);
             if (obs) obs->before(K, elems);
-            launch_pass32(K, grid, cc << K2, s, src, mat, m, n, ell, d0, by_quarter ? qi : -1, shared, tpw, tw_tabs);
+            launch_pass32(K, grid, cc << K2, s, src, mat, m, n, ell, d0, by_quarter ? qi : -1, shared, tpw, tw);
             if (obs) obs->after(K, elems);
             d0 += K;
         }
@@ -760,7 +851,7 @@ __global__ __launch_bounds__(256) void k_induce_accum32(const uint32_t* __restri
     __syncthreads();
     for (int idx = threadIdx.x; idx < qn * 128; idx += blockDim.x) {
         const int i = idx >> 7, e = idx & 127;
-        tabs[idx] = gf32_table_entry(kv[i], e >> 4, (uint32_t)(e & 15));
+        tabs[idx] = gf32_tab_word(kv[i], e);
     }
     __syncthreads();
     for (size_t x = threadIdx.x; x < np; x += blockDim.x) {
@@ -784,7 +875,7 @@ __global__ __launch_bounds__(256) void k_induce_tensor32(const uint32_t* __restr
     const uint32_t* c = cvals + (size_t)i * ell;
     for (int idx = threadIdx.x; idx < ell * 128; idx += blockDim.x) {
         const int e = idx & 127;
-        tabs[idx] = gf32_table_entry(c[idx >> 7], e >> 4, (uint32_t)(e & 15));
+        tabs[idx] = gf32_tab_word(c[idx >> 7], e);
     }
     if (threadIdx.x == 0) {
         pb[0] = 1u;
@@ -872,16 +963,17 @@ __global__ __launch_bounds__(256) void k_induce_prep32(const uint64_t* __restric
     __syncthreads();
     const int ntab = ell + 8;
     for (int idx = threadIdx.x; idx < ntab * 128; idx += blockDim.x) {
-        const int tbl = idx >> 7, e = idx & 127, k = e >> 4;
-        const uint32_t v = (uint32_t)(e & 15);
+        const int tbl = idx >> 7, e = idx & 127;
         uint32_t val;
         if (tbl == 0) {
-            const uint32_t y = v << (4 * k);
+            // squaring table in the gf32_tab_word layout
+            const int k = e >> 4;
+            const uint32_t y = (uint32_t)(e & 15) << (4 * k);
             val = gf32_reduce64(clmul32_portable(y, y));
         } else if (tbl < ell) {
-            val = gf32_table_entry(sks[tbl - 1], k, v);
+            val = gf32_tab_word(sks[tbl - 1], e);
         } else {
-            val = gf32_table_entry(apow[tbl - ell], k, v);
+            val = gf32_tab_word(apow[tbl - ell], e);
         }
         tabs[idx] = val;
     }
@@ -1077,7 +1169,8 @@ template <class F> void launch_glue(typename F::T* bsum, const typename F::T* bn
     template void launch_transpose_scale<F>(const F::T*, size_t, size_t, const F::T*, const F::T*, F::T*, hipStream_t);     \
     template void launch_scale_rows<F>(F::T*, size_t, size_t, const F::T*, hipStream_t);                                    \
     template void launch_fft_stage<F>(F::T*, size_t, size_t, int, const F::T*, int, hipStream_t);                           \
-    template void launch_fft_quarters<F>(const F::T*, F::T*, size_t, size_t, const F::T*, hipStream_t, PassObserver*);                     \
+    template void launch_fft_quarters<F>(const F::T*, F::T*, size_t, size_t, const F::T*, const F::T*, hipStream_t,         \
+                                         PassObserver*);                     \
     template void launch_leaf_hash<F>(const F::T*, size_t, size_t, uint32_t*, hipStream_t);                                 \
     template void launch_fold<F>(const F::T*, size_t, int, const F::T*, F::T*, hipStream_t);                                \
     template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, hipStream_t);                  \

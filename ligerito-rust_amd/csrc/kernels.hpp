@@ -46,10 +46,6 @@ template <class F> void launch_scale_rows(typename F::T* buf, size_t rows, size_
 // (binary_fft.rs:180-206 butterflies); inverse selects the IFFT butterfly.
 template <class F>
 void launch_fft_stage(typename F::T* buf, size_t N, size_t n, int stage, const typename F::T* tw, int inverse, hipStream_t s);
-// Stages 2..log2(4m)-1 of the length-4m FFT on the four quarters of the
-// row-major [4m][n] matrix `mat`.  If msg != nullptr, the first pass reads
-// its input rows from msg (all four quarters start equal to msg), otherwise
-// from mat itself.  tw_tabs holds one table per twiddle index.
 // Sees each GF(2^32) FFT pass launch of launch_fft_quarters: log2 of the
 // pass radix and the elements it reads (and writes), before and after it.
 struct PassObserver {
@@ -60,9 +56,15 @@ protected:
     ~PassObserver() = default;
 };
 
+// Stages 2..log2(4m)-1 of the length-4m FFT on the four quarters of the
+// row-major [4m][n] matrix `mat`.  If msg != nullptr, the first pass reads
+// its input rows from msg (all four quarters start equal to msg), otherwise
+// from mat itself.  tw holds the heap-ordered twiddles; tw_tabs one
+// multiply table per twiddle, read only where the GF(2^32) passes do not
+// apply (GF(2^128), or m == 1), and may be null otherwise.
 template <class F>
-void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw_tabs,
-                         hipStream_t s, PassObserver* obs = nullptr);
+void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw,
+                         const typename F::T* tw_tabs, hipStream_t s, PassObserver* obs = nullptr);
 
 // -------------------------------------------------------------- Merkle ----
 // layer0[row*8 .. +8] = BLAKE3(row bytes) for every row of the [rows][n] matrix.
