@@ -38,6 +38,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--log-n", type=int, default=LOG_N)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-prove measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--batch-jobs", type=int, default=256)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (scales hardware byte counters)")
     args = ap.parse_args()
@@ -123,6 +128,9 @@ def main():
         with open(tf) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
+    batched = None if args.no_batch else _batched(lh, torch, np, cfg, ptr, args.batch_jobs)
+    served = None if args.no_serve else _served(lh, torch, np, args.serve_requests, args.serve_rate)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = _cpu_baseline(poly, L)
@@ -164,8 +172,119 @@ def main():
         "cpu_baseline": cpu,
         "phases_ms_per_prove": {k: v[0] / n_prof for k, v in phases.items()},
         "host_input_coeffs_per_s": N / t_host,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
+
+
+def _gpu_poly(torch, L, seed):
+    # uniform 32-bit elements from a seeded device generator
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(seed)
+    return torch.randint(-(1 << 31), 1 << 31, (1 << L,), dtype=torch.int32, device="cuda:0", generator=g)
+
+
+def _batched(lh, torch, np, cfg24, ptr24, n_jobs, streams=4):
+    """Batched proves beside the plain path: (a) n_jobs polynomials with log2
+    sizes uniform in 16..22 (seed 3), each with its own transcript seed;
+    (b) eight 2^24 proves.  Inputs are in HBM; prover configurations are
+    built before timing, as for the plain path."""
+    rng = np.random.default_rng(3)
+    sizes = [int(x) for x in rng.integers(16, 23, size=n_jobs)]
+    seeds = [1234 + i for i in range(n_jobs)]
+    polys = [_gpu_poly(torch, L, 1000 + i) for i, L in enumerate(sizes)]
+    torch.cuda.synchronize()
+    ptrs = [t.data_ptr() for t in polys]
+    total = sum(1 << L for L in sizes)
+    b = lh.Batch(0, 32, n_streams=streams)
+    b.prove(sizes, ptrs, seeds, on_device=True)  # builds each worker's configurations
+    t0 = time.perf_counter()
+    proofs = b.prove(sizes, ptrs, seeds, on_device=True)
+    t_batch = time.perf_counter() - t0
+    for i in (0, n_jobs // 2, n_jobs - 1):
+        if lh.verifier(sizes[i], proofs[i], seeds[i]) != 1:
+            raise SystemExit(f"verifier rejected batched proof {i}")
+    # the same jobs one after another on the plain path
+    ctx = lh.Context(0)
+    cfgs = {L: lh.ProverConfig(ctx, L, 32) for L in set(sizes)}
+    lh.prover_dev(cfgs[sizes[0]], ptrs[0], seeds[0])
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for L, p, s in zip(sizes, ptrs, seeds):
+        lh.prover_dev(cfgs[L], p, s)
+    ctx.synchronize()
+    t_plain = time.perf_counter() - t0
+    log(f"batched {n_jobs} jobs: {t_batch * 1e3:.1f} ms batched, {t_plain * 1e3:.1f} ms plain")
+    # eight 2^24 proves
+    n24 = 8
+    ptr_list = [ptr24] * n24
+    seeds24 = [1234 + i for i in range(n24)]
+    b.prove([24] * n24, ptr_list, seeds24, on_device=True)
+    t0 = time.perf_counter()
+    b.prove([24] * n24, ptr_list, seeds24, on_device=True)
+    t_b24 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for s in seeds24:
+        lh.prover_dev(cfg24, ptr24, s)
+    t_p24 = time.perf_counter() - t0
+    log(f"eight 2^24 proves: {t_b24 * 1e3:.1f} ms batched, {t_p24 * 1e3:.1f} ms plain")
+    b.close()
+    return {
+        "workload": f"{n_jobs} GF(2^32) polynomials, log2 sizes uniform in 16..22 (seed 3), own transcript seeds, "
+                    f"inputs in HBM, {streams} streams",
+        "coeffs": total,
+        "batched_coeffs_per_s": total / t_batch,
+        "plain_coeffs_per_s": total / t_plain,
+        "batched_over_plain": t_plain / t_batch,
+        "batch_2pow24": {"jobs": n24, "batched_coeffs_per_s": n24 * (1 << 24) / t_b24,
+                         "plain_coeffs_per_s": n24 * (1 << 24) / t_p24, "batched_over_plain": t_p24 / t_b24},
+    }
+
+
+def _served(lh, torch, np, n_req, rate, slots=4, budget=64 << 30):
+    """Open-loop Poisson arrivals of prove requests (log2 sizes uniform in
+    16..24, seed 4) against the prove server; inputs in pinned host memory
+    (the upload is part of each request's latency)."""
+    rng = np.random.default_rng(4)
+    sizes = [int(x) for x in rng.integers(16, 25, size=n_req)]
+    gaps = rng.exponential(1.0 / rate, size=n_req)
+    host = {}
+    for L in sorted(set(sizes)):
+        g = torch.Generator()
+        g.manual_seed(4000 + L)
+        host[L] = torch.randint(-(1 << 31), 1 << 31, (1 << L,), dtype=torch.int32, generator=g).pin_memory()
+    srv = lh.Server(0, 32, n_slots=slots, hbm_budget=budget)
+    # warm every worker's configuration for every size (untimed)
+    warm = [srv.submit(L, host[L].data_ptr(), 1) for L in sorted(host) for _ in range(slots)]
+    for t in warm:
+        srv.wait(t)
+    t_first = time.perf_counter()
+    t_next = t_first
+    tickets = []
+    for i, L in enumerate(sizes):
+        t_next += gaps[i]
+        while time.perf_counter() < t_next:
+            pass
+        tickets.append(srv.submit(L, host[L].data_ptr(), 5000 + i))
+    lats = []
+    for t in tickets:
+        _, lat = srv.wait(t)
+        lats.append(lat)
+    t_end = time.perf_counter()
+    srv.stop()
+    total = sum(1 << L for L in sizes)
+    lats = np.array(lats)
+    log(f"served {n_req} requests at {rate}/s: p50 {np.percentile(lats, 50):.2f} ms, p99 {np.percentile(lats, 99):.2f} ms")
+    return {
+        "workload": f"{n_req} requests, Poisson arrivals at {rate:g}/s (seed 4), log2 sizes uniform in 16..24, "
+                    f"GF(2^32), inputs in pinned host memory, {slots} slots, HBM budget {budget >> 30} GiB",
+        "throughput_coeffs_per_s": total / (t_end - t_first),
+        "throughput_requests_per_s": n_req / (t_end - t_first),
+        "p50_ms": float(np.percentile(lats, 50)),
+        "p99_ms": float(np.percentile(lats, 99)),
+        "mean_ms": float(lats.mean()),
+    }
 
 
 def _splitmix_u32(seed, n):

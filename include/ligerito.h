@@ -173,6 +173,43 @@ void lig_proof_free(lig_proof *p);
  * LIG_E_MERKLE / LIG_E_LIGERO / LIG_E_ASSERT where the reference panics. */
 int lig_verifier(int field_bits, int log_n, const lig_proof *p, uint32_t fs_seed);
 
+/* ---------------------------------------------------------------- batch -- */
+/* No counterpart in the reference (its prover runs one proof at a time):
+ * many independent proves on one device.  A batch holds n_streams workers,
+ * each with its own HIP stream, device memory pool, pinned staging and prover
+ * configurations (built on first use of a size), so the proves of different
+ * jobs overlap on the GPU. */
+typedef struct lig_batch lig_batch;
+int lig_batch_create(int device, int field_bits, int n_streams, lig_batch **out);
+void lig_batch_destroy(lig_batch *b);
+/* Proves n_jobs polynomials: job i has 2^log_n[i] elements (log_n in
+ * [16, 27]) at polys[i] — host memory, or device memory when
+ * polys_on_device — and transcript seed fs_seeds[i].  out[i] receives a proof
+ * identical to lig_prover's for the same input and seed.  Returns when every
+ * job is done; on error no proof is returned (out[] all NULL). */
+int lig_prove_batch(lig_batch *b, size_t n_jobs, const int *log_n, const void *const *polys, int polys_on_device,
+                    const uint32_t *fs_seeds, lig_proof **out);
+/* -------------------------------------------------------------- serving -- */
+/* No counterpart in the reference: a prove server with continuous batching.
+ * n_slots workers (as in a batch) take requests first come, first served; a
+ * request is admitted when a worker is free and its device-memory estimate
+ * fits in hbm_budget bytes beside the requests already running (a request
+ * larger than the whole budget runs alone).  Requests that arrive while
+ * others run are admitted as slots and memory free up, and their GPU work
+ * overlaps. */
+typedef struct lig_server lig_server;
+int lig_serve_start(int device, int field_bits, int n_slots, uint64_t hbm_budget, lig_server **out);
+/* Queues a prove of 2^log_n elements at poly (host memory; it must stay
+ * valid until the request completes) with transcript seed fs_seed. */
+int lig_serve_submit(lig_server *s, int log_n, const void *poly, uint32_t fs_seed, uint64_t *ticket);
+/* 1 with *out set (and the submit-to-completion latency in ms) once the
+ * request is done — the ticket is then released — 0 while it is pending, or
+ * the request's negative status if it failed. */
+int lig_serve_poll(lig_server *s, uint64_t ticket, lig_proof **out, double *latency_ms);
+/* As lig_serve_poll, waiting for completion. */
+int lig_serve_wait(lig_server *s, uint64_t ticket, lig_proof **out, double *latency_ms);
+/* Finishes the running requests, fails the queued ones, frees the server. */
+void lig_serve_stop(lig_server *s);
 #ifdef __cplusplus
 }
 #endif

@@ -18,7 +18,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-mpclmul", "-msse4.1", "-Wall", "-W
               "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]
 
 HIP_SRC = ["kernels.hip"]
-HOST_SRC = ["host_util.cpp", "engine.cpp", "proof.cpp", "capi.cpp"]
+HOST_SRC = ["host_util.cpp", "engine.cpp", "proof.cpp", "batch.cpp", "capi.cpp"]
 
 
 def _run(cmd):

@@ -1,0 +1,243 @@
+// Many independent proves on one device: a batch call and a request server.
+//
+// Neither exists in the reference (its prover runs one proof at a time on
+// the host).  Both keep the single-proof prover (engine.cpp) unchanged and
+// run several of them at once, each on a worker with its own HIP stream,
+// device-memory pool, pinned staging and cached prover configurations, so
+// the kernels of different proofs overlap on the GPU while any one proof
+// waits on its host-side transcript steps.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <exception>
+#include <thread>
+#include <unordered_map>
+
+#include "batch.hpp"
+
+namespace lig {
+
+template <class H> Worker<H>::Worker(int dev) {
+    ctx.device = dev;
+    LIG_HIP(hipSetDevice(dev));
+    LIG_HIP(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
+}
+
+template <class H> Worker<H>::~Worker() {
+    if (!ctx.stream) return;
+    (void)hipSetDevice(ctx.device);
+    (void)hipStreamSynchronize(ctx.stream);
+    cfgs.clear();
+    ctx.pool.drain();
+    (void)hipStreamDestroy(ctx.stream);
+}
+
+template <class H> Config<H>& Worker<H>::config(int log_n) {
+    auto it = cfgs.find(log_n);
+    if (it != cfgs.end()) return *it->second;
+    std::unique_ptr<Config<H>> c(new Config<H>());
+    c->init(&ctx, log_n);
+    Config<H>& ref = *c;
+    cfgs.emplace(log_n, std::move(c));
+    return ref;
+}
+
+template <class H> void Worker<H>::prove_one(int log_n, const void* poly, bool on_device, uint32_t seed, Proof<H>& out) {
+    using T = typename H::T;
+    LIG_HIP(hipSetDevice(ctx.device));
+    Config<H>& cfg = config(log_n);
+    const size_t n = (size_t)1 << log_n;
+    PBuf<T> tmp;
+    const T* d = (const T*)poly;
+    if (!on_device) {
+        tmp = PBuf<T>(&ctx.pool, n);
+        LIG_HIP(hipMemcpyAsync(tmp.p, poly, n * sizeof(T), hipMemcpyHostToDevice, ctx.stream));
+        d = tmp.p;
+    }
+    prove<H>(cfg, d, seed, out);
+}
+
+size_t prove_footprint(int field_bits, int log_n) {
+    // codeword matrix 4N, message and input copies 2N, folds and sumcheck
+    // buffers below N, plus the first Merkle tree (2 * 4N/64 nodes of 32 B)
+    const size_t n = (size_t)1 << log_n, fb = (size_t)field_bits / 8;
+    return 7 * n * fb + (n / 8) * 32 + ((size_t)64 << 20);
+}
+
+// ---------------------------------------------------------------- batch --
+template <class H> Batch<H>::Batch(int dev, int n_streams) : device(dev) {
+    LIG_REQUIRE(n_streams >= 1 && n_streams <= 64, LIG_E_INVALID, "n_streams must be in [1, 64]");
+    for (int i = 0; i < n_streams; i++) workers.emplace_back(new Worker<H>(dev));
+}
+
+template <class H>
+void Batch<H>::run(size_t n_jobs, const int* log_n, const void* const* polys, bool on_device, const uint32_t* seeds,
+                   std::vector<Proof<H>>& out) {
+    for (size_t i = 0; i < n_jobs; i++) {
+        LIG_REQUIRE(polys[i], LIG_E_INVALID, "NULL polynomial");
+        LIG_REQUIRE(log_n[i] >= 16 && log_n[i] <= 27, LIG_E_INVALID, "log_n must be in [16, 27]");
+    }
+    out.assign(n_jobs, Proof<H>());
+    // largest first, so the tail of the batch is made of short proofs
+    std::vector<size_t> order(n_jobs);
+    for (size_t i = 0; i < n_jobs; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return log_n[a] > log_n[b]; });
+    std::atomic<size_t> next{0};
+    std::atomic<bool> failed{false};
+    std::mutex err_mu;
+    int err_code = LIG_OK;
+    std::string err_msg;
+    auto body = [&](Worker<H>* w) {
+        try {
+            for (;;) {
+                if (failed.load()) return;
+                const size_t k = next.fetch_add(1);
+                if (k >= n_jobs) return;
+                const size_t j = order[k];
+                w->prove_one(log_n[j], polys[j], on_device, seeds[j], out[j]);
+            }
+        } catch (const Error& e) {
+            std::lock_guard<std::mutex> g(err_mu);
+            if (!failed.exchange(true)) { err_code = e.code; err_msg = e.what(); }
+        } catch (const std::bad_alloc&) {
+            std::lock_guard<std::mutex> g(err_mu);
+            if (!failed.exchange(true)) { err_code = LIG_E_OOM; err_msg = "host allocation failed"; }
+        } catch (const std::exception& e) {
+            std::lock_guard<std::mutex> g(err_mu);
+            if (!failed.exchange(true)) { err_code = LIG_E_ASSERT; err_msg = e.what(); }
+        }
+    };
+    const size_t nw = std::min(workers.size(), std::max<size_t>(n_jobs, 1));
+    std::vector<std::thread> threads;
+    for (size_t i = 1; i < nw; i++) threads.emplace_back(body, workers[i].get());
+    body(workers[0].get());
+    for (auto& t : threads) t.join();
+    if (failed.load()) throw Error(err_code, err_msg);
+}
+
+// --------------------------------------------------------------- server --
+template <class H> Server<H>::Server(int dev, int n_slots, uint64_t budget) : device(dev), hbm_budget(budget) {
+    LIG_REQUIRE(n_slots >= 1 && n_slots <= 64, LIG_E_INVALID, "n_slots must be in [1, 64]");
+    for (int i = 0; i < n_slots; i++) workers.emplace_back(new Worker<H>(dev));
+    for (int i = 0; i < n_slots; i++) threads.emplace_back([this, i] { loop(workers[i].get()); });
+}
+
+template <class H> Server<H>::~Server() {
+    {
+        std::lock_guard<std::mutex> g(mu);
+        stopping = true;
+        // queued requests that were never admitted fail
+        for (uint64_t t : queue) {
+            auto& r = reqs[t];
+            r.status = LIG_E_INVALID;
+            r.message = "server stopped before the request was admitted";
+            r.done = true;
+        }
+        queue.clear();
+    }
+    cv.notify_all();
+    for (auto& t : threads) t.join();
+}
+
+template <class H> uint64_t Server<H>::submit(int log_n, const void* poly, uint32_t seed) {
+    LIG_REQUIRE(poly, LIG_E_INVALID, "NULL polynomial");
+    LIG_REQUIRE(log_n >= 16 && log_n <= 27, LIG_E_INVALID, "log_n must be in [16, 27]");
+    std::lock_guard<std::mutex> g(mu);
+    LIG_REQUIRE(!stopping, LIG_E_INVALID, "server is stopping");
+    const uint64_t t = ++last_ticket;
+    Req& r = reqs[t];
+    r.log_n = log_n;
+    r.poly = poly;
+    r.seed = seed;
+    r.footprint = prove_footprint(H::BYTES * 8, log_n);
+    r.t_submit = std::chrono::steady_clock::now();
+    queue.push_back(t);
+    cv.notify_all();
+    return t;
+}
+
+template <class H> bool Server<H>::admissible(const Req& r) const {
+    // a request larger than the whole budget runs alone
+    return in_use + r.footprint <= hbm_budget || in_use == 0;
+}
+
+template <class H> void Server<H>::loop(Worker<H>* w) {
+    for (;;) {
+        uint64_t t;
+        size_t fp;
+        int log_n;
+        const void* poly;
+        uint32_t seed;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            // first come, first admitted: the head waits until it fits
+            cv.wait(lk, [&] { return stopping || (!queue.empty() && admissible(reqs[queue.front()])); });
+            if (stopping && queue.empty()) return;
+            if (queue.empty()) continue;
+            t = queue.front();
+            queue.pop_front();
+            Req& r = reqs[t];
+            fp = r.footprint;
+            in_use += fp;
+            r.t_start = std::chrono::steady_clock::now();
+            log_n = r.log_n;
+            poly = r.poly;
+            seed = r.seed;
+        }
+        Proof<H> proof;
+        int status = LIG_OK;
+        std::string msg;
+        try {
+            w->prove_one(log_n, poly, false, seed, proof);
+        } catch (const Error& e) {
+            status = e.code;
+            msg = e.what();
+        } catch (const std::bad_alloc&) {
+            status = LIG_E_OOM;
+            msg = "host allocation failed";
+        } catch (const std::exception& e) {
+            status = LIG_E_ASSERT;
+            msg = e.what();
+        }
+        {
+            std::lock_guard<std::mutex> g(mu);
+            in_use -= fp;
+            Req& r = reqs[t];
+            r.proof = std::move(proof);
+            r.status = status;
+            r.message = msg;
+            r.t_done = std::chrono::steady_clock::now();
+            r.done = true;
+        }
+        cv.notify_all();
+        done_cv.notify_all();
+    }
+}
+
+template <class H> int Server<H>::take(uint64_t ticket, bool wait, Proof<H>& out, double* latency_ms, std::string& msg) {
+    std::unique_lock<std::mutex> lk(mu);
+    auto it = reqs.find(ticket);
+    LIG_REQUIRE(it != reqs.end(), LIG_E_INVALID, "unknown ticket");
+    if (!it->second.done) {
+        if (!wait) return 0;
+        done_cv.wait(lk, [&] { return reqs[ticket].done; });
+        it = reqs.find(ticket);
+    }
+    Req& r = it->second;
+    const int status = r.status;
+    msg = r.message;
+    if (latency_ms) latency_ms[0] = std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count();
+    if (status == LIG_OK) out = std::move(r.proof);
+    reqs.erase(it);
+    return status == LIG_OK ? 1 : status;
+}
+
+template struct Worker<H32>;
+template struct Worker<H128>;
+template struct Batch<H32>;
+template struct Batch<H128>;
+template struct Server<H32>;
+template struct Server<H128>;
+
+}  // namespace lig

@@ -1,0 +1,76 @@
+// Batched proves and the request server (see batch.cpp).
+#pragma once
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace lig {
+
+// One proving lane: its own stream, pool, pinned staging and configurations.
+template <class H> struct Worker {
+    Ctx ctx;
+    std::map<int, std::unique_ptr<Config<H>>> cfgs;
+    explicit Worker(int dev);
+    ~Worker();
+    Config<H>& config(int log_n);
+    // poly: 2^log_n elements in host memory, or device memory if on_device
+    void prove_one(int log_n, const void* poly, bool on_device, uint32_t seed, Proof<H>& out);
+};
+
+// Device bytes one prove of 2^log_n elements holds at its peak (estimate
+// used for admission).
+size_t prove_footprint(int field_bits, int log_n);
+
+template <class H> struct Batch {
+    int device;
+    std::vector<std::unique_ptr<Worker<H>>> workers;
+    Batch(int dev, int n_streams);
+    void run(size_t n_jobs, const int* log_n, const void* const* polys, bool on_device, const uint32_t* seeds,
+             std::vector<Proof<H>>& out);
+};
+
+template <class H> struct Server {
+    struct Req {
+        int log_n = 0;
+        const void* poly = nullptr;
+        uint32_t seed = 0;
+        size_t footprint = 0;
+        bool done = false;
+        int status = 0;
+        std::string message;
+        Proof<H> proof;
+        std::chrono::steady_clock::time_point t_submit, t_start, t_done;
+    };
+    int device;
+    uint64_t hbm_budget;
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    std::deque<uint64_t> queue;
+    std::unordered_map<uint64_t, Req> reqs;
+    uint64_t last_ticket = 0;
+    size_t in_use = 0;
+    bool stopping = false;
+    std::vector<std::unique_ptr<Worker<H>>> workers;
+    std::vector<std::thread> threads;
+
+    Server(int dev, int n_slots, uint64_t budget);
+    ~Server();
+    uint64_t submit(int log_n, const void* poly, uint32_t seed);
+    // 1 with the proof when done, 0 while pending (wait == false), or the
+    // request's negative status; a finished ticket is released.
+    int take(uint64_t ticket, bool wait, Proof<H>& out, double* latency_ms, std::string& msg);
+
+private:
+    bool admissible(const Req& r) const;
+    void loop(Worker<H>* w);
+};
+
+}  // namespace lig

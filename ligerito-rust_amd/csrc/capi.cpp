@@ -3,6 +3,7 @@
 #include <array>
 #include <exception>
 
+#include "batch.hpp"
 #include "engine.hpp"
 
 using namespace lig;
@@ -39,6 +40,18 @@ struct lig_proof {
     int bits;
     Proof<H32> p32;
     Proof<H128> p128;
+};
+
+struct lig_batch {
+    int bits;
+    std::unique_ptr<Batch<H32>> b32;
+    std::unique_ptr<Batch<H128>> b128;
+};
+
+struct lig_server {
+    int bits;
+    std::unique_ptr<Server<H32>> s32;
+    std::unique_ptr<Server<H128>> s128;
 };
 
 namespace {
@@ -511,5 +524,103 @@ int lig_verifier(int field_bits, int log_n, const lig_proof* p, uint32_t fs_seed
         return field_bits == 32 ? verify<H32>(log_n, p->p32, fs_seed) : verify<H128>(log_n, p->p128, fs_seed);
     });
 }
+
+// ------------------------------------------------------------------ batch --
+int lig_batch_create(int device, int field_bits, int n_streams, lig_batch** out) {
+    return guarded([&] {
+        LIG_REQUIRE(out, LIG_E_INVALID, "out is NULL");
+        check_bits(field_bits);
+        int n = lig_device_count();
+        LIG_REQUIRE(n > 0, LIG_E_NO_DEVICE, "no HIP device visible");
+        LIG_REQUIRE(device >= 0 && device < n, LIG_E_INVALID, "device index out of range");
+        std::unique_ptr<lig_batch> b(new lig_batch());
+        b->bits = field_bits;
+        if (field_bits == 32) b->b32.reset(new Batch<H32>(device, n_streams));
+        else b->b128.reset(new Batch<H128>(device, n_streams));
+        *out = b.release();
+        return LIG_OK;
+    });
+}
+
+void lig_batch_destroy(lig_batch* b) { delete b; }
+
+int lig_prove_batch(lig_batch* b, size_t n_jobs, const int* log_n, const void* const* polys, int polys_on_device,
+                    const uint32_t* fs_seeds, lig_proof** out) {
+    return guarded([&] {
+        LIG_REQUIRE(b && out && (n_jobs == 0 || (log_n && polys && fs_seeds)), LIG_E_INVALID, "NULL argument");
+        for (size_t i = 0; i < n_jobs; i++) out[i] = nullptr;
+        std::vector<std::unique_ptr<lig_proof>> res(n_jobs);
+        if (b->bits == 32) {
+            std::vector<Proof<H32>> ps;
+            b->b32->run(n_jobs, log_n, polys, polys_on_device != 0, fs_seeds, ps);
+            for (size_t i = 0; i < n_jobs; i++) {
+                res[i].reset(new lig_proof());
+                res[i]->bits = 32;
+                res[i]->p32 = std::move(ps[i]);
+            }
+        } else {
+            std::vector<Proof<H128>> ps;
+            b->b128->run(n_jobs, log_n, polys, polys_on_device != 0, fs_seeds, ps);
+            for (size_t i = 0; i < n_jobs; i++) {
+                res[i].reset(new lig_proof());
+                res[i]->bits = 128;
+                res[i]->p128 = std::move(ps[i]);
+            }
+        }
+        for (size_t i = 0; i < n_jobs; i++) out[i] = res[i].release();
+        return LIG_OK;
+    });
+}
+
+// ---------------------------------------------------------------- serving --
+int lig_serve_start(int device, int field_bits, int n_slots, uint64_t hbm_budget, lig_server** out) {
+    return guarded([&] {
+        LIG_REQUIRE(out, LIG_E_INVALID, "out is NULL");
+        check_bits(field_bits);
+        int n = lig_device_count();
+        LIG_REQUIRE(n > 0, LIG_E_NO_DEVICE, "no HIP device visible");
+        LIG_REQUIRE(device >= 0 && device < n, LIG_E_INVALID, "device index out of range");
+        LIG_REQUIRE(hbm_budget > 0, LIG_E_INVALID, "hbm_budget must be positive");
+        std::unique_ptr<lig_server> s(new lig_server());
+        s->bits = field_bits;
+        if (field_bits == 32) s->s32.reset(new Server<H32>(device, n_slots, hbm_budget));
+        else s->s128.reset(new Server<H128>(device, n_slots, hbm_budget));
+        *out = s.release();
+        return LIG_OK;
+    });
+}
+
+int lig_serve_submit(lig_server* s, int log_n, const void* poly, uint32_t fs_seed, uint64_t* ticket) {
+    return guarded([&] {
+        LIG_REQUIRE(s && ticket, LIG_E_INVALID, "NULL argument");
+        *ticket = s->bits == 32 ? s->s32->submit(log_n, poly, fs_seed) : s->s128->submit(log_n, poly, fs_seed);
+        return LIG_OK;
+    });
+}
+
+static int serve_take(lig_server* s, uint64_t ticket, bool wait, lig_proof** out, double* latency_ms) {
+    return guarded([&] {
+        LIG_REQUIRE(s && out, LIG_E_INVALID, "NULL argument");
+        *out = nullptr;
+        std::unique_ptr<lig_proof> p(new lig_proof());
+        p->bits = s->bits;
+        std::string msg;
+        const int r = s->bits == 32 ? s->s32->take(ticket, wait, p->p32, latency_ms, msg)
+                                    : s->s128->take(ticket, wait, p->p128, latency_ms, msg);
+        if (r < 0) throw Error(r, msg);
+        if (r == 1) *out = p.release();
+        return r;
+    });
+}
+
+int lig_serve_poll(lig_server* s, uint64_t ticket, lig_proof** out, double* latency_ms) {
+    return serve_take(s, ticket, false, out, latency_ms);
+}
+
+int lig_serve_wait(lig_server* s, uint64_t ticket, lig_proof** out, double* latency_ms) {
+    return serve_take(s, ticket, true, out, latency_ms);
+}
+
+void lig_serve_stop(lig_server* s) { delete s; }
 
 }  // extern "C"

@@ -89,6 +89,14 @@ def lib():
             "lig_proof_size": (SZ, [P]),
             "lig_proof_free": (None, [P]),
             "lig_verifier": (I, [I, I, P, U32]),
+            "lig_batch_create": (I, [I, I, I, PP]),
+            "lig_batch_destroy": (None, [P]),
+            "lig_prove_batch": (I, [P, SZ, C.POINTER(C.c_int), PP, I, C.POINTER(U32), PP]),
+            "lig_serve_start": (I, [I, I, I, C.c_uint64, PP]),
+            "lig_serve_submit": (I, [P, I, P, U32, C.POINTER(C.c_uint64)]),
+            "lig_serve_poll": (I, [P, C.c_uint64, PP, C.POINTER(C.c_double)]),
+            "lig_serve_wait": (I, [P, C.c_uint64, PP, C.POINTER(C.c_double)]),
+            "lig_serve_stop": (None, [P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -411,3 +419,86 @@ def prover_dev(config: ProverConfig, d_poly_ptr: int, fs_seed: int = 1234) -> Pr
 def verifier(log_n: int, proof: Proof, fs_seed: int = 1234) -> int:
     """src/verifier.rs:10-164: 1 accept, 0 reject, negative at the reference's panic sites."""
     return int(lib().lig_verifier(proof.bits, log_n, proof._h, fs_seed))
+
+
+# ---------------------------------------------------------------- batched --
+class Batch:
+    """Many independent proves on one device (lig_batch; no reference counterpart)."""
+
+    def __init__(self, device: int = 0, bits: int = 32, n_streams: int = 4):
+        h = C.c_void_p()
+        _check(lib().lig_batch_create(device, bits, n_streams, C.byref(h)))
+        self._h, self.bits = h, bits
+
+    def prove(self, log_ns, polys, fs_seeds, on_device: bool = False):
+        """polys: host arrays, or device pointers (ints) when on_device."""
+        n = len(log_ns)
+        ln = (C.c_int * n)(*[int(x) for x in log_ns])
+        seeds = (C.c_uint32 * n)(*[int(x) for x in fs_seeds])
+        keep = []
+        ptrs = (C.c_void_p * n)()
+        for i, pl in enumerate(polys):
+            if on_device:
+                ptrs[i] = int(pl)
+            else:
+                a = as_elems(pl, self.bits)
+                keep.append(a)
+                ptrs[i] = a.ctypes.data
+        out = (C.c_void_p * n)()
+        _check(lib().lig_prove_batch(self._h, n, ln, ptrs, 1 if on_device else 0, seeds, out))
+        return [Proof(C.c_void_p(out[i]), self.bits) for i in range(n)]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lig_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Server:
+    """Prove server with continuous batching (lig_server; no reference counterpart)."""
+
+    def __init__(self, device: int = 0, bits: int = 32, n_slots: int = 4, hbm_budget: int = 32 << 30):
+        h = C.c_void_p()
+        _check(lib().lig_serve_start(device, bits, n_slots, hbm_budget, C.byref(h)))
+        self._h, self.bits = h, bits
+        self._keep = {}
+
+    def submit(self, log_n: int, poly, fs_seed: int = 1234) -> int:
+        """poly: host array (kept referenced until its ticket completes) or a host pointer (int)."""
+        t = C.c_uint64()
+        if isinstance(poly, int):
+            ptr, keep = poly, None
+        else:
+            keep = as_elems(poly, self.bits)
+            ptr = keep.ctypes.data
+        _check(lib().lig_serve_submit(self._h, log_n, C.c_void_p(ptr), fs_seed, C.byref(t)))
+        self._keep[t.value] = keep
+        return t.value
+
+    def _take(self, fn, ticket):
+        h = C.c_void_p()
+        lat = C.c_double(0.0)
+        r = _check(fn(self._h, ticket, C.byref(h), C.byref(lat)))
+        if r == 0:
+            return None
+        self._keep.pop(ticket, None)
+        return Proof(h, self.bits), lat.value
+
+    def poll(self, ticket: int):
+        """(proof, latency_ms) once done, else None."""
+        return self._take(lib().lig_serve_poll, ticket)
+
+    def wait(self, ticket: int):
+        return self._take(lib().lig_serve_wait, ticket)
+
+    def stop(self):
+        if getattr(self, "_h", None):
+            lib().lig_serve_stop(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        self.stop()

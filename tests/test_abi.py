@@ -30,6 +30,17 @@ def test_no_device_is_an_error_not_a_fallback(L):
     assert e.value.code == -5  # LIG_E_NO_DEVICE
 
 
+def test_batch_and_server_need_a_device(L):
+    if L.lig_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(lh.LigeritoError) as e:
+        lh.Batch(0, 32, 2)
+    assert e.value.code == -5
+    with pytest.raises(lh.LigeritoError) as e:
+        lh.Server(0, 32, 2)
+    assert e.value.code == -5
+
+
 def test_transcript_matches_oracle(O):
     for seed in (1234, 0, 77):
         a, b = lh.FS(seed), O.FS(seed)
