@@ -298,11 +298,26 @@ template <class H> void open_finish(PendingOpen<H>& po) {
     out.nodes.assign(po.h_nodes, po.h_nodes + po.n_nodes * 32);
 }
 
+// The host transcript in the device layout: pending bytes as big-endian
+// words, zero past the pending length.
+void export_transcript(const Transcript& fs, DevTranscript& d) {
+    LIG_REQUIRE(fs.hasher.blen % 4 == 0, LIG_E_ASSERT, "transcript length is not a whole number of words");
+    memset(&d, 0, sizeof(d));
+    memcpy(d.h, fs.hasher.h, sizeof(d.h));
+    for (uint32_t i = 0; i < fs.hasher.blen; i++) d.buf[i >> 2] |= (uint32_t)fs.hasher.buf[i] << (24 - 8 * (i & 3));
+    d.blen = fs.hasher.blen;
+    d.total_lo = (uint32_t)fs.hasher.total;
+    d.total_hi = (uint32_t)(fs.hasher.total >> 32);
+    d.counter = fs.counter;
+}
+
 // induce_sumcheck_poly (sumcheck_polys.rs:74-160) entirely on the device:
-// basis polynomial into d_basis, enforced sum (unreduced) into d_enf.
+// basis polynomial into d_basis, enforced sum (unreduced) as *n_enf parts
+// at d_enf.
 template <class H>
 void induce_launch(Ctx* ctx, int ell, const PendingOpen<H>& po, size_t n_cols, const Vec<H>& gr, const std::vector<size_t>& q,
-                   typename H::T alpha, PBuf<typename H::T>& d_basis, typename DevField<H>::F::Acc* d_enf, size_t pin_off) {
+                   typename H::T alpha, PBuf<typename H::T>& d_basis, typename DevField<H>::F::Acc* d_enf, int* n_enf,
+                   size_t pin_off) {
     using F = typename DevField<H>::F;
     using T = typename H::T;
     hipStream_t s = ctx->stream;
@@ -328,6 +343,7 @@ void induce_launch(Ctx* ctx, int ell, const PendingOpen<H>& po, size_t n_cols, c
         PhaseTimer pt(ctx, PH_INDUCE);
         launch_induce_prep<F>(po.d_q.p, d_vals.p, d_vals.p + nvals - 1, (int)nq, ell, d_c.p, d_ap.p, s);
         launch_enforced<F>(po.d_rows.p, (int)nq, (int)n_cols, d_vals.p + sks.size(), d_ap.p, d_enf, s);
+        *n_enf = enforced_parts((int)nq, (int)n_cols);
         launch_induce<F>(d_c.p, d_ap.p, (int)nq, ell, d_P.p, d_K.p, d_basis.p, s);
     }
     LIG_HIP(hipGetLastError());
@@ -346,9 +362,10 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
 
     proof = Proof<H>();
     // Pinned staging regions (disjoint, so an in-flight copy never reads a
-    // region the host is refilling): [0,256) results, [256,16K) uploads of
-    // induce constants and fold weights, [16K,..) openings and yr.
-    constexpr size_t PIN_RES = 0, PIN_UP = 256, PIN_OPEN = 16384;
+    // region the host is refilling): [0,256) commit roots, [256,16K) uploads
+    // of induce constants and fold weights, [16K,32K) device transcript
+    // hand-over and logs, [32K,2M) openings, [2M,4M) yr.
+    constexpr size_t PIN_UP = 256, PIN_FSUP = 16384, PIN_FSDN = 16384 + 256, PIN_OPEN = 32768, PIN_YR = (size_t)2 << 20;
     ctx->pinned((size_t)4 << 20);
     Witness<H> w0;
     commit<H>(*cfg.initial_rs, d_poly, cfg.initial_m, cfg.initial_n, w0, PH_ENCODE0);
@@ -384,68 +401,107 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     T alpha = fs.get_field<H>();
     int nv = ilog2(flen);
 
-    // two triple accumulators used alternately (each kernel clears the other)
-    // and the enforced sum
-    PBuf<A> d_acc(&ctx->pool, 8);
+    // Sumcheck sections run on the device transcript: the host hands its
+    // transcript over after the section's host-side steps, a chain of
+    // kernels absorbs each triple and squeezes each challenge, and after the
+    // next synchronisation the host replays the section from the logged
+    // triples (checking that it squeezes the same challenges) to continue
+    // from the identical state.
+    int maxk = 0;
+    for (int k : cfg.ks) maxk = std::max(maxk, k);
+    PBuf<DevTranscript> d_fs(&ctx->pool, 1);
+    PBuf<T> d_tlog(&ctx->pool, 3 * (size_t)(maxk + 1)), d_r(&ctx->pool, (size_t)maxk + 2);
+    // two triple accumulators used alternately: each triple kernel adds into
+    // res_next and clears res_cur, then the names swap, so res_cur holds the
+    // latest triple and res_next is clear; plus the enforced sum
+    PBuf<A> d_acc(&ctx->pool, 6 + ENFORCED_MAX_PARTS);
     A* res_cur = d_acc.p;
     A* res_next = d_acc.p + 3;
     A* d_enf = d_acc.p + 6;
-    launch_zero_acc<F>(res_cur, s);
-    A* h_res = (A*)(ctx->pinned(PIN_RES + 4 * sizeof(A)) + PIN_RES);
-    auto fetch = [&](bool with_enf) {
-        LIG_HIP(hipMemcpyAsync(h_res, res_cur, 3 * sizeof(A), hipMemcpyDeviceToHost, s));
-        if (with_enf) LIG_HIP(hipMemcpyAsync(h_res + 3, d_enf, sizeof(A), hipMemcpyDeviceToHost, s));
-    };
+    int n_enf = 0;
+    launch_zero_acc<F>(res_next, s);
     PBuf<T> d_b, d_b2(&ctx->pool, flen / 2);
-    T t[3];
-    {
-        PendingOpen<H> po;
-        open_launch<H>(w0, q, proof.initial, po, PIN_OPEN);
-        induce_launch<H>(ctx, nv, po, w0.n, gr0, q, alpha, d_b, d_enf, PIN_UP);
-        {
-            PhaseTimer pt(ctx, PH_SUMCHECK);
-            launch_eval013<F>(d_f.p, d_b.p, flen, res_cur, res_next, s);
-        }
-        fetch(true);
-        ctx->sync();
-        std::swap(res_cur, res_next);
-        open_finish<H>(po);
-    }
-    for (int i = 0; i < 3; i++) t[i] = acc_to_elem<H>(h_res[i]);
-    T enforced = acc_to_elem<H>(h_res[3]);
-    w0 = Witness<H>();  // release the first witness
-    LIG_REQUIRE(H::eq(t[0] ^ t[1], enforced), LIG_E_ASSERT, "sumcheck: s0 + s1 != enforced sum (ligerito.rs:35)");
-    proof.tr.insert(proof.tr.end(), t, t + 3);
-    fs.absorb_elems<H>(t, 3);
 
-    auto read_triple = [&]() {
-        fetch(false);
-        ctx->sync();
-        std::swap(res_cur, res_next);
-        for (int k = 0; k < 3; k++) t[k] = acc_to_elem<H>(h_res[k]);
-    };
-
-    for (int i = 0; i < cfg.recursive_steps; i++) {
-        Vec<H> rs;
-        for (int j = 0; j < cfg.ks[i]; j++) {
-            T r = fs.get_field<H>();
-            {
-                PhaseTimer pt(ctx, PH_SUMCHECK);
-                launch_sumcheck_fold<F>(d_f.p, d_b.p, flen, r, d_f2.p, d_b2.p, res_cur, res_next, s);
-            }
-            LIG_HIP(hipGetLastError());
-            read_triple();
+    struct Section {
+        int n0 = 0, k = 0;  // squeezes of the leading step, rounds
+    } sec;
+    DevTranscript* h_fsup = (DevTranscript*)(ctx->pin + PIN_FSUP);
+    uint8_t* h_fsdn = ctx->pin + PIN_FSDN;
+    // Launches: leading step (absorb the triple in res_cur, check it against
+    // the enforced sum, squeeze n0), optional glue with the first squeeze,
+    // then k fold rounds, each followed by its transcript step.
+    auto launch_section = [&](int n0, int k, PBuf<T>* d_bnew) {
+        export_transcript(fs, *h_fsup);
+        LIG_HIP(hipMemcpyAsync(d_fs.p, h_fsup, sizeof(DevTranscript), hipMemcpyHostToDevice, s));
+        PhaseTimer pt(ctx, PH_SUMCHECK);
+        launch_fs_step<F>(d_fs.p, res_cur, d_enf, n_enf, d_tlog.p, d_r.p, n0, s);
+        if (d_bnew) launch_glue<F>(d_b.p, d_bnew->p, flen, d_r.p, s);
+        for (int j = 0; j < k; j++) {
+            launch_sumcheck_fold<F>(d_f.p, d_b.p, flen, d_r.p + (n0 - 1) + j, d_f2.p, d_b2.p, res_next, res_cur, s);
+            std::swap(res_cur, res_next);
+            launch_fs_step<F>(d_fs.p, res_cur, nullptr, 0, d_tlog.p + 3 * (j + 1), d_r.p + n0 + j, j + 1 < k ? 1 : 0, s);
             std::swap(d_f, d_f2);
             std::swap(d_b, d_b2);
             flen /= 2;
-            proof.tr.insert(proof.tr.end(), t, t + 3);
-            fs.absorb_elems<H>(t, 3);
-            rs.push_back(r);
         }
+        LIG_HIP(hipGetLastError());
+        sec.n0 = n0;
+        sec.k = k;
+    };
+    const size_t n_tlog = 3 * (size_t)(maxk + 1), n_r = (size_t)maxk + 2;
+    auto download_section = [&]() {
+        LIG_HIP(hipMemcpyAsync(h_fsdn, d_tlog.p, n_tlog * sizeof(T), hipMemcpyDeviceToHost, s));
+        LIG_HIP(hipMemcpyAsync(h_fsdn + n_tlog * sizeof(T), d_r.p, n_r * sizeof(T), hipMemcpyDeviceToHost, s));
+        LIG_HIP(hipMemcpyAsync(h_fsdn + (n_tlog + n_r) * sizeof(T), d_fs.p, sizeof(DevTranscript), hipMemcpyDeviceToHost, s));
+    };
+    Vec<H> fold_rs;  // challenges of the section's fold rounds
+    // after the stream has been synchronized
+    auto replay_section = [&](const char* what) {
+        const T* tl = (const T*)h_fsdn;
+        const T* rr = tl + n_tlog;
+        const DevTranscript* dts = (const DevTranscript*)(rr + n_r);
+        LIG_REQUIRE(dts->err == 0, LIG_E_ASSERT, what);
+        int ri = 0;
+        fold_rs.clear();
+        auto squeeze_check = [&]() {
+            T v = fs.get_field<H>();
+            LIG_REQUIRE(H::eq(v, rr[ri]), LIG_E_ASSERT, "device transcript diverged from the host transcript");
+            if (ri >= sec.n0 - 1) fold_rs.push_back(v);
+            ri++;
+        };
+        proof.tr.insert(proof.tr.end(), tl, tl + 3);
+        fs.absorb_elems<H>(tl, 3);
+        for (int i = 0; i < sec.n0; i++) squeeze_check();
+        for (int j = 0; j < sec.k; j++) {
+            const T* t3 = tl + 3 * (j + 1);
+            proof.tr.insert(proof.tr.end(), t3, t3 + 3);
+            fs.absorb_elems<H>(t3, 3);
+            if (j + 1 < sec.k) squeeze_check();
+        }
+    };
+
+    std::unique_ptr<PendingOpen<H>> pending(new PendingOpen<H>());
+    open_launch<H>(w0, q, proof.initial, *pending, PIN_OPEN);
+    induce_launch<H>(ctx, nv, *pending, w0.n, gr0, q, alpha, d_b, d_enf, &n_enf, PIN_UP);
+    {
+        PhaseTimer pt(ctx, PH_SUMCHECK);
+        launch_eval013<F>(d_f.p, d_b.p, flen, res_next, res_cur, s);
+        std::swap(res_cur, res_next);
+    }
+    // SumcheckProverInstance::new (ligerito.rs:32-41) and the rounds of the
+    // first step (prover.rs:96-102)
+    launch_section(1, cfg.ks[0], nullptr);
+    w0 = Witness<H>();  // the first witness is released once its kernels ran
+
+    for (int i = 0; i < cfg.recursive_steps; i++) {
         if (i == cfg.recursive_steps - 1) {
-            T* hy = (T*)(ctx->pinned(PIN_OPEN + flen * sizeof(T)) + PIN_OPEN);
+            // final step (prover.rs:104-142): yr, queries, openings
+            T* hy = (T*)(ctx->pin + PIN_YR);
+            download_section();
             LIG_HIP(hipMemcpyAsync(hy, d_f.p, flen * sizeof(T), hipMemcpyDeviceToHost, s));
             ctx->sync();
+            replay_section("sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
+            open_finish<H>(*pending);
             proof.yr.assign(hy, hy + flen);
             fs.absorb_elems<H>(proof.yr.data(), flen);
             q = fs.get_distinct_queries(wprev.M, S);
@@ -456,7 +512,10 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             return;
         }
         Witness<H> wnext;
-        commit<H>(*cfg.rs[i + 1], d_f.p, cfg.dims_m[i + 1], cfg.dims_n[i + 1], wnext);
+        download_section();
+        commit<H>(*cfg.rs[i + 1], d_f.p, cfg.dims_m[i + 1], cfg.dims_n[i + 1], wnext);  // synchronizes
+        replay_section("sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
+        open_finish<H>(*pending);
         proof.rec_roots.emplace_back();
         memcpy(proof.rec_roots.back().data(), wnext.root, 32);
         fs.absorb(wnext.root, 32);
@@ -466,34 +525,20 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         nv = ilog2(flen);
         proof.rec_proofs.emplace_back();
         PBuf<T> d_bnew;
-        {
-            PendingOpen<H> po;
-            open_launch<H>(wprev, q, proof.rec_proofs.back(), po, PIN_OPEN);
-            Vec<H> gr = lagrange_basis<H>(rs);
-            induce_launch<H>(ctx, nv, po, wprev.n, gr, q, a, d_bnew, d_enf, PIN_UP);
-            // introduce_new (ligerito.rs:50-57): triple of f with the new basis alone
-            {
-                PhaseTimer pt(ctx, PH_SUMCHECK);
-                launch_eval013<F>(d_f.p, d_bnew.p, flen, res_cur, res_next, s);
-            }
-            fetch(true);
-            ctx->sync();
-            std::swap(res_cur, res_next);
-            open_finish<H>(po);
-        }
-        for (int k = 0; k < 3; k++) t[k] = acc_to_elem<H>(h_res[k]);
-        T enf = acc_to_elem<H>(h_res[3]);
-        LIG_REQUIRE(H::eq(t[0] ^ t[1], enf), LIG_E_ASSERT, "sumcheck: s0 + s1 != h (ligerito.rs:52)");
-        proof.tr.insert(proof.tr.end(), t, t + 3);
-        fs.absorb_elems<H>(t, 3);
-        // glue (ligerito.rs:59-67): the summed basis absorbs beta * b_new;
-        // folds and the 0/1/3 evaluations are linear in the basis polys.
-        T beta = fs.get_field<H>();
+        pending.reset(new PendingOpen<H>());
+        open_launch<H>(wprev, q, proof.rec_proofs.back(), *pending, PIN_OPEN);
+        Vec<H> gr = lagrange_basis<H>(fold_rs);
+        induce_launch<H>(ctx, nv, *pending, wprev.n, gr, q, a, d_bnew, d_enf, &n_enf, PIN_UP);
+        // introduce_new (ligerito.rs:50-57): triple of f with the new basis
+        // alone; then glue (ligerito.rs:59-67) with the next challenge: the
+        // summed basis absorbs beta * b_new (folds and the 0/1/3 evaluations
+        // are linear in the basis polys); then the rounds of step i+1
         {
             PhaseTimer pt(ctx, PH_SUMCHECK);
-            launch_glue<F>(d_b.p, d_bnew.p, flen, beta, s);
+            launch_eval013<F>(d_f.p, d_bnew.p, flen, res_next, res_cur, s);
+            std::swap(res_cur, res_next);
         }
-        LIG_HIP(hipGetLastError());
+        launch_section(2, cfg.ks[i + 1], &d_bnew);
         wprev = std::move(wnext);
     }
     throw Error(LIG_E_ASSERT, "Prover completed without entering final step (prover.rs:198)");

@@ -31,6 +31,13 @@ template <> struct Ops<F32> {
     __device__ static T mul(T a, T b) { return gf32_reduce64(clmul32_portable(a, b)); }
     __device__ static Acc acc_zero() { return 0ull; }
     __device__ static void acc_mul(Acc& acc, T a, T b) { acc ^= clmul32_portable(a, b); }
+    __device__ static T acc_reduce(Acc a) { return gf32_reduce64(a); }
+    __device__ static bool eq(T a, T b) { return a == b; }
+    // little-endian bytes as 32-bit words (BinaryField::to_bytes)
+    static constexpr int WORDS = 1;
+    __device__ static uint32_t word(T v, int) { return v; }
+    // get_field: the low bits of the squeezed u128 (four little-endian words)
+    __device__ static T from_squeeze(const uint32_t o[4]) { return o[0]; }
     // multiplication by the constants 2 and 3 used by the sumcheck (x = 3, 1 + x = 2)
     __device__ static T mul2(T a) { return (a << 1) ^ ((0u - (a >> 31)) & GF32_POLY); }
 };
@@ -54,6 +61,16 @@ template <> struct Ops<F128> {
     __device__ static T mul(T a, T b) { return gf128_mul_portable(a, b); }
     __device__ static Acc acc_zero() { return {0ull, 0ull}; }
     __device__ static void acc_mul(Acc& acc, T a, T b) { acc ^= gf128_mul_portable(a, b); }
+    __device__ static T acc_reduce(Acc a) { return a; }
+    __device__ static bool eq(T a, T b) { return a.lo == b.lo && a.hi == b.hi; }
+    static constexpr int WORDS = 4;
+    __device__ static uint32_t word(T v, int i) {
+        const uint64_t x = i < 2 ? v.lo : v.hi;
+        return (uint32_t)(x >> (32 * (i & 1)));
+    }
+    __device__ static T from_squeeze(const uint32_t o[4]) {
+        return {(uint64_t)o[0] | ((uint64_t)o[1] << 32), (uint64_t)o[2] | ((uint64_t)o[3] << 32)};
+    }
     __device__ static T mul2(T a) {
         uint64_t c = a.hi >> 63;
         return {(a.lo << 1) ^ ((0ull - c) & GF128_POLY), (a.hi << 1) | (a.lo >> 63)};

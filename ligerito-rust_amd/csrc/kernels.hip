@@ -11,6 +11,7 @@
 #include "blake3.hpp"
 #include "fields.hpp"
 #include "kernels.hpp"
+#include "transcript_dev.hpp"
 
 namespace lig {
 
@@ -1005,16 +1006,19 @@ void launch_induce_prep(const uint64_t* q, const typename F::T* sks, const typen
     hipLaunchKernelGGL(k_induce_prep<F>, dim3((nq + 63) / 64), dim3(64), 0, s, q, sks, alpha, nq, ell, cvals, ap);
 }
 
+// Partial enforced sums: workgroup g XOR-accumulates the products of its
+// share of the (row, column) pairs and writes parts[g]; the transcript step
+// that checks the sum adds the parts.
 template <class F>
-__global__ __launch_bounds__(1024) void k_enforced(const typename F::T* __restrict__ rows, int nrows, int ncols,
-                                                   const typename F::T* __restrict__ gr, const typename F::T* __restrict__ ap,
-                                                   typename F::Acc* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_enforced(const typename F::T* __restrict__ rows, int nrows, int ncols,
+                                                  const typename F::T* __restrict__ gr, const typename F::T* __restrict__ ap,
+                                                  typename F::Acc* __restrict__ parts) {
     using T = typename F::T;
     using A = typename F::Acc;
-    __shared__ A red[1024];
+    __shared__ A red[256];
     const int lg = 31 - __clz(ncols);
     A acc = Ops<F>::acc_zero();
-    for (int p = threadIdx.x; p < nrows * ncols; p += blockDim.x) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nrows * ncols; p += gridDim.x * blockDim.x) {
         const T prod = Ops<F>::mul(rows[p], gr[p & (ncols - 1)]);
         Ops<F>::acc_mul(acc, prod, ap[p >> lg]);
     }
@@ -1024,13 +1028,18 @@ __global__ __launch_bounds__(1024) void k_enforced(const typename F::T* __restri
         if ((int)threadIdx.x < w) red[threadIdx.x] ^= red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = red[0];
+    if (threadIdx.x == 0) parts[blockIdx.x] = red[0];
+}
+
+int enforced_parts(int nrows, int ncols) {
+    const int g = (nrows * ncols + 255) / 256;
+    return g < 1 ? 1 : (g > ENFORCED_MAX_PARTS ? ENFORCED_MAX_PARTS : g);
 }
 
 template <class F>
 void launch_enforced(const typename F::T* rows, int nrows, int ncols, const typename F::T* gr, const typename F::T* ap,
-                     typename F::Acc* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_enforced<F>, dim3(1), dim3(1024), 0, s, rows, nrows, ncols, gr, ap, out);
+                     typename F::Acc* parts, hipStream_t s) {
+    hipLaunchKernelGGL(k_enforced<F>, dim3(enforced_parts(nrows, ncols)), dim3(256), 0, s, rows, nrows, ncols, gr, ap, parts);
 }
 
 // -------------------------------------------------------------- sumcheck ---
@@ -1102,12 +1111,13 @@ template <class F> void launch_zero_acc(typename F::Acc* res, hipStream_t s) {
 
 template <class F>
 __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __restrict__ f, const typename F::T* __restrict__ b,
-                                                       size_t len, typename F::T r, typename F::T* __restrict__ f2,
-                                                       typename F::T* __restrict__ b2, typename F::Acc* res,
-                                                       typename F::Acc* res_clear) {
+                                                       size_t len, const typename F::T* __restrict__ rp,
+                                                       typename F::T* __restrict__ f2, typename F::T* __restrict__ b2,
+                                                       typename F::Acc* res, typename F::Acc* res_clear) {
     using T = typename F::T;
     using A = typename F::Acc;
     if (blockIdx.x == 0 && threadIdx.x < 3) res_clear[threadIdx.x] = A{};
+    const T r = *rp;
     __shared__ T tab[F::TAB];
     for (int e = threadIdx.x; e < F::TAB; e += blockDim.x) tab[e] = Ops<F>::table_entry(r, e);
     __syncthreads();
@@ -1142,16 +1152,19 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
 }
 
 template <class F>
-void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t len, typename F::T r, typename F::T* f2,
-                          typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear, hipStream_t s) {
+void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t len, const typename F::T* r,
+                          typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
+                          hipStream_t s) {
     size_t h2 = len / 4;
     hipLaunchKernelGGL(k_sumcheck_fold<F>, dim3(grid_for(h2 ? h2 : 1, 256, 1024)), dim3(256), 0, s, f, b, len, r, f2, b2, res,
                        res_clear);
 }
 
 template <class F>
-__global__ void k_glue(typename F::T* __restrict__ bsum, const typename F::T* __restrict__ bnew, size_t len, typename F::T beta) {
+__global__ void k_glue(typename F::T* __restrict__ bsum, const typename F::T* __restrict__ bnew, size_t len,
+                       const typename F::T* __restrict__ betap) {
     using T = typename F::T;
+    const T beta = *betap;
     __shared__ T tab[F::TAB];
     for (int e = threadIdx.x; e < F::TAB; e += blockDim.x) tab[e] = Ops<F>::table_entry(beta, e);
     __syncthreads();
@@ -1159,8 +1172,54 @@ __global__ void k_glue(typename F::T* __restrict__ bsum, const typename F::T* __
         bsum[i] ^= Ops<F>::mul_tab(tab, bnew[i]);
 }
 
-template <class F> void launch_glue(typename F::T* bsum, const typename F::T* bnew, size_t len, typename F::T beta, hipStream_t s) {
+template <class F>
+void launch_glue(typename F::T* bsum, const typename F::T* bnew, size_t len, const typename F::T* beta, hipStream_t s) {
     hipLaunchKernelGGL(k_glue<F>, dim3(grid_for(len, 256, 2048)), dim3(256), 0, s, bsum, bnew, len, beta);
+}
+
+// One wave runs the transcript step; every lane computes the same values
+// and lane 0 writes them.
+template <class F>
+__global__ __launch_bounds__(64) void k_fs_step(DevTranscript* __restrict__ st, const typename F::Acc* __restrict__ res,
+                                                const typename F::Acc* __restrict__ enf, int n_enf,
+                                                typename F::T* __restrict__ tlog, typename F::T* __restrict__ rout,
+                                                int n_squeeze) {
+    using T = typename F::T;
+    using A = typename F::Acc;
+    dfs::State s;
+    s.load(st);
+    uint32_t err = 0;
+    if (res) {
+        T t[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) t[k] = Ops<F>::acc_reduce(dfs::uniform_acc(res[k]));
+        if (enf) {
+            A e = Ops<F>::acc_zero();
+            for (int i = 0; i < n_enf; i++) e ^= dfs::uniform_acc(enf[i]);
+            if (!Ops<F>::eq(t[0] ^ t[1], Ops<F>::acc_reduce(e))) err = 1;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int i = 0; i < Ops<F>::WORDS; i++) s.word_le(Ops<F>::word(t[k], i));
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 3; k++) tlog[k] = t[k];
+    }
+    for (int i = 0; i < n_squeeze; i++) {
+        uint32_t o[4];
+        s.squeeze(o);
+        if (threadIdx.x == 0) rout[i] = Ops<F>::from_squeeze(o);
+    }
+    if (threadIdx.x == 0) {
+        s.store(st);
+        if (err) st->err = 1;
+    }
+}
+
+template <class F>
+void launch_fs_step(DevTranscript* st, const typename F::Acc* res, const typename F::Acc* enf, int n_enf,
+                    typename F::T* tlog, typename F::T* rout, int n_squeeze, hipStream_t s) {
+    hipLaunchKernelGGL(k_fs_step<F>, dim3(1), dim3(64), 0, s, st, res, enf, n_enf, tlog, rout, n_squeeze);
 }
 
 // ------------------------------------------------------- instantiations ---
@@ -1177,9 +1236,10 @@ template <class F> void launch_glue(typename F::T* bsum, const typename F::T* bn
     template void launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t);                   \
     template void launch_eval013<F>(const F::T*, const F::T*, size_t, F::Acc*, F::Acc*, hipStream_t);                       \
     template void launch_zero_acc<F>(F::Acc*, hipStream_t);                                                                 \
-    template void launch_sumcheck_fold<F>(const F::T*, const F::T*, size_t, F::T, F::T*, F::T*, F::Acc*, F::Acc*,         \
+    template void launch_sumcheck_fold<F>(const F::T*, const F::T*, size_t, const F::T*, F::T*, F::T*, F::Acc*, F::Acc*,  \
                                           hipStream_t);                                                                     \
-    template void launch_glue<F>(F::T*, const F::T*, size_t, F::T, hipStream_t);                                           \
+    template void launch_fs_step<F>(DevTranscript*, const F::Acc*, const F::Acc*, int, F::T*, F::T*, int, hipStream_t);     \
+    template void launch_glue<F>(F::T*, const F::T*, size_t, const F::T*, hipStream_t);                                           \
     template void launch_induce_prep<F>(const uint64_t*, const F::T*, const F::T*, int, int, F::T*, F::T*, hipStream_t);   \
     template void launch_enforced<F>(const F::T*, int, int, const F::T*, const F::T*, F::Acc*, hipStream_t);
 

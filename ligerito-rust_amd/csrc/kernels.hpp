@@ -26,6 +26,17 @@ struct F128 {
     static constexpr int TAB_STRIDE = 512;
 };
 
+// Device copy of a Fiat–Shamir transcript (host Transcript: SHA-256 state,
+// pending block as big-endian words zero past blen, byte count, squeeze
+// counter), plus a flag device-side checks set.
+struct DevTranscript {
+    uint32_t h[8];
+    uint32_t buf[16];
+    uint32_t blen, total_lo, total_hi, counter;
+    uint32_t err;
+    uint32_t pad[3];
+};
+
 // FFT tile shape per field: up to 2^K rows x C columns per pass.
 template <class F> struct PassShape;
 
@@ -95,11 +106,13 @@ void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, 
 template <class F>
 void launch_induce_prep(const uint64_t* q, const typename F::T* sks, const typename F::T* alpha, int nq, int ell,
                         typename F::T* cvals, typename F::T* ap, hipStream_t s);
-// out[0] = sum_i ap[i] * (row_i . gr) (unreduced), the enforced sum of
-// sumcheck_polys.rs:249-256.
+// The enforced sum of sumcheck_polys.rs:249-256, sum_i ap[i] * (row_i . gr)
+// (unreduced), as enforced_parts(nrows, ncols) parts whose XOR is the sum.
+constexpr int ENFORCED_MAX_PARTS = 64;
+int enforced_parts(int nrows, int ncols);
 template <class F>
 void launch_enforced(const typename F::T* rows, int nrows, int ncols, const typename F::T* gr, const typename F::T* ap,
-                     typename F::Acc* out, hipStream_t s);
+                     typename F::Acc* parts, hipStream_t s);
 
 // ------------------------------------------------------------ sumcheck ----
 // res[0..3) = (sum f0*g0, sum f1*g1, sum f2*g2) unreduced (eval_013_product).
@@ -111,11 +124,22 @@ void launch_eval013(const typename F::T* f, const typename F::T* g, size_t len, 
 template <class F> void launch_zero_acc(typename F::Acc* res, hipStream_t s);
 // Folds f and b by r (length len -> len/2) into f2, b2 and writes the
 // triple of the folded pair (ligerito.rs:119-127).
+// r is read from device memory (written by launch_fs_step).
 template <class F>
-void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t len, typename F::T r, typename F::T* f2,
-                          typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear, hipStream_t s);
+void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t len, const typename F::T* r,
+                          typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
+                          hipStream_t s);
+// One transcript step on the device: if res is given, reduce the triple
+// (three accumulators) and absorb it (QuadraticEvals::to_bytes,
+// sumcheck.rs:29-35), logging it to tlog and, if enf is given, flagging
+// st->err when t0 + t1 differs from the enforced sum, the XOR of the n_enf
+// parts at enf (ligerito.rs:35,52); then squeeze n_squeeze field elements
+// into rout (get_field).
+template <class F>
+void launch_fs_step(DevTranscript* st, const typename F::Acc* res, const typename F::Acc* enf, int n_enf,
+                    typename F::T* tlog, typename F::T* rout, int n_squeeze, hipStream_t s);
 // bsum[i] ^= beta * bnew[i]   (glue, ligerito.rs:59-67)
 template <class F>
-void launch_glue(typename F::T* bsum, const typename F::T* bnew, size_t len, typename F::T beta, hipStream_t s);
+void launch_glue(typename F::T* bsum, const typename F::T* bnew, size_t len, const typename F::T* beta, hipStream_t s);
 
 }  // namespace lig
