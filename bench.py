@@ -105,9 +105,18 @@ def main():
                                                "pass64")}
     ctx.profile(False)
 
+    t_v = time.perf_counter()
     ok = lh.verifier(L, proof, FS_SEED)
+    t_v = time.perf_counter() - t_v
     if ok != 1:
         log(f"verifier rejected the GPU proof ({ok})")
+        sys.exit(1)
+    lh.verifier_dev(ctx, L, proof, FS_SEED)
+    t_vd = time.perf_counter()
+    ok = lh.verifier_dev(ctx, L, proof, FS_SEED)
+    t_vd = time.perf_counter() - t_vd
+    if ok != 1:
+        log(f"GPU-induce verifier rejected the GPU proof ({ok})")
         sys.exit(1)
 
     # host-resident input (PCIe-inclusive drop-in semantics), reported beside
@@ -172,6 +181,7 @@ def main():
         "cpu_baseline": cpu,
         "phases_ms_per_prove": {k: v[0] / n_prof for k, v in phases.items()},
         "host_input_coeffs_per_s": N / t_host,
+        "verify_ms": {"host": t_v * 1e3, "gpu_induce": t_vd * 1e3},
         "batched": batched,
         "served": served,
     }

@@ -172,6 +172,10 @@ void lig_proof_free(lig_proof *p);
  * CPU).  Returns 1 (accept), 0 (final sumcheck check false), or
  * LIG_E_MERKLE / LIG_E_LIGERO / LIG_E_ASSERT where the reference panics. */
 int lig_verifier(int field_bits, int log_n, const lig_proof *p, uint32_t fs_seed);
+/* The same verifier with its basis inductions (induce_sumcheck_poly at
+ * verifier.rs:54-61,148-155, the bulk of its arithmetic) on the context's
+ * GPU; same results and status codes as lig_verifier. */
+int lig_verifier_dev(lig_ctx *ctx, int field_bits, int log_n, const lig_proof *p, uint32_t fs_seed);
 
 /* ---------------------------------------------------------------- batch -- */
 /* No counterpart in the reference (its prover runs one proof at a time):

@@ -525,6 +525,16 @@ int lig_verifier(int field_bits, int log_n, const lig_proof* p, uint32_t fs_seed
     });
 }
 
+int lig_verifier_dev(lig_ctx* ctx, int field_bits, int log_n, const lig_proof* p, uint32_t fs_seed) {
+    return guarded([&] {
+        LIG_REQUIRE(ctx && p, LIG_E_INVALID, "NULL argument");
+        LIG_REQUIRE(p->bits == field_bits, LIG_E_INVALID, "proof field does not match field_bits");
+        LIG_HIP(hipSetDevice(ctx->c.device));
+        return field_bits == 32 ? verify<H32>(log_n, p->p32, fs_seed, &ctx->c)
+                                : verify<H128>(log_n, p->p128, fs_seed, &ctx->c);
+    });
+}
+
 // ------------------------------------------------------------------ batch --
 int lig_batch_create(int device, int field_bits, int n_streams, lig_batch** out) {
     return guarded([&] {

@@ -242,9 +242,6 @@ template <class H> void Config<H>::init(Ctx* c, int L) {
 // ----------------------------------------------------------------- prover --
 namespace {
 
-template <class H> typename H::T acc_to_elem(const typename DevField<H>::F::Acc& a);
-template <> uint32_t acc_to_elem<H32>(const uint64_t& a) { return gf32_reduce64(a); }
-template <> u128v acc_to_elem<H128>(const u128v& a) { return a; }
 
 // Openings (prover.rs:72-76 / 114-125, merkle_tree.rs:98-111) whose copies
 // to pinned host memory are in flight on the context stream.
@@ -332,9 +329,7 @@ void induce_launch(Ctx* ctx, int ell, const PendingOpen<H>& po, size_t n_cols, c
     memcpy(h, sks.data(), sks.size() * sizeof(T));
     memcpy(h + sks.size(), gr.data(), gr.size() * sizeof(T));
     h[nvals - 1] = alpha;
-    int clo = ell - 8;
-    if (clo < 8) clo = ell < 8 ? ell : 8;
-    if (clo > 10) clo = 10;
+    const int clo = induce_low_bits(ell);
     PBuf<T> d_vals(&ctx->pool, nvals), d_c(&ctx->pool, nq * ell), d_ap(&ctx->pool, nq);
     PBuf<T> d_P(&ctx->pool, nq << clo), d_K(&ctx->pool, nq << (ell - clo));
     LIG_HIP(hipMemcpyAsync(d_vals.p, h, nvals * sizeof(T), hipMemcpyHostToDevice, s));

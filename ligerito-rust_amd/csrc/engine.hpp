@@ -115,6 +115,11 @@ template <class H> struct DevField;
 template <> struct DevField<H32> { using F = F32; };
 template <> struct DevField<H128> { using F = F128; };
 
+// An unreduced device accumulator as a field element.
+template <class H> typename H::T acc_reduce_host(const typename DevField<H>::F::Acc& a);
+template <> inline uint32_t acc_reduce_host<H32>(const uint64_t& a) { return gf32_reduce64(a); }
+template <> inline u128v acc_reduce_host<H128>(const u128v& a) { return a; }
+
 // ReedSolomonEncoding<F> (reed_solomon.rs:7-64) with device tables.
 template <class H> struct RsPlan {
     using T = typename H::T;
@@ -181,7 +186,9 @@ template <class H> struct Proof {
 // prover (prover.rs:15-199) on a device-resident poly.
 template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<H>& out);
 // verifier (verifier.rs:10-164) on the host; 1 accept, 0 reject, <0 panic sites.
-template <class H> int verify(int L, const Proof<H>& p, uint32_t fs_seed);
+// verifier (verifier.rs:10-164) on the host; with dev, the basis
+// inductions (verifier.rs:54-61,148-155) run on that context's GPU.
+template <class H> int verify(int L, const Proof<H>& p, uint32_t fs_seed, Ctx* dev = nullptr);
 template <class H> std::vector<uint8_t> serialize(const Proof<H>& p);
 template <class H> bool deserialize(const uint8_t* b, size_t len, Proof<H>& p);
 template <class H> size_t proof_size(const Proof<H>& p);

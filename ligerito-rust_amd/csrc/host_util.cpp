@@ -167,8 +167,25 @@ void Transcript::squeeze(uint64_t* lo, uint64_t* hi) {
     memcpy(b, &counter, 4);
     hasher.update(b, 4);
     counter++;
-    uint8_t d[32];
-    hasher.digest(d);
+    // finalize a copy: pending bytes, 0x80, zeros, 64-bit big-endian length
+    uint8_t blk[128];
+    const uint32_t n = hasher.blen;
+    memcpy(blk, hasher.buf, n);
+    blk[n] = 0x80;
+    const size_t len = n < 56 ? 64 : 128;
+    memset(blk + n + 1, 0, len - 8 - n - 1);
+    const uint64_t bits = hasher.total * 8;
+    for (int i = 0; i < 8; i++) blk[len - 8 + i] = (uint8_t)(bits >> (56 - 8 * i));
+    uint32_t st[8];
+    memcpy(st, hasher.h, 32);
+    compress(st, blk, len / 64);
+    uint8_t d[16];
+    for (int i = 0; i < 4; i++) {
+        d[4 * i] = (uint8_t)(st[i] >> 24);
+        d[4 * i + 1] = (uint8_t)(st[i] >> 16);
+        d[4 * i + 2] = (uint8_t)(st[i] >> 8);
+        d[4 * i + 3] = (uint8_t)st[i];
+    }
     memcpy(lo, d, 8);
     memcpy(hi, d + 8, 8);
 }

@@ -648,8 +648,49 @@ __global__ void k_leaf_hash(const typename F::T* __restrict__ mat, size_t rows, 
     }
 }
 
+// Rows of NB whole 64-byte blocks (one chunk): each lane loads its whole row
+// first (all loads in flight together, so each row's cache lines are read
+// from HBM once), then chains the NB compressions from registers.
+template <int NB>
+__global__ __launch_bounds__(128) void k_leaf_hash_blocks(const uint32_t* __restrict__ mat, size_t rows,
+                                                          uint32_t* __restrict__ layer0) {
+    for (size_t row = blockIdx.x * (size_t)blockDim.x + threadIdx.x; row < rows; row += (size_t)gridDim.x * blockDim.x) {
+        const uint4* p = reinterpret_cast<const uint4*>(mat + row * 16 * NB);
+        uint4 v[4 * NB];
+#pragma unroll
+        for (int i = 0; i < 4 * NB; i++) v[i] = p[i];
+        uint32_t cv[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) cv[i] = b3::iv(i);
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            uint32_t m[16];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                m[4 * i] = v[4 * b + i].x;
+                m[4 * i + 1] = v[4 * b + i].y;
+                m[4 * i + 2] = v[4 * b + i].z;
+                m[4 * i + 3] = v[4 * b + i].w;
+            }
+            const uint32_t fl = (b == 0 ? b3::CHUNK_START : 0u) | (b == NB - 1 ? (b3::CHUNK_END | b3::ROOT) : 0u);
+            b3::compress_cv(cv, m, 0, 64, fl, cv);
+        }
+        uint4* o = reinterpret_cast<uint4*>(layer0 + row * 8);
+        o[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
+        o[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+    }
+}
+
 template <class F> void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, hipStream_t s) {
-    hipLaunchKernelGGL(k_leaf_hash<F>, dim3(grid_for(rows, 128)), dim3(128), 0, s, mat, rows, n, layer0);
+    const size_t words = n * (sizeof(typename F::T) / 4);
+    const dim3 grid(grid_for(rows, 128)), block(128);
+    const uint32_t* m = reinterpret_cast<const uint32_t*>(mat);
+    switch (words) {
+        case 16: hipLaunchKernelGGL(k_leaf_hash_blocks<1>, grid, block, 0, s, m, rows, layer0); return;
+        case 32: hipLaunchKernelGGL(k_leaf_hash_blocks<2>, grid, block, 0, s, m, rows, layer0); return;
+        case 64: hipLaunchKernelGGL(k_leaf_hash_blocks<4>, grid, block, 0, s, m, rows, layer0); return;
+        default: hipLaunchKernelGGL(k_leaf_hash<F>, grid, block, 0, s, mat, rows, n, layer0); return;
+    }
 }
 
 // ----------------------------------------------------------- Merkle levels --
@@ -692,8 +733,39 @@ __global__ __launch_bounds__(256) void k_merkle_subtree(uint32_t* __restrict__ t
     }
 }
 
+// One tree level: each thread hashes one pair of children.
+__global__ __launch_bounds__(256) void k_merkle_level(uint32_t* __restrict__ tree, size_t in_off, size_t n_out) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_out; i += (size_t)gridDim.x * blockDim.x) {
+        const uint4* src = reinterpret_cast<const uint4*>(tree + (in_off + 2 * i) * 8);
+        uint32_t msg[16];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint4 v = src[k];
+            msg[4 * k] = v.x;
+            msg[4 * k + 1] = v.y;
+            msg[4 * k + 2] = v.z;
+            msg[4 * k + 3] = v.w;
+        }
+        uint32_t out[8];
+        b3::hash64(msg, out);
+        uint4* dst = reinterpret_cast<uint4*>(tree + (in_off + 2 * n_out + i) * 8);
+        dst[0] = make_uint4(out[0], out[1], out[2], out[3]);
+        dst[1] = make_uint4(out[4], out[5], out[6], out[7]);
+    }
+}
+
+// Wide levels one launch each (every lane busy); once a level is narrower
+// than MERKLE_WIDE parents, workgroups of k_merkle_subtree finish the top.
+constexpr size_t MERKLE_WIDE = (size_t)1 << 15;
+
 void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s) {
     size_t in_off = 0, nodes = n_leaves;
+    while (nodes / 2 >= MERKLE_WIDE) {
+        const size_t n_out = nodes / 2;
+        hipLaunchKernelGGL(k_merkle_level, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, s, tree, in_off, n_out);
+        in_off += nodes;
+        nodes = n_out;
+    }
     while (nodes > 1) {
         int lg = 0;
         while (((size_t)1 << lg) < nodes) lg++;
@@ -864,6 +936,13 @@ __global__ __launch_bounds__(256) void k_induce_accum32(const uint32_t* __restri
     }
 }
 
+int induce_low_bits(int ell) {
+    int clo = ell - 8;
+    if (clo < 8) clo = ell < 8 ? ell : 8;
+    if (clo > 10) clo = 10;
+    return clo;
+}
+
 // GF(2^32) variant: P and K by doubling, P[x + 2^t] = P[x] * c_t, with
 // the per-query tables of every c_t in LDS.
 constexpr int TENSOR_MAXK = 4096;
@@ -901,9 +980,7 @@ __global__ __launch_bounds__(256) void k_induce_tensor32(const uint32_t* __restr
 template <class F>
 void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
                    typename F::T* basis, hipStream_t s) {
-    int clo = ell - 8;
-    if (clo < 8) clo = ell < 8 ? ell : 8;
-    if (clo > 10) clo = 10;
+    const int clo = induce_low_bits(ell);
     size_t nk = (size_t)1 << (ell - clo);
     if constexpr (F::BITS == 32) {
         if (ell <= PREP_MAXELL && nk <= (size_t)TENSOR_MAXK && clo <= 10)

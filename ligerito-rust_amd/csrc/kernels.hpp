@@ -96,6 +96,9 @@ void launch_gather_rows(const typename F::T* mat, size_t n, const uint64_t* rows
                         hipStream_t s);
 
 // ------------------------------------------------------------ induce ------
+// The basis index splits into clo low bits (P tables, nq << clo elements)
+// and ell - clo high bits (K tables, nq << (ell - clo) elements).
+int induce_low_bits(int ell);
 // cvals[i*ell + t] = W_t(q_i); ap[i] = alpha^i.  Produces basis[2^ell].
 template <class F>
 void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,

@@ -147,8 +147,57 @@ template <class H> bool ligero_verify(const std::vector<size_t>& q, const Rlp<H>
 }
 }  // namespace
 
-template <class H> int verify(int L, const Proof<H>& p, uint32_t fs_seed) {
+namespace {
+// induce_sumcheck_poly (sumcheck_polys.rs:74-160) for the verifier on the
+// GPU: the prover's induce kernels on the opened rows, the basis polynomial
+// and the enforced-sum parts copied back.
+template <class H>
+void induce_device(Ctx* ctx, int n, const Vec<H>& sks, const std::vector<typename H::T>& rows, size_t n_rows, size_t n_cols,
+                   const Vec<H>& v_ch, const std::vector<size_t>& queries, typename H::T alpha, Vec<H>& basis_out,
+                   typename H::T& enforced) {
+    using F = typename DevField<H>::F;
     using T = typename H::T;
+    using A = typename F::Acc;
+    LIG_REQUIRE(((size_t)1 << v_ch.size()) == n_cols, LIG_E_ASSERT, "opened row length != 2^k");
+    LIG_REQUIRE(rows.size() == n_rows * n_cols && queries.size() == n_rows, LIG_E_ASSERT, "opening shape");
+    Vec<H> gr = lagrange_basis<H>(v_ch);
+    hipStream_t s = ctx->stream;
+    std::vector<uint64_t> q64(queries.begin(), queries.end());
+    std::vector<T> vals(sks.begin(), sks.end());
+    vals.insert(vals.end(), gr.begin(), gr.end());
+    vals.push_back(alpha);
+    const int clo = induce_low_bits(n);
+    PBuf<uint64_t> d_q(&ctx->pool, n_rows);
+    PBuf<T> d_rows(&ctx->pool, rows.size()), d_vals(&ctx->pool, vals.size());
+    PBuf<T> d_c(&ctx->pool, n_rows * n), d_ap(&ctx->pool, n_rows);
+    PBuf<T> d_P(&ctx->pool, n_rows << clo), d_K(&ctx->pool, n_rows << (n - clo)), d_basis(&ctx->pool, (size_t)1 << n);
+    PBuf<A> d_parts(&ctx->pool, ENFORCED_MAX_PARTS);
+    LIG_HIP(hipMemcpyAsync(d_q.p, q64.data(), n_rows * 8, hipMemcpyHostToDevice, s));
+    LIG_HIP(hipMemcpyAsync(d_rows.p, rows.data(), rows.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    LIG_HIP(hipMemcpyAsync(d_vals.p, vals.data(), vals.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    launch_induce_prep<F>(d_q.p, d_vals.p, d_vals.p + vals.size() - 1, (int)n_rows, n, d_c.p, d_ap.p, s);
+    launch_enforced<F>(d_rows.p, (int)n_rows, (int)n_cols, d_vals.p + sks.size(), d_ap.p, d_parts.p, s);
+    launch_induce<F>(d_c.p, d_ap.p, (int)n_rows, n, d_P.p, d_K.p, d_basis.p, s);
+    LIG_HIP(hipGetLastError());
+    basis_out.resize((size_t)1 << n);
+    std::vector<A> parts(ENFORCED_MAX_PARTS);
+    const int np = enforced_parts((int)n_rows, (int)n_cols);
+    LIG_HIP(hipMemcpyAsync(basis_out.data(), d_basis.p, basis_out.size() * sizeof(T), hipMemcpyDeviceToHost, s));
+    LIG_HIP(hipMemcpyAsync(parts.data(), d_parts.p, np * sizeof(A), hipMemcpyDeviceToHost, s));
+    ctx->sync();
+    A acc{};
+    for (int i = 0; i < np; i++) acc ^= parts[i];
+    enforced = acc_reduce_host<H>(acc);
+}
+}  // namespace
+
+template <class H> int verify(int L, const Proof<H>& p, uint32_t fs_seed, Ctx* dev) {
+    using T = typename H::T;
+    auto induce = [&](int n, const Vec<H>& sks, const std::vector<T>& rows, size_t n_rows, size_t n_cols, const Vec<H>& v_ch,
+                      const std::vector<size_t>& queries, T alpha, Vec<H>& basis_out, T& enforced) {
+        if (dev) induce_device<H>(dev, n, sks, rows, n_rows, n_cols, v_ch, queries, alpha, basis_out, enforced);
+        else induce_host<H>(n, sks, rows, n_rows, n_cols, v_ch, queries, alpha, basis_out, enforced);
+    };
     LIG_REQUIRE(L >= 16 && L <= 27, LIG_E_INVALID, "verifier config: log_n must be in [16, 27]");
     const int R = config_steps(L), initial_dim = L - 6, initial_k = 6, log_inv_rate = 2;
     std::vector<int> log_dims, ks;
@@ -171,7 +220,7 @@ template <class H> int verify(int L, const Proof<H>& p, uint32_t fs_seed) {
     Vec<H> sks = eval_sk_at_vks<H>((size_t)1 << initial_dim);
     Vec<H> b1;
     T h1;
-    induce_host<H>(initial_dim, sks, p.initial.rows, S, p.initial.n_cols, pe0, q, alpha, b1, h1);
+    induce(initial_dim, sks, p.initial.rows, S, p.initial.n_cols, pe0, q, alpha, b1, h1);
 
     // SumcheckVerifierInstance (ligerito.rs:143-296)
     std::vector<Vec<H>> basis{b1};
@@ -238,7 +287,7 @@ template <class H> int verify(int L, const Proof<H>& p, uint32_t fs_seed) {
         sks = eval_sk_at_vks<H>((size_t)1 << log_dims[i]);
         Vec<H> bn;
         T h;
-        induce_host<H>(log_dims[i], sks, lp.rows, S, lp.n_cols, rs, q, a, bn, h);
+        induce(log_dims[i], sks, lp.rows, S, lp.n_cols, rs, q, a, bn, h);
         read_tr(g);
         if (!H::eq(g[0] ^ g[1], h)) return LIG_E_ASSERT;
         basis.push_back(bn);
@@ -257,7 +306,7 @@ template bool deserialize<H32>(const uint8_t*, size_t, Proof<H32>&);
 template bool deserialize<H128>(const uint8_t*, size_t, Proof<H128>&);
 template size_t proof_size<H32>(const Proof<H32>&);
 template size_t proof_size<H128>(const Proof<H128>&);
-template int verify<H32>(int, const Proof<H32>&, uint32_t);
-template int verify<H128>(int, const Proof<H128>&, uint32_t);
+template int verify<H32>(int, const Proof<H32>&, uint32_t, Ctx*);
+template int verify<H128>(int, const Proof<H128>&, uint32_t, Ctx*);
 
 }  // namespace lig

@@ -89,6 +89,7 @@ def lib():
             "lig_proof_size": (SZ, [P]),
             "lig_proof_free": (None, [P]),
             "lig_verifier": (I, [I, I, P, U32]),
+            "lig_verifier_dev": (I, [P, I, I, P, U32]),
             "lig_batch_create": (I, [I, I, I, PP]),
             "lig_batch_destroy": (None, [P]),
             "lig_prove_batch": (I, [P, SZ, C.POINTER(C.c_int), PP, I, C.POINTER(U32), PP]),
@@ -419,6 +420,11 @@ def prover_dev(config: ProverConfig, d_poly_ptr: int, fs_seed: int = 1234) -> Pr
 def verifier(log_n: int, proof: Proof, fs_seed: int = 1234) -> int:
     """src/verifier.rs:10-164: 1 accept, 0 reject, negative at the reference's panic sites."""
     return int(lib().lig_verifier(proof.bits, log_n, proof._h, fs_seed))
+
+
+def verifier_dev(ctx: Context, log_n: int, proof: Proof, fs_seed: int = 1234) -> int:
+    """verifier with the basis inductions on the GPU; same results as verifier()."""
+    return int(lib().lig_verifier_dev(ctx.handle, proof.bits, log_n, proof._h, fs_seed))
 
 
 # ---------------------------------------------------------------- batched --

@@ -142,3 +142,34 @@ def test_prover_2pow24_bit_exact(O, ctx):
     assert lh.verifier(24, p) == 1
     want = O.prove(poly, 24, 32)
     assert got == want
+
+
+@pytest.mark.parametrize("bits,L", [(32, 16), (128, 16), (32, 20)])
+def test_gpu_verifier_matches_host_verifier(O, ctx, bits, L):
+    # verifier-side induce on the GPU: same verdicts as the host verifier on
+    # valid proofs and on tampered ones
+    say("verifier_dev", bits, L)
+    poly = col(O, 3, 1 << L, bits)
+    good = O.prove(poly, L, bits)
+    p = lh.Proof.from_bytes(good, bits)
+    assert lh.verifier_dev(ctx, L, p) == 1 == lh.verifier(L, p)
+    assert lh.verifier_dev(ctx, L, p, fs_seed=7) == lh.verifier(L, p, fs_seed=7)
+    rng = np.random.default_rng(L + bits)
+    blob = bytearray(good)
+    for _ in range(4):
+        pos = int(rng.integers(44, len(blob)))
+        t = bytearray(blob)
+        t[pos] ^= 1 << int(rng.integers(0, 8))
+        try:
+            pt = lh.Proof.from_bytes(bytes(t), bits)
+        except lh.LigeritoError:
+            continue
+        assert lh.verifier_dev(ctx, L, pt) == lh.verifier(L, pt)
+
+
+def test_gpu_verifier_2pow24(O, ctx):
+    say("verifier_dev 2^24")
+    cfg = lh.ProverConfig(ctx, 24, 32)
+    poly = col(O, 2, 1 << 24, 32)
+    p = lh.prover(cfg, poly)
+    assert lh.verifier_dev(ctx, 24, p) == 1
