@@ -196,16 +196,162 @@ size_t Transcript::get_query(size_t n) {
     return (size_t)(lo % (uint64_t)n);  // (u128 as usize) % n on a 64-bit target
 }
 
+namespace {
+__attribute__((target("avx2"))) inline __m256i rotr8(__m256i x, int n) {
+    return _mm256_or_si256(_mm256_srli_epi32(x, n), _mm256_slli_epi32(x, 32 - n));
+}
+
+// Eight independent SHA-256 compressions with AVX2 (lane i: state st[i],
+// block blk[i]); the FIPS 180-4 rounds on 8 x 32-bit lanes.
+__attribute__((target("avx2"))) void sha_x8(const uint32_t st[8][8], const uint8_t blk[8][64], uint32_t out[8][8]) {
+#define rotr rotr8
+    __m256i w[16], v[8], s0[8];
+    for (int t = 0; t < 16; t++) {
+        alignas(32) uint32_t lane[8];
+        for (int i = 0; i < 8; i++) {
+            const uint8_t* q = blk[i] + 4 * t;
+            lane[i] = (uint32_t)q[0] << 24 | (uint32_t)q[1] << 16 | (uint32_t)q[2] << 8 | q[3];
+        }
+        w[t] = _mm256_load_si256((const __m256i*)lane);
+    }
+    for (int k = 0; k < 8; k++) {
+        alignas(32) uint32_t lane[8];
+        for (int i = 0; i < 8; i++) lane[i] = st[i][k];
+        v[k] = s0[k] = _mm256_load_si256((const __m256i*)lane);
+    }
+    for (int t = 0; t < 64; t++) {
+        __m256i wt;
+        if (t < 16) {
+            wt = w[t];
+        } else {
+            const __m256i w15 = w[(t - 15) & 15], w2 = w[(t - 2) & 15];
+            const __m256i sg0 = _mm256_xor_si256(_mm256_xor_si256(rotr(w15, 7), rotr(w15, 18)), _mm256_srli_epi32(w15, 3));
+            const __m256i sg1 = _mm256_xor_si256(_mm256_xor_si256(rotr(w2, 17), rotr(w2, 19)), _mm256_srli_epi32(w2, 10));
+            wt = _mm256_add_epi32(_mm256_add_epi32(w[t & 15], sg0), _mm256_add_epi32(w[(t - 7) & 15], sg1));
+            w[t & 15] = wt;
+        }
+        const __m256i e = v[4], a = v[0];
+        const __m256i S1 = _mm256_xor_si256(_mm256_xor_si256(rotr(e, 6), rotr(e, 11)), rotr(e, 25));
+        const __m256i ch = _mm256_xor_si256(_mm256_and_si256(e, v[5]), _mm256_andnot_si256(e, v[6]));
+        const __m256i t1 = _mm256_add_epi32(_mm256_add_epi32(_mm256_add_epi32(v[7], S1), _mm256_add_epi32(ch, wt)),
+                                            _mm256_set1_epi32((int)kSha[t]));
+        const __m256i S0 = _mm256_xor_si256(_mm256_xor_si256(rotr(a, 2), rotr(a, 13)), rotr(a, 22));
+        const __m256i mj = _mm256_xor_si256(_mm256_xor_si256(_mm256_and_si256(a, v[1]), _mm256_and_si256(a, v[2])),
+                                            _mm256_and_si256(v[1], v[2]));
+        const __m256i t2 = _mm256_add_epi32(S0, mj);
+        v[7] = v[6];
+        v[6] = v[5];
+        v[5] = v[4];
+        v[4] = _mm256_add_epi32(v[3], t1);
+        v[3] = v[2];
+        v[2] = v[1];
+        v[1] = v[0];
+        v[0] = _mm256_add_epi32(t1, t2);
+    }
+    for (int k = 0; k < 8; k++) {
+        alignas(32) uint32_t lane[8];
+        _mm256_store_si256((__m256i*)lane, _mm256_add_epi32(v[k], s0[k]));
+        for (int i = 0; i < 8; i++) out[i][k] = lane[i];
+    }
+#undef rotr
+}
+
+bool have_avx2() {
+    static int cached = -1;
+    if (cached < 0) {
+        unsigned a = 0, b = 0, c = 0, d = 0;
+        cached = (__get_cpuid_count(7, 0, &a, &b, &c, &d) && (b & (1u << 5))) ? 1 : 0;
+        if (getenv("LIG_NO_AVX2")) cached = 0;
+    }
+    return cached == 1;
+}
+}  // namespace
+
+// get_distinct_queries (emulated_fs.rs:61-75).  Without the SHA extensions,
+// squeezes are generated in groups of eight: the running state absorbs each
+// counter in turn, the finalizations that fit one block run eight at a time
+// (AVX2), the others one by one; afterwards the transcript is rewound to
+// exactly the squeezes the rejection sampling consumed.  With the SHA
+// extensions one squeeze at a time is faster.
 std::vector<size_t> Transcript::get_distinct_queries(size_t n, size_t s) {
     LIG_REQUIRE(s <= n, LIG_E_INVALID, "get_distinct_queries: s > n never terminates");
     std::vector<size_t> q;
     q.reserve(s);
     std::vector<uint8_t> seen((n + 7) / 8, 0);
+    if (!have_avx2() || have_sha_ni()) {
+        while (q.size() < s) {
+            size_t v = get_query(n);
+            if (!(seen[v >> 3] & (1u << (v & 7)))) {
+                seen[v >> 3] |= (uint8_t)(1u << (v & 7));
+                q.push_back(v);
+            }
+        }
+        std::sort(q.begin(), q.end());
+        return q;
+    }
     while (q.size() < s) {
-        size_t v = get_query(n);
-        if (!(seen[v >> 3] & (1u << (v & 7)))) {
-            seen[v >> 3] |= (uint8_t)(1u << (v & 7));
-            q.push_back(v);
+        const Sha256 h0 = hasher;
+        const uint32_t c0 = counter;
+        uint32_t st[8][8];
+        alignas(32) uint8_t blk[8][64];
+        uint64_t lo[8];
+        int single[8], n1 = 0;
+        uint32_t out[8][8];
+        for (int k = 0; k < 8; k++) {
+            uint8_t b[4];
+            memcpy(b, &counter, 4);
+            hasher.update(b, 4);
+            counter++;
+            const uint32_t nb = hasher.blen;
+            const uint64_t bits = hasher.total * 8;
+            if (nb < 56) {
+                memcpy(st[n1], hasher.h, 32);
+                memcpy(blk[n1], hasher.buf, nb);
+                blk[n1][nb] = 0x80;
+                memset(blk[n1] + nb + 1, 0, 56 - nb - 1);
+                for (int i = 0; i < 8; i++) blk[n1][56 + i] = (uint8_t)(bits >> (56 - 8 * i));
+                single[n1++] = k;
+            } else {
+                // two-block finalization on the scalar path
+                uint8_t d[32];
+                hasher.digest(d);
+                memcpy(&lo[k], d, 8);
+            }
+        }
+        for (int i = n1; i < 8; i++) {
+            memcpy(st[i], st[0], 32);
+            memcpy(blk[i], blk[0], 64);
+        }
+        if (n1) sha_x8(st, blk, out);
+        for (int i = 0; i < n1; i++) {
+            uint8_t d[8];
+            for (int w = 0; w < 2; w++) {
+                d[4 * w] = (uint8_t)(out[i][w] >> 24);
+                d[4 * w + 1] = (uint8_t)(out[i][w] >> 16);
+                d[4 * w + 2] = (uint8_t)(out[i][w] >> 8);
+                d[4 * w + 3] = (uint8_t)out[i][w];
+            }
+            memcpy(&lo[single[i]], d, 8);
+        }
+        int used = 0;
+        for (int k = 0; k < 8 && q.size() < s; k++) {
+            used = k + 1;
+            const size_t v = (size_t)(lo[k] % (uint64_t)n);
+            if (!(seen[v >> 3] & (1u << (v & 7)))) {
+                seen[v >> 3] |= (uint8_t)(1u << (v & 7));
+                q.push_back(v);
+            }
+        }
+        if (used < 8) {
+            // rewind to the state after exactly `used` squeezes
+            hasher = h0;
+            counter = c0;
+            for (int k = 0; k < used; k++) {
+                uint8_t b[4];
+                memcpy(b, &counter, 4);
+                hasher.update(b, 4);
+                counter++;
+            }
         }
     }
     std::sort(q.begin(), q.end());
