@@ -204,13 +204,13 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
     {
         PhaseTimer pt(ctx, PH_MERKLE);
         launch_leaf_hash<F>(w.mat.p, M, n, w.tree.p, s);
-        launch_merkle_levels(w.tree.p, M, s);
+        uint8_t* pin = ctx->pinned(32);
+        (void)pin;
+        launch_merkle_levels(w.tree.p, M, s, (uint32_t*)ctx->pin_dev);
     }
     LIG_HIP(hipGetLastError());
-    uint8_t* pin = ctx->pinned(32);
-    LIG_HIP(hipMemcpyAsync(pin, w.tree.p + (2 * M - 2) * 8, 32, hipMemcpyDeviceToHost, s));
-    ctx->sync();
-    memcpy(w.root, pin, 32);
+    ctx->sync();  // the last tree kernel wrote the root to pinned offset 0
+    memcpy(w.root, ctx->pin, 32);
 }
 
 // ----------------------------------------------------------------- config --
@@ -247,9 +247,8 @@ namespace {
 // to pinned host memory are in flight on the context stream.
 template <class H> struct PendingOpen {
     Rlp<H>* out = nullptr;
-    PBuf<uint64_t> d_q, d_nodes_idx;
+    PBuf<uint64_t> d_q;
     PBuf<typename H::T> d_rows;
-    PBuf<uint32_t> d_nodes;
     size_t n_nodes = 0;
     uint8_t* h_rows = nullptr;
     uint8_t* h_nodes = nullptr;
@@ -273,16 +272,15 @@ void open_launch(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, Pendi
     po.d_q = PBuf<uint64_t>(&ctx->pool, q.size() + std::max<size_t>(nodes.size(), 1));
     LIG_HIP(hipMemcpyAsync(po.d_q.p, h_idx, idx_bytes, hipMemcpyHostToDevice, s));
     po.d_rows = PBuf<T>(&ctx->pool, q.size() * w.n);
-    po.d_nodes = PBuf<uint32_t>(&ctx->pool, std::max<size_t>(nodes.size(), 1) * 8);
-    {
-        PhaseTimer pt(ctx, PH_OPEN);
-        launch_gather_rows<F>(w.mat.p, w.n, po.d_q.p, q.size(), po.d_rows.p, s);
-        launch_gather_nodes(w.tree.p, po.d_q.p + q.size(), nodes.size(), po.d_nodes.p, s);
-    }
     po.h_rows = pin + pin_off + idx_bytes;
     po.h_nodes = po.h_rows + rows_bytes;
-    LIG_HIP(hipMemcpyAsync(po.h_rows, po.d_rows.p, rows_bytes, hipMemcpyDeviceToHost, s));
-    if (nodes_bytes) LIG_HIP(hipMemcpyAsync(po.h_nodes, po.d_nodes.p, nodes_bytes, hipMemcpyDeviceToHost, s));
+    // the gathers write the proof's rows and nodes straight to pinned memory
+    uint8_t* dev_rows = ctx->pin_dev + pin_off + idx_bytes;
+    {
+        PhaseTimer pt(ctx, PH_OPEN);
+        launch_gather_rows<F>(w.mat.p, w.n, po.d_q.p, q.size(), po.d_rows.p, (T*)dev_rows, s);
+        launch_gather_nodes(w.tree.p, po.d_q.p + q.size(), nodes.size(), (uint32_t*)(dev_rows + rows_bytes), s);
+    }
     out.n_rows = q.size();
     out.n_cols = w.n;
 }
@@ -356,11 +354,11 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     Transcript fs(fs_seed);
 
     proof = Proof<H>();
-    // Pinned staging regions (disjoint, so an in-flight copy never reads a
+    // Pinned staging regions (disjoint, so nothing in flight touches a
     // region the host is refilling): [0,256) commit roots, [256,16K) uploads
     // of induce constants and fold weights, [16K,32K) device transcript
-    // hand-over and logs, [32K,2M) openings, [2M,4M) yr.
-    constexpr size_t PIN_UP = 256, PIN_FSUP = 16384, PIN_FSDN = 16384 + 256, PIN_OPEN = 32768, PIN_YR = (size_t)2 << 20;
+    // logs, [32K,2M) openings, [2M,4M) yr.
+    constexpr size_t PIN_UP = 256, PIN_FSDN = 16384, PIN_OPEN = 32768, PIN_YR = (size_t)2 << 20;
     ctx->pinned((size_t)4 << 20);
     Witness<H> w0;
     commit<H>(*cfg.initial_rs, d_poly, cfg.initial_m, cfg.initial_n, w0, PH_ENCODE0);
@@ -420,21 +418,28 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     struct Section {
         int n0 = 0, k = 0;  // squeezes of the leading step, rounds
     } sec;
-    DevTranscript* h_fsup = (DevTranscript*)(ctx->pin + PIN_FSUP);
+    // Section logs go straight to pinned memory: triples, then squeezed
+    // challenges, then the check flag.
+    const size_t n_tlog = 3 * (size_t)(maxk + 1), n_r = (size_t)maxk + 2;
     uint8_t* h_fsdn = ctx->pin + PIN_FSDN;
+    T* dev_tlog = (T*)(ctx->pin_dev + PIN_FSDN);
+    T* dev_rlog = dev_tlog + n_tlog;
+    uint32_t* dev_err = (uint32_t*)(dev_rlog + n_r);
     // Launches: leading step (absorb the triple in res_cur, check it against
     // the enforced sum, squeeze n0), optional glue with the first squeeze,
     // then k fold rounds, each followed by its transcript step.
     auto launch_section = [&](int n0, int k, PBuf<T>* d_bnew) {
-        export_transcript(fs, *h_fsup);
-        LIG_HIP(hipMemcpyAsync(d_fs.p, h_fsup, sizeof(DevTranscript), hipMemcpyHostToDevice, s));
+        DevTranscript init;
+        export_transcript(fs, init);
+        *(uint32_t*)(h_fsdn + (n_tlog + n_r) * sizeof(T)) = 0;
         PhaseTimer pt(ctx, PH_SUMCHECK);
-        launch_fs_step<F>(d_fs.p, res_cur, d_enf, n_enf, d_tlog.p, d_r.p, n0, s);
+        launch_fs_step<F>(d_fs.p, &init, res_cur, d_enf, n_enf, dev_tlog, d_r.p, dev_rlog, n0, dev_err, s);
         if (d_bnew) launch_glue<F>(d_b.p, d_bnew->p, flen, d_r.p, s);
         for (int j = 0; j < k; j++) {
             launch_sumcheck_fold<F>(d_f.p, d_b.p, flen, d_r.p + (n0 - 1) + j, d_f2.p, d_b2.p, res_next, res_cur, s);
             std::swap(res_cur, res_next);
-            launch_fs_step<F>(d_fs.p, res_cur, nullptr, 0, d_tlog.p + 3 * (j + 1), d_r.p + n0 + j, j + 1 < k ? 1 : 0, s);
+            launch_fs_step<F>(d_fs.p, nullptr, res_cur, nullptr, 0, dev_tlog + 3 * (j + 1), d_r.p + n0 + j,
+                              dev_rlog + n0 + j, j + 1 < k ? 1 : 0, dev_err, s);
             std::swap(d_f, d_f2);
             std::swap(d_b, d_b2);
             flen /= 2;
@@ -443,19 +448,12 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         sec.n0 = n0;
         sec.k = k;
     };
-    const size_t n_tlog = 3 * (size_t)(maxk + 1), n_r = (size_t)maxk + 2;
-    auto download_section = [&]() {
-        LIG_HIP(hipMemcpyAsync(h_fsdn, d_tlog.p, n_tlog * sizeof(T), hipMemcpyDeviceToHost, s));
-        LIG_HIP(hipMemcpyAsync(h_fsdn + n_tlog * sizeof(T), d_r.p, n_r * sizeof(T), hipMemcpyDeviceToHost, s));
-        LIG_HIP(hipMemcpyAsync(h_fsdn + (n_tlog + n_r) * sizeof(T), d_fs.p, sizeof(DevTranscript), hipMemcpyDeviceToHost, s));
-    };
     Vec<H> fold_rs;  // challenges of the section's fold rounds
     // after the stream has been synchronized
     auto replay_section = [&](const char* what) {
         const T* tl = (const T*)h_fsdn;
         const T* rr = tl + n_tlog;
-        const DevTranscript* dts = (const DevTranscript*)(rr + n_r);
-        LIG_REQUIRE(dts->err == 0, LIG_E_ASSERT, what);
+        LIG_REQUIRE(*(const uint32_t*)(rr + n_r) == 0, LIG_E_ASSERT, what);
         int ri = 0;
         fold_rs.clear();
         auto squeeze_check = [&]() {
@@ -492,7 +490,6 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         if (i == cfg.recursive_steps - 1) {
             // final step (prover.rs:104-142): yr, queries, openings
             T* hy = (T*)(ctx->pin + PIN_YR);
-            download_section();
             LIG_HIP(hipMemcpyAsync(hy, d_f.p, flen * sizeof(T), hipMemcpyDeviceToHost, s));
             ctx->sync();
             replay_section("sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
@@ -507,7 +504,6 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             return;
         }
         Witness<H> wnext;
-        download_section();
         commit<H>(*cfg.rs[i + 1], d_f.p, cfg.dims_m[i + 1], cfg.dims_n[i + 1], wnext);  // synchronizes
         replay_section("sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
         open_finish<H>(*pending);

@@ -42,17 +42,20 @@ struct Ctx {
     std::vector<Mark> marks;
     double phase_ms[PH_COUNT] = {};
     uint64_t phase_n[PH_COUNT] = {};
-    // Pinned host staging for small transfers (grown on demand; contents
-    // are only valid until the next call that uses it).
+    // Pinned, device-mapped, coherent host staging (grown on demand;
+    // contents are only valid until the next call that uses it).  Kernels
+    // write results the host needs straight into it (through pin_dev).
     uint8_t* pin = nullptr;
+    uint8_t* pin_dev = nullptr;
     size_t pin_cap = 0;
     uint8_t* pinned(size_t bytes) {
         if (bytes > pin_cap) {
             sync();
             if (pin) (void)hipHostFree(pin);
-            pin = nullptr;
+            pin = pin_dev = nullptr;
             size_t cap = std::max<size_t>(bytes, (size_t)1 << 20);
-            LIG_HIP(hipHostMalloc((void**)&pin, cap, hipHostMallocDefault));
+            LIG_HIP(hipHostMalloc((void**)&pin, cap, hipHostMallocMapped | hipHostMallocCoherent));
+            LIG_HIP(hipHostGetDevicePointer((void**)&pin_dev, pin, 0));
             pin_cap = cap;
         }
         return pin;

@@ -695,7 +695,8 @@ template <class F> void launch_leaf_hash(const typename F::T* mat, size_t rows, 
 
 // ----------------------------------------------------------- Merkle levels --
 // A workgroup turns 2^g nodes of layer `lin` into the g layers above them.
-__global__ __launch_bounds__(256) void k_merkle_subtree(uint32_t* __restrict__ tree, size_t in_off, size_t n_in, int g) {
+__global__ __launch_bounds__(256) void k_merkle_subtree(uint32_t* __restrict__ tree, size_t in_off, size_t n_in, int g,
+                                                        uint32_t* __restrict__ root_out) {
     __shared__ uint32_t buf[2][256][8];
     const size_t base = blockIdx.x * ((size_t)1 << g);  // first input node of this group
     size_t lay_off = in_off, lay_n = n_in;
@@ -725,6 +726,8 @@ __global__ __launch_bounds__(256) void k_merkle_subtree(uint32_t* __restrict__ t
                 dst[w] = out[w];
                 buf[cur ^ 1][i][w] = out[w];
             }
+            if (root_out && cnt == 1 && lay_n == 2)
+                for (int w = 0; w < 8; w++) root_out[w] = out[w];
         }
         __syncthreads();
         cur ^= 1;
@@ -758,7 +761,7 @@ __global__ __launch_bounds__(256) void k_merkle_level(uint32_t* __restrict__ tre
 // than MERKLE_WIDE parents, workgroups of k_merkle_subtree finish the top.
 constexpr size_t MERKLE_WIDE = (size_t)1 << 15;
 
-void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s) {
+void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out) {
     size_t in_off = 0, nodes = n_leaves;
     while (nodes / 2 >= MERKLE_WIDE) {
         const size_t n_out = nodes / 2;
@@ -771,7 +774,7 @@ void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s) {
         while (((size_t)1 << lg) < nodes) lg++;
         int g = lg < 9 ? lg : 9;
         size_t groups = nodes >> g;
-        hipLaunchKernelGGL(k_merkle_subtree, dim3((unsigned)groups), dim3(256), 0, s, tree, in_off, nodes, g);
+        hipLaunchKernelGGL(k_merkle_subtree, dim3((unsigned)groups), dim3(256), 0, s, tree, in_off, nodes, g, root_out);
         // advance to the top layer produced
         for (int l = 0; l < g; l++) {
             in_off += nodes;
@@ -832,17 +835,20 @@ void launch_fold(const typename F::T* poly, size_t m, int kk, const typename F::
 
 template <class F>
 __global__ void k_gather_rows(const typename F::T* __restrict__ mat, size_t n, const uint64_t* __restrict__ rows, size_t count,
-                              typename F::T* __restrict__ out) {
+                              typename F::T* __restrict__ out, typename F::T* __restrict__ out2) {
     size_t total = count * n;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
-        out[i] = mat[rows[i / n] * n + (i % n)];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const typename F::T v = mat[rows[i / n] * n + (i % n)];
+        out[i] = v;
+        if (out2) out2[i] = v;
+    }
 }
 
 template <class F>
 void launch_gather_rows(const typename F::T* mat, size_t n, const uint64_t* rows, size_t count, typename F::T* out,
-                        hipStream_t s) {
+                        typename F::T* out2, hipStream_t s) {
     if (!count) return;
-    hipLaunchKernelGGL(k_gather_rows<F>, dim3(grid_for(count * n, 256)), dim3(256), 0, s, mat, n, rows, count, out);
+    hipLaunchKernelGGL(k_gather_rows<F>, dim3(grid_for(count * n, 256)), dim3(256), 0, s, mat, n, rows, count, out, out2);
 }
 
 // ---------------------------------------------------------------- induce ---
@@ -1257,14 +1263,16 @@ void launch_glue(typename F::T* bsum, const typename F::T* bnew, size_t len, con
 // One wave runs the transcript step; every lane computes the same values
 // and lane 0 writes them.
 template <class F>
-__global__ __launch_bounds__(64) void k_fs_step(DevTranscript* __restrict__ st, const typename F::Acc* __restrict__ res,
+__global__ __launch_bounds__(64) void k_fs_step(DevTranscript* __restrict__ st, DevTranscript init, int use_init,
+                                                const typename F::Acc* __restrict__ res,
                                                 const typename F::Acc* __restrict__ enf, int n_enf,
                                                 typename F::T* __restrict__ tlog, typename F::T* __restrict__ rout,
-                                                int n_squeeze) {
+                                                typename F::T* __restrict__ rout2, int n_squeeze,
+                                                uint32_t* __restrict__ err_out) {
     using T = typename F::T;
     using A = typename F::Acc;
     dfs::State s;
-    s.load(st);
+    s.load(use_init ? &init : st);
     uint32_t err = 0;
     if (res) {
         T t[3];
@@ -1285,18 +1293,25 @@ __global__ __launch_bounds__(64) void k_fs_step(DevTranscript* __restrict__ st, 
     for (int i = 0; i < n_squeeze; i++) {
         uint32_t o[4];
         s.squeeze(o);
-        if (threadIdx.x == 0) rout[i] = Ops<F>::from_squeeze(o);
+        if (threadIdx.x == 0) {
+            rout[i] = Ops<F>::from_squeeze(o);
+            if (rout2) rout2[i] = Ops<F>::from_squeeze(o);
+        }
     }
     if (threadIdx.x == 0) {
         s.store(st);
-        if (err) st->err = 1;
+        if (err) *err_out = 1;
     }
 }
 
 template <class F>
-void launch_fs_step(DevTranscript* st, const typename F::Acc* res, const typename F::Acc* enf, int n_enf,
-                    typename F::T* tlog, typename F::T* rout, int n_squeeze, hipStream_t s) {
-    hipLaunchKernelGGL(k_fs_step<F>, dim3(1), dim3(64), 0, s, st, res, enf, n_enf, tlog, rout, n_squeeze);
+void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename F::Acc* res, const typename F::Acc* enf,
+                    int n_enf, typename F::T* tlog, typename F::T* rout, typename F::T* rout2, int n_squeeze,
+                    uint32_t* err_out, hipStream_t s) {
+    DevTranscript iv{};
+    if (init) iv = *init;
+    hipLaunchKernelGGL(k_fs_step<F>, dim3(1), dim3(64), 0, s, st, iv, init ? 1 : 0, res, enf, n_enf, tlog, rout, rout2,
+                       n_squeeze, err_out);
 }
 
 // ------------------------------------------------------- instantiations ---
@@ -1309,13 +1324,14 @@ void launch_fs_step(DevTranscript* st, const typename F::Acc* res, const typenam
                                          PassObserver*);                     \
     template void launch_leaf_hash<F>(const F::T*, size_t, size_t, uint32_t*, hipStream_t);                                 \
     template void launch_fold<F>(const F::T*, size_t, int, const F::T*, F::T*, hipStream_t);                                \
-    template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, hipStream_t);                  \
+    template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, F::T*, hipStream_t);                  \
     template void launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t);                   \
     template void launch_eval013<F>(const F::T*, const F::T*, size_t, F::Acc*, F::Acc*, hipStream_t);                       \
     template void launch_zero_acc<F>(F::Acc*, hipStream_t);                                                                 \
     template void launch_sumcheck_fold<F>(const F::T*, const F::T*, size_t, const F::T*, F::T*, F::T*, F::Acc*, F::Acc*,  \
                                           hipStream_t);                                                                     \
-    template void launch_fs_step<F>(DevTranscript*, const F::Acc*, const F::Acc*, int, F::T*, F::T*, int, hipStream_t);     \
+    template void launch_fs_step<F>(DevTranscript*, const DevTranscript*, const F::Acc*, const F::Acc*, int, F::T*, F::T*,  \
+                                    F::T*, int, uint32_t*, hipStream_t);                                                    \
     template void launch_glue<F>(F::T*, const F::T*, size_t, const F::T*, hipStream_t);                                           \
     template void launch_induce_prep<F>(const uint64_t*, const F::T*, const F::T*, int, int, F::T*, F::T*, hipStream_t);   \
     template void launch_enforced<F>(const F::T*, int, int, const F::T*, const F::T*, F::Acc*, hipStream_t);

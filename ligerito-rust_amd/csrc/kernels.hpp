@@ -81,8 +81,10 @@ void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m,
 // layer0[row*8 .. +8] = BLAKE3(row bytes) for every row of the [rows][n] matrix.
 template <class F> void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, hipStream_t s);
 // Builds layers 1.. in place after layer 0 (contiguous node layout,
-// merkle_tree.rs:22-60).  n_leaves must be a power of two.
-void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s);
+// merkle_tree.rs:22-60).  n_leaves must be a power of two, at least 2.
+// If root_out is given, the root's 8 words are also written there (for
+// instance to mapped host memory).
+void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out = nullptr);
 // out[i] = node at byte offset offs[i] of the tree (32 bytes each).
 void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t count, uint32_t* out, hipStream_t s);
 
@@ -91,9 +93,10 @@ void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t 
 // kk variables, multilinear_poly.rs:66-87).  gr_tabs: 2^kk tables.
 template <class F>
 void launch_fold(const typename F::T* poly, size_t m, int kk, const typename F::T* gr_tabs, typename F::T* out, hipStream_t s);
+// out (and out2, if given) receive the count rows listed in rows.
 template <class F>
 void launch_gather_rows(const typename F::T* mat, size_t n, const uint64_t* rows, size_t count, typename F::T* out,
-                        hipStream_t s);
+                        typename F::T* out2, hipStream_t s);
 
 // ------------------------------------------------------------ induce ------
 // The basis index splits into clo low bits (P tables, nq << clo elements)
@@ -132,15 +135,18 @@ template <class F>
 void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t len, const typename F::T* r,
                           typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
                           hipStream_t s);
-// One transcript step on the device: if res is given, reduce the triple
+// One transcript step on the device, on the state at st (or, if init is
+// given, on *init, passed by value): if res is given, reduce the triple
 // (three accumulators) and absorb it (QuadraticEvals::to_bytes,
-// sumcheck.rs:29-35), logging it to tlog and, if enf is given, flagging
-// st->err when t0 + t1 differs from the enforced sum, the XOR of the n_enf
-// parts at enf (ligerito.rs:35,52); then squeeze n_squeeze field elements
-// into rout (get_field).
+// sumcheck.rs:29-35), logging it to tlog and, if enf is given, setting
+// *err_out to 1 when t0 + t1 differs from the enforced sum, the XOR of the
+// n_enf parts at enf (ligerito.rs:35,52); then squeeze n_squeeze field
+// elements into rout (and rout2, if given) (get_field).  The new state is
+// stored at st.
 template <class F>
-void launch_fs_step(DevTranscript* st, const typename F::Acc* res, const typename F::Acc* enf, int n_enf,
-                    typename F::T* tlog, typename F::T* rout, int n_squeeze, hipStream_t s);
+void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename F::Acc* res, const typename F::Acc* enf,
+                    int n_enf, typename F::T* tlog, typename F::T* rout, typename F::T* rout2, int n_squeeze,
+                    uint32_t* err_out, hipStream_t s);
 // bsum[i] ^= beta * bnew[i]   (glue, ligerito.rs:59-67)
 template <class F>
 void launch_glue(typename F::T* bsum, const typename F::T* bnew, size_t len, const typename F::T* beta, hipStream_t s);
