@@ -40,6 +40,7 @@ def main():
     ap.add_argument("--log-n", type=int, default=LOG_N)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-prove measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-commit", action="store_true", help="skip the isolated encode+commit measurement")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrival rate, requests/s")
@@ -137,6 +138,7 @@ def main():
         with open(tf) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
+    commit_c = None if args.no_commit else _commit_c(lh, torch, ctx)
     batched = None if args.no_batch else _batched(lh, torch, np, cfg, ptr, args.batch_jobs)
     served = None if args.no_serve else _served(lh, torch, np, args.serve_requests, args.serve_rate)
 
@@ -182,6 +184,7 @@ def main():
         "phases_ms_per_prove": {k: v[0] / n_prof for k, v in phases.items()},
         "host_input_coeffs_per_s": N / t_host,
         "verify_ms": {"host": t_v * 1e3, "gpu_induce": t_vd * 1e3},
+        "commit_c": commit_c,
         "batched": batched,
         "served": served,
     }
@@ -193,6 +196,49 @@ def _gpu_poly(torch, L, seed):
     g = torch.Generator(device="cuda:0")
     g.manual_seed(seed)
     return torch.randint(-(1 << 31), 1 << 31, (1 << L,), dtype=torch.int32, device="cuda:0", generator=g)
+
+
+def _commit_c(lh, torch, ctx, reps=5):
+    """Isolated encode+commit (ligero_commit): 4096 rows of 2^12 GF(2^32)
+    elements (and 1024 of GF(2^128)) encoded at rate 1/4 to 2^14, one
+    16 KiB leaf per encoded row, full Merkle tree.  Inputs in HBM."""
+    out = {}
+    for bits, m, n in ((32, 1 << 12, 4096), (128, 1 << 12, 1024)):
+        words = m * n * (bits // 32)
+        g = torch.Generator(device="cuda:0")
+        g.manual_seed(77 + bits)
+        d = torch.randint(-(1 << 31), 1 << 31, (words,), dtype=torch.int32, device="cuda:0", generator=g)
+        torch.cuda.synchronize()
+        rs = lh.ReedSolomonEncoding(ctx, m, 4 * m, bits)
+        lh.ligero_commit_dev(d.data_ptr(), m, n, rs)
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            lh.ligero_commit_dev(d.data_ptr(), m, n, rs)
+        dt = (time.perf_counter() - t0) / reps
+        enc_ms, _ = ctx.profile_read("encode")
+        mk_ms, _ = ctx.profile_read("merkle")
+        ctx.profile(False)
+        enc_s = enc_ms / reps / 1e3
+        mk_s = mk_ms / reps / 1e3
+        fbytes = bits // 8
+        alg = 5 * m * n * fbytes
+        leaf_bytes = n * fbytes
+        chunks = (leaf_bytes + 1023) // 1024
+        comps = 4 * m * (leaf_bytes // 64 + (chunks - 1)) + (4 * m - 1)
+        out[f"gf{bits}"] = {
+            "shape": f"{n} messages x 2^12 -> 2^14, GF(2^{bits})",
+            "commit_ms": dt * 1e3,
+            "encode_ms": enc_s * 1e3,
+            "encode_GBps": alg / enc_s / 1e9,
+            "encode_frac_of_hbm": alg / enc_s / 1e9 / HBM_PEAK_GBS,
+            "merkle_ms": mk_s * 1e3,
+            "merkle_compressions": comps,
+            "merkle_Tops": comps * 792 / mk_s / 1e12,
+            "merkle_frac_of_valu": comps * 792 / mk_s / 1e12 / 39.3,
+        }
+        log(f"commit C gf{bits}: {dt * 1e3:.3f} ms (encode {enc_s * 1e3:.3f} ms, merkle {mk_s * 1e3:.3f} ms)")
+    return out
 
 
 def _batched(lh, torch, np, cfg24, ptr24, n_jobs, streams=4):

@@ -247,7 +247,8 @@ int lig_build_merkle_tree(lig_ctx* ctx, const void* leaves, size_t n_leaves, siz
         // a leaf is a row of leaf_len/4 32-bit words
         PBuf<uint32_t> d = upload<uint32_t>(c, leaves, n_leaves * leaf_len / 4);
         PBuf<uint32_t> tree(&c.pool, (2 * n_leaves - 1) * 8);
-        launch_leaf_hash<F32>(d.p, n_leaves, leaf_len / 4, tree.p, c.stream);
+        PBuf<uint32_t> scratch(&c.pool, leaf_hash_scratch_words(n_leaves, leaf_len));
+        launch_leaf_hash<F32>(d.p, n_leaves, leaf_len / 4, tree.p, scratch.n ? scratch.p : nullptr, c.stream);
         launch_merkle_levels(tree.p, n_leaves, c.stream);
         LIG_HIP(hipGetLastError());
         LIG_HIP(hipMemcpyAsync(layers_out, tree.p, (2 * n_leaves - 1) * 32, hipMemcpyDeviceToHost, c.stream));

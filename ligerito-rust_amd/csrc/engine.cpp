@@ -203,7 +203,8 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
     w.tree = PBuf<uint32_t>(&ctx->pool, (2 * M - 1) * 8);
     {
         PhaseTimer pt(ctx, PH_MERKLE);
-        launch_leaf_hash<F>(w.mat.p, M, n, w.tree.p, s);
+        PBuf<uint32_t> scratch(&ctx->pool, leaf_hash_scratch_words(M, n * sizeof(T)));
+        launch_leaf_hash<F>(w.mat.p, M, n, w.tree.p, scratch.n ? scratch.p : nullptr, s);
         uint8_t* pin = ctx->pinned(32);
         (void)pin;
         launch_merkle_levels(w.tree.p, M, s, (uint32_t*)ctx->pin_dev);

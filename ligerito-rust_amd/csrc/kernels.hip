@@ -681,10 +681,87 @@ __global__ __launch_bounds__(128) void k_leaf_hash_blocks(const uint32_t* __rest
     }
 }
 
-template <class F> void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, hipStream_t s) {
+// Leaves of nch whole 1 KiB chunks (nch a power of two, 2..256): one
+// thread per chunk chains its 16 block compressions (chunk counter = chunk
+// index) and writes the chunk's chaining value ...
+__global__ __launch_bounds__(256) void k_leaf_chunks(const uint32_t* __restrict__ mat, size_t rows, int lgnch,
+                                                     uint32_t* __restrict__ cvs) {
+    const size_t total = rows << lgnch;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t ch = (uint32_t)(t & ((1u << lgnch) - 1));
+        const uint4* p = reinterpret_cast<const uint4*>(mat + t * 256);
+        uint32_t cv[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) cv[i] = b3::iv(i);
+        uint4 nx[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) nx[i] = p[i];
+        for (int b = 0; b < 16; b++) {
+            uint32_t m[16];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                m[4 * i] = nx[i].x;
+                m[4 * i + 1] = nx[i].y;
+                m[4 * i + 2] = nx[i].z;
+                m[4 * i + 3] = nx[i].w;
+            }
+            if (b < 15) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) nx[i] = p[4 * (b + 1) + i];
+            }
+            const uint32_t fl = (b == 0 ? b3::CHUNK_START : 0u) | (b == 15 ? b3::CHUNK_END : 0u);
+            b3::compress_cv(cv, m, ch, 64, fl, cv);
+        }
+        uint4* o = reinterpret_cast<uint4*>(cvs + t * 8);
+        o[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
+        o[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+    }
+}
+
+// ... then one thread per leaf combines its chunk values with PARENT nodes
+// (a perfect binary tree, ROOT on the top), in place in cvs.
+__global__ __launch_bounds__(128) void k_leaf_chunk_tree(uint32_t* __restrict__ cvs, size_t rows, int lgnch,
+                                                         uint32_t* __restrict__ layer0) {
+    for (size_t row = blockIdx.x * (size_t)blockDim.x + threadIdx.x; row < rows; row += (size_t)gridDim.x * blockDim.x) {
+        uint32_t* c = cvs + (row << lgnch) * 8;
+        uint32_t out[8];
+        for (int cnt = 1 << (lgnch - 1); cnt >= 1; cnt >>= 1) {
+            for (int i = 0; i < cnt; i++) {
+                uint32_t m[16];
+#pragma unroll
+                for (int w = 0; w < 16; w++) m[w] = c[16 * i + w];
+                uint32_t k[8];
+#pragma unroll
+                for (int w = 0; w < 8; w++) k[w] = b3::iv(w);
+                b3::compress_cv(k, m, 0, 64, b3::PARENT | (cnt == 1 ? b3::ROOT : 0u), out);
+#pragma unroll
+                for (int w = 0; w < 8; w++) c[8 * i + w] = out[w];
+            }
+        }
+        uint4* o = reinterpret_cast<uint4*>(layer0 + row * 8);
+        o[0] = make_uint4(out[0], out[1], out[2], out[3]);
+        o[1] = make_uint4(out[4], out[5], out[6], out[7]);
+    }
+}
+
+size_t leaf_hash_scratch_words(size_t rows, size_t row_bytes) {
+    const size_t nch = row_bytes / 1024;
+    if (row_bytes % 1024 != 0 || nch < 2 || nch > 256 || (nch & (nch - 1)) != 0) return 0;
+    return rows * nch * 8;
+}
+
+template <class F>
+void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, uint32_t* scratch, hipStream_t s) {
     const size_t words = n * (sizeof(typename F::T) / 4);
     const dim3 grid(grid_for(rows, 128)), block(128);
     const uint32_t* m = reinterpret_cast<const uint32_t*>(mat);
+    if (scratch && leaf_hash_scratch_words(rows, words * 4)) {
+        int lgnch = 0;
+        while (((size_t)1024 << lgnch) < words * 4) lgnch++;
+        hipLaunchKernelGGL(k_leaf_chunks, dim3(grid_for(rows << lgnch, 256)), dim3(256), 0, s, m, rows, lgnch, scratch);
+        hipLaunchKernelGGL(k_leaf_chunk_tree, grid, block, 0, s, scratch, rows, lgnch, layer0);
+        return;
+    }
     switch (words) {
         case 16: hipLaunchKernelGGL(k_leaf_hash_blocks<1>, grid, block, 0, s, m, rows, layer0); return;
         case 32: hipLaunchKernelGGL(k_leaf_hash_blocks<2>, grid, block, 0, s, m, rows, layer0); return;
@@ -1322,7 +1399,7 @@ void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename
     template void launch_fft_stage<F>(F::T*, size_t, size_t, int, const F::T*, int, hipStream_t);                           \
     template void launch_fft_quarters<F>(const F::T*, F::T*, size_t, size_t, const F::T*, const F::T*, hipStream_t,         \
                                          PassObserver*);                     \
-    template void launch_leaf_hash<F>(const F::T*, size_t, size_t, uint32_t*, hipStream_t);                                 \
+    template void launch_leaf_hash<F>(const F::T*, size_t, size_t, uint32_t*, uint32_t*, hipStream_t);                                 \
     template void launch_fold<F>(const F::T*, size_t, int, const F::T*, F::T*, hipStream_t);                                \
     template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, F::T*, hipStream_t);                  \
     template void launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t);                   \

@@ -79,7 +79,12 @@ void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m,
 
 // -------------------------------------------------------------- Merkle ----
 // layer0[row*8 .. +8] = BLAKE3(row bytes) for every row of the [rows][n] matrix.
-template <class F> void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, hipStream_t s);
+// Leaves of whole 1 KiB chunks (a power of two of them, 2..256) hash chunk-
+// parallel when scratch is given: leaf_hash_scratch_words(rows, row bytes)
+// words (0 when the leaf shape does not qualify).
+size_t leaf_hash_scratch_words(size_t rows, size_t row_bytes);
+template <class F>
+void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, uint32_t* scratch, hipStream_t s);
 // Builds layers 1.. in place after layer 0 (contiguous node layout,
 // merkle_tree.rs:22-60).  n_leaves must be a power of two, at least 2.
 // If root_out is given, the root's 8 words are also written there (for
