@@ -42,6 +42,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-commit", action="store_true", help="skip the isolated encode+commit measurement")
     ap.add_argument("--batch-jobs", type=int, default=256)
+    ap.add_argument("--batch-streams", type=int, default=12)
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -139,7 +140,7 @@ def main():
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
     commit_c = None if args.no_commit else _commit_c(lh, torch, ctx)
-    batched = None if args.no_batch else _batched(lh, torch, np, cfg, ptr, args.batch_jobs)
+    batched = None if args.no_batch else _batched(lh, torch, np, cfg, ptr, args.batch_jobs, args.batch_streams)
     served = None if args.no_serve else _served(lh, torch, np, args.serve_requests, args.serve_rate)
 
     cpu = None
@@ -241,7 +242,7 @@ def _commit_c(lh, torch, ctx, reps=5):
     return out
 
 
-def _batched(lh, torch, np, cfg24, ptr24, n_jobs, streams=4):
+def _batched(lh, torch, np, cfg24, ptr24, n_jobs, streams=12):
     """Batched proves beside the plain path: (a) n_jobs polynomials with log2
     sizes uniform in 16..22 (seed 3), each with its own transcript seed;
     (b) eight 2^24 proves.  Inputs are in HBM; prover configurations are
@@ -298,7 +299,7 @@ def _batched(lh, torch, np, cfg24, ptr24, n_jobs, streams=4):
     }
 
 
-def _served(lh, torch, np, n_req, rate, slots=4, budget=64 << 30):
+def _served(lh, torch, np, n_req, rate, slots=8, budget=64 << 30):
     """Open-loop Poisson arrivals of prove requests (log2 sizes uniform in
     16..24, seed 4) against the prove server; inputs in pinned host memory
     (the upload is part of each request's latency)."""
