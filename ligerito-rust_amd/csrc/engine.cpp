@@ -72,9 +72,9 @@ template <class H> void RsPlan<H>::init(Ctx* c, size_t msg_len, size_t block_len
     const size_t M = block_len, m = msg_len;
     d_tw.alloc(M - 1);
     LIG_HIP(hipMemcpyAsync(d_tw.p, tw.data(), (M - 1) * sizeof(T), hipMemcpyHostToDevice, s));
-    // per-twiddle multiply tables: only the generic passes read them (the
-    // GF(2^32) passes build theirs from d_tw in LDS)
-    if (F::BITS != 32 || m == 1) {
+    // per-twiddle multiply tables: only the generic passes (m == 1) read
+    // them; the register passes build theirs from d_tw in LDS
+    if (m == 1) {
         d_tw_tabs.alloc((M - 1) * F::TAB);
         launch_build_tables<F>(d_tw.p, M - 1, d_tw_tabs.p, s);
     }

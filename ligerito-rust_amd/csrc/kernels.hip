@@ -268,6 +268,7 @@ __global__ __launch_bounds__(256) void k_fft_pass(const typename F::T* __restric
 
 static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw, hipStream_t s,
                            PassObserver* obs);
+static void fft_quarters128(const u128v* msg, u128v* mat, size_t m, size_t n, const u128v* tw, hipStream_t s);
 
 template <class F>
 void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw,
@@ -279,6 +280,11 @@ void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m,
     if constexpr (F::BITS == 32) {
         if (ell > 0) {
             fft_quarters32(msg, mat, m, n, tw, s, obs);
+            return;
+        }
+    } else {
+        if (ell > 0) {
+            fft_quarters128(msg, mat, m, n, tw, s);
             return;
         }
     }
@@ -293,6 +299,149 @@ void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m,
         int shared = (p == 0 && msg) ? 1 : 0;
         hipLaunchKernelGGL((k_fft_pass<F, K, C>), grid, dim3(256), 0, s, src, mat, m, n, ell, d0, k, shared, tw_tabs);
         d0 += k;
+    }
+}
+
+// ------------------------------------------------ GF(2^128) register pass --
+// Multiply by the table at t (32 nibble positions x 16 entries, position k
+// of w at entries 16k..16k+15; the lo word holds positions 0..15).
+__device__ __forceinline__ u128v mul_tab128_lds(const uint4* __restrict__ t, u128v w) {
+    uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint4 a = t[16 * k + (int)((w.lo >> (4 * k)) & 15u)];
+        const uint4 b = t[256 + 16 * k + (int)((w.hi >> (4 * k)) & 15u)];
+        acc.x ^= a.x ^ b.x;
+        acc.y ^= a.y ^ b.y;
+        acc.z ^= a.z ^ b.z;
+        acc.w ^= a.w ^ b.w;
+    }
+    return {(uint64_t)acc.x | ((uint64_t)acc.y << 32), (uint64_t)acc.z | ((uint64_t)acc.w << 32)};
+}
+
+__device__ __forceinline__ uint4 u128_to4(u128v v) {
+    return make_uint4((uint32_t)v.lo, (uint32_t)(v.lo >> 32), (uint32_t)v.hi, (uint32_t)(v.hi >> 32));
+}
+
+// One launch = K stages (d0..d0+K-1) of every quarter's size-m FFT.  Each
+// thread owns one column of a 2^K-row tile (rows S apart) and runs the K
+// stages in registers; a workgroup of 64*J threads covers J consecutive jobs
+// (column chunks / tiles) that share B, hence the 2^K - 1 twiddle tables it
+// builds in LDS from the plain twiddles (entry v of position k = lambda *
+// v * x^(4k), from c_b = lambda * x^(4k+b)).
+template <int K>
+__global__ __launch_bounds__(256) void k_fft_pass128(const u128v* __restrict__ src, u128v* __restrict__ dst, size_t m,
+                                                     size_t n, int ell, int d0, int q_arg, int src_shared,
+                                                     const u128v* __restrict__ tw) {
+    constexpr int R = 1 << K, NT = R - 1;
+    __shared__ __attribute__((aligned(16))) uint4 tabs[NT * 512];
+    const size_t q = q_arg >= 0 ? (size_t)q_arg : (size_t)blockIdx.y;
+    const int cc = n < 64 ? (int)n : 64;
+    const size_t nchunks = (n + 63) >> 6;  // a power of two
+    const int lgnc = 63 - __clzll((unsigned long long)nchunks);
+    const int sh = ell - d0 - K;
+    const size_t S = (size_t)1 << sh;
+    const int J = (int)(blockDim.x >> 6), sub = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const size_t job0 = (size_t)blockIdx.x * J;
+    const size_t B = (job0 >> lgnc) >> sh;
+    for (int idx = threadIdx.x; idx < NT * 32; idx += blockDim.x) {
+        const int tt = idx >> 5, k = idx & 31;
+        const int j = 31 - __clz(tt + 1);
+        const size_t tp = (size_t)(tt + 1) - ((size_t)1 << j);
+        const int d = d0 + j;
+        const size_t first = ((size_t)1 << (d + 2)) - 1;
+        const u128v lam = tw[first + (q << d) + (B << j) + tp];
+        // lam * x^(4k): shift into four words, reduce
+        const int s4 = 4 * k;
+        uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+        if (s4 == 0) {
+            w0 = lam.lo;
+            w1 = lam.hi;
+        } else if (s4 < 64) {
+            w0 = lam.lo << s4;
+            w1 = (lam.hi << s4) | (lam.lo >> (64 - s4));
+            w2 = lam.hi >> (64 - s4);
+        } else if (s4 == 64) {
+            w1 = lam.lo;
+            w2 = lam.hi;
+        } else {
+            const int r = s4 - 64;
+            w1 = lam.lo << r;
+            w2 = (lam.hi << r) | (lam.lo >> (64 - r));
+            w3 = lam.hi >> (64 - r);
+        }
+        u128v c[4];
+        c[0] = gf128_reduce256(w0, w1, w2, w3);
+#pragma unroll
+        for (int b = 1; b < 4; b++) {
+            const uint64_t top = c[b - 1].hi >> 63;
+            c[b] = {(c[b - 1].lo << 1) ^ ((0ull - top) & GF128_POLY), (c[b - 1].hi << 1) | (c[b - 1].lo >> 63)};
+        }
+        uint4* tdst = tabs + tt * 512 + k * 16;
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            u128v e = {0ull, 0ull};
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                if ((v >> b) & 1) e = e ^ c[b];
+            tdst[v] = u128_to4(e);
+        }
+    }
+    __syncthreads();
+    const size_t job = job0 + sub;
+    const size_t tile_id = job >> lgnc;
+    const size_t c0 = (job & (nchunks - 1)) << 6;
+    const size_t lo = tile_id & (S - 1);
+    const size_t rbase = (B << (ell - d0)) + lo;
+    if (lane >= cc) return;
+    const size_t qoff = src_shared ? 0 : q * m;
+    u128v v[R];
+#pragma unroll
+    for (int a = 0; a < R; a++) v[a] = src[(qoff + rbase + a * S) * n + c0 + lane];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int h = R >> (j + 1);
+#pragma unroll
+        for (int a = 0; a < R; a++) {
+            if (a & h) continue;
+            const int tt = (1 << j) - 1 + (a >> (K - j));
+            const u128v u = v[a] ^ mul_tab128_lds(tabs + tt * 512, v[a + h]);
+            v[a] = u;
+            v[a + h] = v[a + h] ^ u;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < R; a++) dst[(q * m + rbase + a * S) * n + c0 + lane] = v[a];
+}
+
+static void fft_quarters128(const u128v* msg, u128v* mat, size_t m, size_t n, const u128v* tw, hipStream_t s) {
+    int ell = 0;
+    while (((size_t)1 << ell) < m) ell++;
+    const int KMAX = 3;
+    const int npass = (ell + KMAX - 1) / KMAX;
+    const size_t nchunks = (n + 63) / 64;
+    const int cc = n < 64 ? (int)n : 64;
+    (void)cc;
+    const bool by_quarter = m * n * 16 >= ((size_t)16 << 20);
+    for (int qi = 0; qi < (by_quarter ? 4 : 1); qi++) {
+        int d0 = 0;
+        for (int p = 0; p < npass; p++) {
+            const int K = ell / npass + (p < ell % npass ? 1 : 0);
+            const size_t S = (size_t)1 << (ell - d0 - K);
+            const size_t jobs = ((size_t)1 << (ell - K)) * nchunks;
+            int J = 1;
+            while (J < 4 && (size_t)(J * 2) <= S * nchunks && jobs % (size_t)(J * 2) == 0) J *= 2;
+            const u128v* src = (p == 0 && msg) ? msg : mat;
+            const int shared = (p == 0 && msg) ? 1 : 0;
+            dim3 grid((unsigned)(jobs / J), by_quarter ? 1 : 4);
+            const int q = by_quarter ? qi : -1;
+            switch (K) {
+                case 1: hipLaunchKernelGGL(k_fft_pass128<1>, grid, dim3(64 * J), 0, s, src, mat, m, n, ell, d0, q, shared, tw); break;
+                case 2: hipLaunchKernelGGL(k_fft_pass128<2>, grid, dim3(64 * J), 0, s, src, mat, m, n, ell, d0, q, shared, tw); break;
+                default: hipLaunchKernelGGL(k_fft_pass128<3>, grid, dim3(64 * J), 0, s, src, mat, m, n, ell, d0, q, shared, tw); break;
+            }
+            d0 += K;
+        }
     }
 }
 
