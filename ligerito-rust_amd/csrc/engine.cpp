@@ -196,7 +196,7 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
         } p64(ctx);
         const bool time_passes = ctx->profiling && encode_phase == PH_ENCODE0;
         PhaseTimer pt(ctx, encode_phase);
-        launch_transpose_scale<F>(d_poly, m, n, rs.d_pis_lo.p, rs.d_pis_hi.p, msg.p, s);
+        launch_transpose_scale<F>(d_poly, m, n, rs.d_pis.p, rs.d_pis_lo.p, rs.d_pis_hi.p, msg.p, s);
         launch_fft_quarters<F>(msg.p, w.mat.p, m, n, rs.d_tw.p, rs.d_tw_tabs.p, s, time_passes ? &p64 : nullptr);
         // msg returns to the pool; stream order keeps the FFT's reads first
     }
@@ -374,7 +374,15 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     const size_t N = cfg.initial_m * cfg.initial_n;
     size_t flen = N >> cfg.initial_k;
     PBuf<T> d_f(&ctx->pool, flen), d_f2(&ctx->pool, flen / 2);
-    {
+    bool folded = false;
+    if constexpr (F::BITS == 32) {
+        // the weights travel as a kernel argument
+        PhaseTimer pt(ctx, PH_FOLD);
+        folded = launch_fold32_weights(reinterpret_cast<const uint32_t*>(d_poly), flen, cfg.initial_k,
+                                       reinterpret_cast<const uint32_t*>(gr0.data()), reinterpret_cast<uint32_t*>(d_f.p), s);
+        LIG_HIP(hipGetLastError());
+    }
+    if (!folded) {
         T* h = (T*)(ctx->pinned(PIN_UP + gr0.size() * sizeof(T)) + PIN_UP);
         memcpy(h, gr0.data(), gr0.size() * sizeof(T));
         PBuf<T> d_gr(&ctx->pool, gr0.size()), d_grt(&ctx->pool, gr0.size() * F::TAB);

@@ -6,6 +6,7 @@
 // arrays.  All arithmetic is exact binary-field arithmetic.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <stdexcept>
 
 #include "blake3.hpp"
@@ -29,6 +30,34 @@ static inline unsigned grid_for(size_t work, unsigned block, unsigned cap = 6553
     if (g < 1) g = 1;
     if (g > cap) g = cap;
     return (unsigned)g;
+}
+
+// Workgroup barrier over LDS traffic only: waits for this wave's LDS
+// operations (lgkmcnt), not for its global stores, which may stay in flight
+// (__syncthreads would wait for them too).  The empty asm statements keep the
+// compiler from moving memory accesses across the barrier.
+#define LIG_LDS_BARRIER()                    \
+    do {                                     \
+        asm volatile("" ::: "memory");       \
+        __builtin_amdgcn_s_waitcnt(0xC07F);  \
+        __builtin_amdgcn_s_barrier();        \
+        asm volatile("" ::: "memory");       \
+    } while (0)
+
+// Writes the 16 entries of nibble position k (k = 0..7) of the
+// multiply-by-lam table at t (128 words, entry v at word 16k + v):
+// entry v = XOR over the bits b of v of c_b = lam * x^(4k+b).
+__device__ __forceinline__ void gf32_tab_build(uint32_t* t, uint32_t lam, int k) {
+    const uint32_t c0 = gf32_reduce64((uint64_t)lam << (4 * k));
+    const uint32_t c1 = (c0 << 1) ^ ((0u - (c0 >> 31)) & GF32_POLY);
+    const uint32_t c2 = (c1 << 1) ^ ((0u - (c1 >> 31)) & GF32_POLY);
+    const uint32_t c3 = (c2 << 1) ^ ((0u - (c2 >> 31)) & GF32_POLY);
+    const uint32_t c01 = c0 ^ c1, c23 = c2 ^ c3;
+    uint4* d4 = reinterpret_cast<uint4*>(t + k * 16);
+    d4[0] = make_uint4(0u, c0, c1, c01);
+    d4[1] = make_uint4(c2, c2 ^ c0, c2 ^ c1, c2 ^ c01);
+    d4[2] = make_uint4(c3, c3 ^ c0, c3 ^ c1, c3 ^ c01);
+    d4[3] = make_uint4(c23, c23 ^ c0, c23 ^ c1, c23 ^ c01);
 }
 
 // ------------------------------------------------------------------ tables --
@@ -102,58 +131,89 @@ __device__ __forceinline__ uint32_t mul_tab_lds(const uint32_t* tabs, uint32_t t
 
 // GF(2^32) variant: the 64 low-part pis tables stay in LDS for the whole
 // workgroup, which walks several 64x64 tiles; each tile adds one table.
+// Tiles of 64 rows x 2^LCT columns (2^LCT = min(n, 64); m >= 64), with
+// workgroups persistent over tiles.  A tile's rows i0..i0+63 are scaled by
+// pis[i0 + r]: the workgroup builds one multiply table per row in LDS, so
+// each element costs one table multiply.  The next tile's loads are issued
+// before this tile is scaled and stored; the barriers wait for LDS traffic
+// only.
+template <int LCT>
 __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __restrict__ poly, size_t m, size_t n,
-                                                          const uint32_t* __restrict__ lo_tabs,
-                                                          const uint32_t* __restrict__ hi_tabs, uint32_t* __restrict__ msg) {
-    __shared__ __attribute__((aligned(256))) uint32_t tlo[64 * 128];
-    __shared__ __attribute__((aligned(256))) uint32_t thi[128];
-    __shared__ uint32_t tile[64][65];
-    const int rt = m < 64 ? (int)m : 64;
-    const int ct = n < 64 ? (int)n : 64;
-    const int lrt = 31 - __clz(rt), lct = 31 - __clz(ct);
-    const size_t row_tiles = (m + 63) / 64, col_tiles = (n + 63) / 64;
-    stage_words<32>(tlo, lo_tabs, rt * 128);
-    for (size_t tile_i = blockIdx.x; tile_i < row_tiles * col_tiles; tile_i += gridDim.x) {
-        const size_t i0 = (tile_i % row_tiles) * 64;
-        const size_t c0 = (tile_i / row_tiles) * 64;
-        if (threadIdx.x < 128) thi[threadIdx.x] = hi_tabs[(i0 >> 6) * 128 + threadIdx.x];
-        if (rt == 64 && ct == 64) {
-            uint32_t tv[16];
+                                                          const uint32_t* __restrict__ pis, uint32_t* __restrict__ msg) {
+    constexpr int CT = 1 << LCT, KPT = (64 * CT) / 256;  // elements per thread
+    static_assert(KPT >= 1, "at least four columns");
+    __shared__ __attribute__((aligned(256))) uint32_t rtab[64 * 128];
+    __shared__ uint32_t tile[CT][65];
+    const size_t row_tiles = m / 64, ntiles = row_tiles * (n / CT);
+    const int tid = threadIdx.x;
+    uint32_t tv[KPT], lam[2];
+    auto load = [&](size_t ti) {
+        const size_t i0 = (ti % row_tiles) * 64, c0 = (ti / row_tiles) * CT;
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const int idx = threadIdx.x + 256 * k;
-                tv[k] = poly[(c0 + (idx >> 6)) * m + i0 + (idx & 63)];
-            }
+        for (int k = 0; k < KPT; k++) {
+            const int idx = tid + 256 * k;
+            tv[k] = poly[(c0 + (idx >> 6)) * m + i0 + (idx & 63)];
+        }
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const int idx = threadIdx.x + 256 * k;
-                tile[idx >> 6][idx & 63] = tv[k];
-            }
-        } else {
-            for (int idx = threadIdx.x; idx < rt * ct; idx += blockDim.x) {
-                int r = idx & (rt - 1), c = idx >> lrt;
-                tile[c][r] = poly[(c0 + c) * m + i0 + r];
-            }
+        for (int h = 0; h < 2; h++) lam[h] = pis[i0 + ((tid + 256 * h) >> 3)];
+    };
+    if (blockIdx.x < ntiles) load(blockIdx.x);
+    for (size_t tile_i = blockIdx.x; tile_i < ntiles; tile_i += gridDim.x) {
+        const size_t i0 = (tile_i % row_tiles) * 64, c0 = (tile_i / row_tiles) * CT;
+#pragma unroll
+        for (int k = 0; k < KPT; k++) {
+            const int idx = tid + 256 * k;
+            tile[idx >> 6][idx & 63] = tv[k];
         }
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < rt * ct; idx += blockDim.x) {
-            int c = idx & (ct - 1), r = idx >> lct;
-            uint32_t x = mul_tab_lds(tlo, (uint32_t)(r * 512), tile[c][r]);
-            x = mul_tab_lds(thi, 0u, x);
-            msg[(i0 + r) * n + c0 + c] = x;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int item = tid + 256 * h;
+            gf32_tab_build(rtab + (item >> 3) * 128, lam[h], item & 7);
         }
-        __syncthreads();
+        LIG_LDS_BARRIER();
+        if (tile_i + gridDim.x < ntiles) load(tile_i + gridDim.x);
+#pragma unroll
+        for (int k = 0; k < KPT; k++) {
+            const int idx = tid + 256 * k;
+            const int c = idx & (CT - 1), r = idx >> LCT;
+            msg[(i0 + r) * n + c0 + c] = mul_tab_lds(rtab, (uint32_t)(r * 512), tile[c][r]);
+        }
+        LIG_LDS_BARRIER();  // tile and tables are read before the next tile overwrites them
     }
 }
 
+// Compute units of the current device (cached per device).
+static int device_cu_count() {
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int cu = cached[dev].load(std::memory_order_relaxed);
+    if (cu <= 0) {
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0) cu = 256;
+        cached[dev].store(cu, std::memory_order_relaxed);
+    }
+    return cu;
+}
+
 template <class F>
-void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const typename F::T* lo_tabs,
-                            const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s) {
+void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const typename F::T* pis,
+                            const typename F::T* lo_tabs, const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s) {
     size_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
     if constexpr (F::BITS == 32) {
-        unsigned g = (unsigned)(tiles < 1024 ? tiles : 1024);
-        hipLaunchKernelGGL(k_transpose_scale32, dim3(g), dim3(256), 0, s, poly, m, n, lo_tabs, hi_tabs, msg);
-        return;
+        if (m >= 64 && n >= 4 && (m & (m - 1)) == 0 && (n & (n - 1)) == 0) {
+            int lct = 0;
+            while ((1 << lct) < (n < 64 ? (int)n : 64)) lct++;
+            // three workgroups fit a CU (49 KB of LDS at 64 columns)
+            const size_t cap = 3 * (size_t)device_cu_count();
+            const unsigned g = (unsigned)(tiles < cap ? tiles : cap);
+            switch (lct) {
+                case 2: hipLaunchKernelGGL(k_transpose_scale32<2>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
+                case 3: hipLaunchKernelGGL(k_transpose_scale32<3>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
+                case 4: hipLaunchKernelGGL(k_transpose_scale32<4>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
+                case 5: hipLaunchKernelGGL(k_transpose_scale32<5>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
+                default: hipLaunchKernelGGL(k_transpose_scale32<6>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
+            }
+        }
     }
     hipLaunchKernelGGL(k_transpose_scale<F>, dim3((unsigned)tiles), dim3(256), 0, s, poly, m, n, lo_tabs, hi_tabs, msg);
 }
@@ -591,21 +651,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
                 const int d = d0 + j;
                 const size_t first = ((size_t)1 << (d + 2)) - 1;
                 const uint32_t lam = tw[first + ((size_t)q << d) + ((size_t)B << j) + tp];
-                const uint32_t c0 = gf32_reduce64((uint64_t)lam << (4 * k));
-                const uint32_t c1 = (c0 << 1) ^ ((0u - (c0 >> 31)) & GF32_POLY);
-                const uint32_t c2 = (c1 << 1) ^ ((0u - (c1 >> 31)) & GF32_POLY);
-                const uint32_t c3 = (c2 << 1) ^ ((0u - (c2 >> 31)) & GF32_POLY);
-                const uint32_t c01 = c0 ^ c1, c23 = c2 ^ c3;
-                uint4* tdst4 = reinterpret_cast<uint4*>(tabs + tt * 128 + k * 16);
-                tdst4[0] = make_uint4(0u, c0, c1, c01);
-                tdst4[1] = make_uint4(c2, c2 ^ c0, c2 ^ c1, c2 ^ c01);
-                tdst4[2] = make_uint4(c3, c3 ^ c0, c3 ^ c1, c3 ^ c01);
-                tdst4[3] = make_uint4(c23, c23 ^ c0, c23 ^ c1, c23 ^ c01);
+                gf32_tab_build(tabs + tt * 128, lam, k);
             }
         } else {
-            __syncthreads();  // previous tile's exchange reads are done
+            LIG_LDS_BARRIER();  // previous tile's exchange reads are done
         }
-        __syncthreads();
+        LIG_LDS_BARRIER();  // tables complete (this tile's loads may still be in flight)
 
 #pragma unroll
         for (int j = 0; j < K1; j++) {
@@ -635,7 +686,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
         } else {
 #pragma unroll
             for (int a = 0; a < L1; a++) xch[(a * L2 + g) * 64 + c] = v[a];
-            __syncthreads();
+            LIG_LDS_BARRIER();
             constexpr int NC2 = L1 / L2;
 #pragma unroll
             for (int i = 0; i < NC2; i++) {
@@ -983,9 +1034,114 @@ __global__ __launch_bounds__(256) void k_merkle_level(uint32_t* __restrict__ tre
     }
 }
 
-// Wide levels one launch each (every lane busy); once a level is narrower
-// than MERKLE_WIDE parents, workgroups of k_merkle_subtree finish the top.
-constexpr size_t MERKLE_WIDE = (size_t)1 << 15;
+// ---------------------------------------- four-lane parent compression ---
+// Near the root a level has too few parents to fill the GPU, so its time is
+// the latency of one compression.  Four lanes of a quad share one parent:
+// lane k holds column k of the 4x4 state (words k, 4+k, 8+k, 12+k), runs
+// the column-step G_k, then the diagonal-step G_(4+k) after rotating rows
+// 1, 2, 3 left by 1, 2, 3 lanes (quad DPP moves), and rotates them back.
+// Message word for round r: column step S_r[2k], S_r[2k+1]; diagonal step
+// S_r[8+2k], S_r[9+2k], with S_r the BLAKE3 schedule of compress_cv.
+namespace b3x4 {
+constexpr uint8_t SCHED[7][16] = {{0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15},
+                                  {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8},
+                                  {3, 4, 10, 12, 13, 2, 7, 14, 6, 5, 9, 0, 11, 15, 8, 1},
+                                  {10, 7, 12, 9, 14, 3, 13, 15, 4, 0, 11, 2, 5, 8, 1, 6},
+                                  {12, 13, 9, 11, 15, 10, 14, 8, 7, 2, 5, 3, 0, 1, 6, 4},
+                                  {9, 14, 11, 5, 8, 12, 15, 1, 13, 3, 0, 10, 2, 6, 4, 7},
+                                  {11, 15, 5, 0, 1, 9, 8, 6, 14, 10, 2, 12, 3, 4, 7, 13}};
+// word indices of slot s (0, 1: column step; 2, 3: diagonal step) for the
+// four lanes, four bits per lane
+constexpr uint32_t packed(int r, int s) {
+    uint32_t v = 0;
+    for (int k = 0; k < 4; k++) v |= (uint32_t)SCHED[r][(s < 2 ? 2 * k + s : 8 + 2 * k + (s - 2))] << (4 * k);
+    return v;
+}
+// lane i of each quad receives lane sel_i: quad_perm(1,2,3,0) = 0x39,
+// (2,3,0,1) = 0x4E, (3,0,1,2) = 0x93
+template <int CTRL>
+__device__ __forceinline__ uint32_t qperm(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+}  // namespace b3x4
+
+// Subtree of depth g per workgroup (blockDim.x / 4 quads): 2^g nodes of the
+// layer at in_off become the g layers above them.  Dynamic LDS holds two
+// layers of at most 2^g nodes (2^(g+6) bytes).  Each level's nodes go to LDS
+// for the next level and to the tree in global memory; the barrier between
+// levels does not wait for the global stores.
+__global__ __launch_bounds__(1024) void k_merkle_subtree4(uint32_t* __restrict__ tree, size_t in_off, size_t n_in, int g,
+                                                          uint32_t* __restrict__ root_out) {
+    using namespace b3x4;
+    extern __shared__ uint32_t sub4_lds[];
+    uint32_t* lv[2] = {sub4_lds, sub4_lds + ((size_t)8 << g)};
+    const int k = threadIdx.x & 3, quad = threadIdx.x >> 2, nquads = blockDim.x >> 2;
+    const size_t base = blockIdx.x * ((size_t)1 << g);  // first input node of this workgroup
+    // message word index per (round, slot) for this lane
+    uint32_t widx[28];
+#pragma unroll
+    for (int r = 0; r < 7; r++)
+#pragma unroll
+        for (int s = 0; s < 4; s++) widx[4 * r + s] = (packed(r, s) >> (4 * k)) & 15u;
+    // initial state column k: IV[k], IV[4+k], IV[k], (counter lo, hi, 64, flags)[k]
+    const uint32_t iv_lo = k == 0 ? b3::iv(0) : k == 1 ? b3::iv(1) : k == 2 ? b3::iv(2) : b3::iv(3);
+    const uint32_t iv_hi = k == 0 ? b3::iv(4) : k == 1 ? b3::iv(5) : k == 2 ? b3::iv(6) : b3::iv(7);
+    const uint32_t d_init = k == 2 ? 64u : k == 3 ? (uint32_t)(b3::CHUNK_START | b3::CHUNK_END | b3::ROOT) : 0u;
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(tree + (in_off + base) * 8);
+        uint4* dst = reinterpret_cast<uint4*>(lv[0]);
+        for (int i = threadIdx.x; i < (2 << g); i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+    size_t lay_off = in_off, lay_n = n_in;
+    for (int l = 1; l <= g; l++) {
+        const int cnt = 1 << (g - l);
+        const uint32_t* in = lv[(l - 1) & 1];
+        uint32_t* out = lv[l & 1];
+        const size_t out_off = lay_off + lay_n;
+        const size_t out_base = base >> l;
+        for (int p = quad; p < cnt; p += nquads) {
+            const uint32_t* m = in + 16 * p;  // left child, then right child
+            uint32_t mw[28];
+#pragma unroll
+            for (int i = 0; i < 28; i++) mw[i] = m[widx[i]];
+            uint32_t a = iv_lo, b = iv_hi, c = iv_lo, d = d_init;
+#pragma unroll
+            for (int r = 0; r < 7; r++) {
+                LIG_B3_G(a, b, c, d, mw[4 * r], mw[4 * r + 1])
+                b = qperm<0x39>(b);
+                c = qperm<0x4E>(c);
+                d = qperm<0x93>(d);
+                LIG_B3_G(a, b, c, d, mw[4 * r + 2], mw[4 * r + 3])
+                b = qperm<0x93>(b);
+                c = qperm<0x4E>(c);
+                d = qperm<0x39>(d);
+            }
+            const uint32_t lo = a ^ c, hi = b ^ d;  // words k and 4 + k of the parent
+            out[8 * p + k] = lo;
+            out[8 * p + 4 + k] = hi;
+            uint32_t* dst = tree + (out_off + out_base + p) * 8;
+            dst[k] = lo;
+            dst[4 + k] = hi;
+            if (root_out && cnt == 1 && lay_n == 2) {
+                root_out[k] = lo;
+                root_out[4 + k] = hi;
+            }
+        }
+        LIG_LDS_BARRIER();
+        lay_off = out_off;
+        lay_n >>= 1;
+    }
+}
+
+// Wide levels one launch each, one lane per parent (throughput-bound);
+// once a level has fewer than MERKLE_WIDE parents, k_merkle_subtree4
+// finishes the top with four lanes per parent (latency-bound).
+constexpr size_t MERKLE_WIDE = (size_t)1 << 17;
+// The top (at most 2^SUB4_TOP nodes) in one 1024-thread workgroup; below
+// it, 256-thread workgroups of depth SUB4_G.
+constexpr int SUB4_G = 7, SUB4_TOP = 10;
 
 void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out) {
     size_t in_off = 0, nodes = n_leaves;
@@ -998,10 +1154,12 @@ void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32
     while (nodes > 1) {
         int lg = 0;
         while (((size_t)1 << lg) < nodes) lg++;
-        int g = lg < 9 ? lg : 9;
-        size_t groups = nodes >> g;
-        hipLaunchKernelGGL(k_merkle_subtree, dim3((unsigned)groups), dim3(256), 0, s, tree, in_off, nodes, g, root_out);
-        // advance to the top layer produced
+        const bool top = lg <= SUB4_TOP;
+        const int g = top ? lg : SUB4_G;
+        const size_t groups = nodes >> g;
+        const size_t lds = (size_t)64 << g;  // two layers of 2^g nodes
+        hipLaunchKernelGGL(k_merkle_subtree4, dim3((unsigned)groups), dim3(top ? 1024 : 256), lds, s, tree, in_off, nodes, g,
+                           root_out);
         for (int l = 0; l < g; l++) {
             in_off += nodes;
             nodes >>= 1;
@@ -1034,28 +1192,62 @@ __global__ void k_fold(const typename F::T* __restrict__ poly, size_t m, int kk,
     }
 }
 
-__global__ __launch_bounds__(256) void k_fold32(const uint32_t* __restrict__ poly, size_t m, int kk,
-                                                 const uint32_t* __restrict__ gr_tabs, uint32_t* __restrict__ out) {
-    __shared__ __attribute__((aligned(256))) uint32_t tabs[64 * 128];
+// One output row per thread.  Its 2^kk column values are loaded 16 at a
+// time with buffer loads (lane offset in one register, column offset in a
+// scalar register), the next 16 issued before the current 16 are
+// multiplied; the first 16 are in flight while the workgroup builds its
+// 2^kk multiply-by-weight tables in LDS from the weights (a kernel
+// argument, copied to LDS by one lane).
+struct FoldWeights32 {
+    uint32_t w[1 << FOLD32_MAXK];
+};
+__global__ __launch_bounds__(256) void k_fold32(const uint32_t* __restrict__ poly, size_t m, int kk, FoldWeights32 gr,
+                                                 uint32_t* __restrict__ out) {
+    constexpr int NC = 1 << FOLD32_MAXK, BATCH = 16;
+    __shared__ __attribute__((aligned(256))) uint32_t tabs[NC * 128];
+    __shared__ uint32_t wsh[NC];
     const int nc = 1 << kk;
-    stage_words<32>(tabs, gr_tabs, nc * 128);
-    __syncthreads();
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) {
-        uint32_t acc = 0;
-#pragma unroll 8
-        for (int j = 0; j < nc; j++) acc ^= mul_tab_lds(tabs, (uint32_t)(j * 512), poly[(size_t)j * m + i]);
-        out[i] = acc;
+    const uint32_t col_bytes = (uint32_t)(m * 4);
+    const __amdgpu_buffer_rsrc_t rsrc = buf_rsrc(poly, col_bytes * (uint32_t)nc);
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const uint32_t roff = (uint32_t)(i * 4);  // rows past m read other columns or 0; never stored
+    uint32_t v[BATCH];
+#pragma unroll
+    for (int t = 0; t < BATCH; t++) v[t] = t < nc ? buf_ld(rsrc, roff, (uint32_t)t * col_bytes) : 0u;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NC; j++) wsh[j] = gr.w[j];
     }
+    LIG_LDS_BARRIER();
+    for (int idx = threadIdx.x; idx < nc * 8; idx += blockDim.x) gf32_tab_build(tabs + (idx >> 3) * 128, wsh[idx >> 3], idx & 7);
+    LIG_LDS_BARRIER();
+    uint32_t acc = 0;
+    for (int j0 = 0; j0 < nc; j0 += BATCH) {
+        uint32_t w[BATCH];
+#pragma unroll
+        for (int t = 0; t < BATCH; t++) w[t] = v[t];
+        if (j0 + BATCH < nc) {
+#pragma unroll
+            for (int t = 0; t < BATCH; t++)
+                v[t] = j0 + BATCH + t < nc ? buf_ld(rsrc, roff, (uint32_t)(j0 + BATCH + t) * col_bytes) : 0u;
+        }
+#pragma unroll
+        for (int t = 0; t < BATCH; t++)
+            if (j0 + t < nc) acc ^= mul_tab_lds(tabs, (uint32_t)((j0 + t) * 512), w[t]);
+    }
+    if (i < m) out[i] = acc;
+}
+
+bool launch_fold32_weights(const uint32_t* poly, size_t m, int kk, const uint32_t* gr_host, uint32_t* out, hipStream_t s) {
+    if (kk < 0 || kk > FOLD32_MAXK || (m << kk) * 4 >= ((size_t)1 << 31) || (m + 255) / 256 > 0x7fffffffu) return false;
+    FoldWeights32 gr{};
+    for (int j = 0; j < (1 << kk); j++) gr.w[j] = gr_host[j];
+    hipLaunchKernelGGL(k_fold32, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, poly, m, kk, gr, out);
+    return true;
 }
 
 template <class F>
 void launch_fold(const typename F::T* poly, size_t m, int kk, const typename F::T* gr_tabs, typename F::T* out, hipStream_t s) {
-    if constexpr (F::BITS == 32) {
-        if (kk <= 6) {
-            hipLaunchKernelGGL(k_fold32, dim3(grid_for(m, 256, 1024)), dim3(256), 0, s, poly, m, kk, gr_tabs, out);
-            return;
-        }
-    }
     hipLaunchKernelGGL(k_fold<F>, dim3(grid_for(m, 256, 4096)), dim3(256), 0, s, poly, m, kk, gr_tabs, out);
 }
 
@@ -1352,6 +1544,17 @@ void launch_enforced(const typename F::T* rows, int nrows, int ncols, const type
 }
 
 // -------------------------------------------------------------- sumcheck ---
+// XOR over the 64 lanes of a wave (every lane gets the total)
+__device__ __forceinline__ uint32_t wave_xor32(uint32_t x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x ^= (uint32_t)__shfl_xor((int)x, d, 64);
+    return x;
+}
+__device__ __forceinline__ uint64_t wave_xor(uint64_t x) {
+    return (uint64_t)wave_xor32((uint32_t)x) | ((uint64_t)wave_xor32((uint32_t)(x >> 32)) << 32);
+}
+__device__ __forceinline__ u128v wave_xor(u128v x) { return {wave_xor(x.lo), wave_xor(x.hi)}; }
+
 template <class F> __device__ inline void acc_atomic_xor(typename F::Acc* dst, typename F::Acc v);
 template <> __device__ inline void acc_atomic_xor<F32>(uint64_t* dst, uint64_t v) {
     if (v) atomicXor(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)v);
@@ -1362,25 +1565,26 @@ template <> __device__ inline void acc_atomic_xor<F128>(u128v* dst, u128v v) {
 }
 
 // XOR-reduces three accumulators across the workgroup and folds them into res.
+// XOR of three accumulators over the workgroup (a multiple of 64 threads,
+// at most 1024), added into res with atomics: wave shuffles, then one
+// partial per wave through LDS.
 template <class F> __device__ inline void block_xor3(typename F::Acc a0, typename F::Acc a1, typename F::Acc a2, typename F::Acc* res) {
     using A = typename F::Acc;
-    __shared__ A red[3][256];
-    red[0][threadIdx.x] = a0;
-    red[1][threadIdx.x] = a1;
-    red[2][threadIdx.x] = a2;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            red[0][threadIdx.x] ^= red[0][threadIdx.x + w];
-            red[1][threadIdx.x] ^= red[1][threadIdx.x + w];
-            red[2][threadIdx.x] ^= red[2][threadIdx.x + w];
-        }
-        __syncthreads();
+    __shared__ A red[3][16];
+    a0 = wave_xor(a0);
+    a1 = wave_xor(a1);
+    a2 = wave_xor(a2);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = a0;
+        red[1][w] = a1;
+        red[2][w] = a2;
     }
-    if (threadIdx.x == 0) {
-        acc_atomic_xor<F>(&res[0], red[0][0]);
-        acc_atomic_xor<F>(&res[1], red[1][0]);
-        acc_atomic_xor<F>(&res[2], red[2][0]);
+    LIG_LDS_BARRIER();
+    if (threadIdx.x < 3) {
+        A t = red[threadIdx.x][0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) t ^= red[threadIdx.x][i];
+        acc_atomic_xor<F>(&res[threadIdx.x], t);
     }
 }
 
@@ -1426,11 +1630,18 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
     using T = typename F::T;
     using A = typename F::Acc;
     if (blockIdx.x == 0 && threadIdx.x < 3) res_clear[threadIdx.x] = A{};
+    const size_t L2 = len / 2, h2 = L2 / 2;
+    const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    // the first element group is loaded while the challenge's table is built
+    T fa{}, fb{}, fc{}, fd{}, ba{}, bb{}, bc{}, bd{};
+    if (i0 < h2) {
+        fa = f[i0], fb = f[i0 + L2], fc = f[i0 + h2], fd = f[i0 + h2 + L2];
+        ba = b[i0], bb = b[i0 + L2], bc = b[i0 + h2], bd = b[i0 + h2 + L2];
+    }
     const T r = *rp;
     __shared__ T tab[F::TAB];
     for (int e = threadIdx.x; e < F::TAB; e += blockDim.x) tab[e] = Ops<F>::table_entry(r, e);
-    __syncthreads();
-    const size_t L2 = len / 2, h2 = L2 / 2;
+    LIG_LDS_BARRIER();
     A s0 = Ops<F>::acc_zero(), s1 = Ops<F>::acc_zero(), s2 = Ops<F>::acc_zero();
     if (h2 == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1438,10 +1649,12 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
             b2[0] = b[0] ^ Ops<F>::mul_tab(tab, b[0] ^ b[1]);
         }
     } else {
-        for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < h2; i += (size_t)gridDim.x * blockDim.x) {
+        for (size_t i = i0; i < h2; i += stride) {
+            if (i != i0) {
+                fa = f[i], fb = f[i + L2], fc = f[i + h2], fd = f[i + h2 + L2];
+                ba = b[i], bb = b[i + L2], bc = b[i + h2], bd = b[i + h2 + L2];
+            }
             // (1+r)*e[i] + r*e[i+L2] = e[i] + r*(e[i] + e[i+L2])  (multilinear_poly.rs:71-73)
-            T fa = f[i], fb = f[i + L2], fc = f[i + h2], fd = f[i + h2 + L2];
-            T ba = b[i], bb = b[i + L2], bc = b[i + h2], bd = b[i + h2 + L2];
             T f0 = fa ^ Ops<F>::mul_tab(tab, fa ^ fb);
             T f1 = fc ^ Ops<F>::mul_tab(tab, fc ^ fd);
             T g0 = ba ^ Ops<F>::mul_tab(tab, ba ^ bb);
@@ -1505,9 +1718,11 @@ __global__ __launch_bounds__(64) void k_fs_step(DevTranscript* __restrict__ st, 
 #pragma unroll
         for (int k = 0; k < 3; k++) t[k] = Ops<F>::acc_reduce(dfs::uniform_acc(res[k]));
         if (enf) {
+            // parts loaded across the lanes, then XOR-reduced over the wave
             A e = Ops<F>::acc_zero();
-            for (int i = 0; i < n_enf; i++) e ^= dfs::uniform_acc(enf[i]);
-            if (!Ops<F>::eq(t[0] ^ t[1], Ops<F>::acc_reduce(e))) err = 1;
+            for (int i = threadIdx.x; i < n_enf; i += 64) e ^= enf[i];
+            e = wave_xor(e);
+            if (!Ops<F>::eq(t[0] ^ t[1], Ops<F>::acc_reduce(dfs::uniform_acc(e)))) err = 1;
         }
 #pragma unroll
         for (int k = 0; k < 3; k++)
@@ -1543,7 +1758,8 @@ void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename
 // ------------------------------------------------------- instantiations ---
 #define LIG_INST(F)                                                                                                         \
     template void launch_build_tables<F>(const F::T*, size_t, F::T*, hipStream_t);                                          \
-    template void launch_transpose_scale<F>(const F::T*, size_t, size_t, const F::T*, const F::T*, F::T*, hipStream_t);     \
+    template void launch_transpose_scale<F>(const F::T*, size_t, size_t, const F::T*, const F::T*, const F::T*, F::T*,      \
+                                            hipStream_t);                                                                   \
     template void launch_scale_rows<F>(F::T*, size_t, size_t, const F::T*, hipStream_t);                                    \
     template void launch_fft_stage<F>(F::T*, size_t, size_t, int, const F::T*, int, hipStream_t);                           \
     template void launch_fft_quarters<F>(const F::T*, F::T*, size_t, size_t, const F::T*, const F::T*, hipStream_t,         \

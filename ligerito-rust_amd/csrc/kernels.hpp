@@ -46,11 +46,12 @@ template <class F> void launch_build_tables(const typename F::T* lams, size_t co
 
 // ------------------------------------------------------------ Reed–Solomon --
 // msg[i*n + c] = poly[c*m + i] * pis[i]  (poly is column-major as in
-// ligero.rs:11-27; msg is row-major).  pis[i] = pis[i & 63] * pis[i & ~63];
-// lo_tabs has min(m,64) tables, hi_tabs has max(m/64,1) tables.
+// ligero.rs:11-27; msg is row-major).  GF(2^32) reads the plain pis (m
+// words); GF(2^128) uses pis[i] = pis[i & 63] * pis[i & ~63] with lo_tabs
+// (min(m,64) tables) and hi_tabs (max(m/64,1) tables).
 template <class F>
-void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const typename F::T* lo_tabs,
-                            const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s);
+void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const typename F::T* pis,
+                            const typename F::T* lo_tabs, const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s);
 // buf[i*n + c] *= pis[i] for i < rows (general multiply).
 template <class F> void launch_scale_rows(typename F::T* buf, size_t rows, size_t n, const typename F::T* pis, hipStream_t s);
 // One additive-FFT stage on a row-major [N][n] buffer with general multiply
@@ -98,6 +99,11 @@ void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t 
 // kk variables, multilinear_poly.rs:66-87).  gr_tabs: 2^kk tables.
 template <class F>
 void launch_fold(const typename F::T* poly, size_t m, int kk, const typename F::T* gr_tabs, typename F::T* out, hipStream_t s);
+// The same for GF(2^32) with the weights themselves (host memory, passed as
+// a kernel argument); the kernel builds its tables.  Returns false (nothing
+// launched) unless kk <= FOLD32_MAXK and the 2^kk columns span < 2 GiB.
+constexpr int FOLD32_MAXK = 6;
+bool launch_fold32_weights(const uint32_t* poly, size_t m, int kk, const uint32_t* gr_host, uint32_t* out, hipStream_t s);
 // out (and out2, if given) receive the count rows listed in rows.
 template <class F>
 void launch_gather_rows(const typename F::T* mat, size_t n, const uint64_t* rows, size_t count, typename F::T* out,

@@ -69,6 +69,9 @@ def lib():
             "oracle_prove": (I, [I, I, P, C.c_uint32, P, SZ, C.POINTER(SZ)]),
             "oracle_verify": (I, [I, I, P, SZ, C.c_uint32]),
             "oracle_splitmix_fill": (None, [C.c_uint64, I, SZ, P]),
+            "oracle_set_threads": (None, [I]),
+            "oracle_prove_timed": (I, [I, I, P, C.c_uint32, C.POINTER(C.c_double)]),
+            "oracle_get_threads": (I, []),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -387,14 +390,27 @@ def prove(poly, L: int, bits: int = 32, fs_seed: int = 1234) -> bytes:
     """src/prover.rs:15-199 on the generated config for 2^L (cfg20/cfg24 at L=20/24)."""
     p = as_elems(poly, bits)
     ln = C.c_size_t(0)
-    rc = lib().oracle_prove(bits, L, _p(p), fs_seed, None, 0, C.byref(ln))
+    out = np.zeros(16 << 20, dtype=np.uint8)
+    rc = lib().oracle_prove(bits, L, _p(p), fs_seed, _p(out), out.size, C.byref(ln))
+    if rc == -11:  # larger than the first buffer: prove again into one of the right size
+        out = np.zeros(ln.value, dtype=np.uint8)
+        rc = lib().oracle_prove(bits, L, _p(p), fs_seed, _p(out), out.size, C.byref(ln))
     if rc:
         raise RuntimeError(f"oracle prover failed: {rc}")
-    out = np.zeros(ln.value, dtype=np.uint8)
-    rc = lib().oracle_prove(bits, L, _p(p), fs_seed, _p(out), ln.value, C.byref(ln))
+    return out[: ln.value].tobytes()
+
+
+def prove_timed(poly, L: int, bits: int = 32, fs_seed: int = 1234, threads: int = 0) -> float:
+    """Seconds of prover() alone (config built before the clock, as
+    src/main.rs:29-49 times it), on `threads` OpenMP threads (0 = default)."""
+    p = as_elems(poly, bits)
+    if threads:
+        lib().oracle_set_threads(threads)
+    t = C.c_double(0)
+    rc = lib().oracle_prove_timed(bits, L, _p(p), fs_seed, C.byref(t))
     if rc:
         raise RuntimeError(f"oracle prover failed: {rc}")
-    return out.tobytes()
+    return t.value
 
 
 def verify(proof: bytes, L: int, bits: int = 32, fs_seed: int = 1234) -> int:

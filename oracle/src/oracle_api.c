@@ -11,6 +11,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
+#include <time.h>
 
 #include "blake3.h"
 #include "fs.h"
@@ -49,6 +51,11 @@
 
 #define EXPORT __attribute__((visibility("default")))
 
+/* Thread count of the OpenMP loops (column encode, query induce); the
+ * single-core CPU baseline sets it to 1. */
+EXPORT void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+EXPORT int oracle_get_threads(void) { return omp_get_max_threads(); }
+
 /* Dispatch helper: body is instantiated for each field with T and P set. */
 #define DISPATCH(bits, BODY16, BODY32, BODY128) \
     switch (bits) {                              \
@@ -60,7 +67,7 @@
 
 /* ---- element (de)serialisation helpers -------------------------------- */
 #define LOADV(T, P, src, n, dst)                                         \
-    T *dst = (T *)malloc(((n) ? (n) : 1) * sizeof(T));                   \
+    T *dst = (T *)calloc((size_t)(n) != 0 ? (size_t)(n) : 1, sizeof(T)); \
     for (size_t _i = 0; _i < (size_t)(n); _i++) dst[_i] = P##_from_le((const uint8_t *)(src) + _i * sizeof(T));
 #define STOREV(T, P, src, n, dst) \
     for (size_t _i = 0; _i < (size_t)(n); _i++) P##_to_le((src)[_i], (uint8_t *)(dst) + _i * sizeof(T));
@@ -114,6 +121,7 @@ EXPORT int oracle_compute_twiddles(int bits, int log_n, uint8_t *out) {
 }
 
 EXPORT int oracle_fft(int bits, uint8_t *v, size_t n, const uint8_t *tw, int inverse) {
+    if (n < 2) return -1;
 #define BODY(T, P)                                  \
     {                                               \
         LOADV(T, P, v, n, vv);                      \
@@ -399,6 +407,31 @@ EXPORT int oracle_prove(int bits, int L, const uint8_t *poly, uint32_t fs_seed, 
         }                                                            \
         free(pv);                                                    \
         P##_config_free(&cfg);                                       \
+    }
+    DISPATCH(bits, BODY(uint16_t, gf16), BODY(uint32_t, gf32), BODY(unsigned __int128, gf128));
+#undef BODY
+    return rc;
+}
+
+/* The timing the reference's main prints (main.rs:29-49): the configuration
+ * (twiddles, pis) is built before the clock starts and the proof bytes are
+ * not serialized; *prove_s is the wall time of prover() alone. */
+EXPORT int oracle_prove_timed(int bits, int L, const uint8_t *poly, uint32_t fs_seed, double *prove_s) {
+    int rc = 0;
+#define BODY(T, P)                                                              \
+    {                                                                           \
+        P##_config cfg;                                                         \
+        if (P##_config_new(&cfg, L)) return -10;                                \
+        LOADV(T, P, poly, (size_t)1 << L, pv);                                  \
+        P##_proof pr;                                                           \
+        struct timespec t0, t1;                                                 \
+        clock_gettime(CLOCK_MONOTONIC, &t0);                                    \
+        rc = P##_prover(&cfg, pv, fs_seed, &pr);                                \
+        clock_gettime(CLOCK_MONOTONIC, &t1);                                    \
+        *prove_s = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec); \
+        if (rc == 0) P##_proof_free(&pr);                                       \
+        free(pv);                                                               \
+        P##_config_free(&cfg);                                                  \
     }
     DISPATCH(bits, BODY(uint16_t, gf16), BODY(uint32_t, gf32), BODY(unsigned __int128, gf128));
 #undef BODY

@@ -114,6 +114,22 @@ def test_prove_verify_L16(O, bits):
     assert O.verify(pr2, 16, bits) != 1
 
 
+def test_proof_independent_of_thread_count(O):
+    # the OpenMP loops (column encode, per-query induce) reduce by XOR, so the
+    # bytes must not depend on how many threads ran them
+    poly = O.splitmix(8, 1 << 16, 32)
+    default = O.lib().oracle_get_threads()
+    try:
+        O.lib().oracle_set_threads(1)
+        one = O.prove(poly, 16, 32)
+        assert O.prove_timed(poly, 16, 32, threads=1) > 0
+        O.lib().oracle_set_threads(3)
+        assert O.prove(poly, 16, 32) == one
+    finally:
+        O.lib().oracle_set_threads(default)
+    assert O.prove(poly, 16, 32) == one
+
+
 def test_verify_rejects_tampering(O):
     poly = O.splitmix(1, 1 << 16, 32)
     pr = bytearray(O.prove(poly, 16, 32))
