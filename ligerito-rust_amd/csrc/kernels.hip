@@ -44,20 +44,30 @@ static inline unsigned grid_for(size_t work, unsigned block, unsigned cap = 6553
         asm volatile("" ::: "memory");       \
     } while (0)
 
-// Writes the 16 entries of nibble position k (k = 0..7) of the
-// multiply-by-lam table at t (128 words, entry v at word 16k + v):
-// entry v = XOR over the bits b of v of c_b = lam * x^(4k+b).
-__device__ __forceinline__ void gf32_tab_build(uint32_t* t, uint32_t lam, int k) {
-    const uint32_t c0 = gf32_reduce64((uint64_t)lam << (4 * k));
-    const uint32_t c1 = (c0 << 1) ^ ((0u - (c0 >> 31)) & GF32_POLY);
-    const uint32_t c2 = (c1 << 1) ^ ((0u - (c1 >> 31)) & GF32_POLY);
-    const uint32_t c3 = (c2 << 1) ^ ((0u - (c2 >> 31)) & GF32_POLY);
+// Writes the 16 entries of nibble position k (k = 0..7) of a GF(2^32)
+// table at t (128 words, entry v at word 16k + v) for a GF(2)-linear map L
+// given the images c_b = L(x^(4k+b)), b = 0..3: entry v = XOR of c_b over
+// the bits b of v.
+__device__ __forceinline__ void gf32_tab_write(uint32_t* t, int k, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3) {
     const uint32_t c01 = c0 ^ c1, c23 = c2 ^ c3;
     uint4* d4 = reinterpret_cast<uint4*>(t + k * 16);
     d4[0] = make_uint4(0u, c0, c1, c01);
     d4[1] = make_uint4(c2, c2 ^ c0, c2 ^ c1, c2 ^ c01);
     d4[2] = make_uint4(c3, c3 ^ c0, c3 ^ c1, c3 ^ c01);
     d4[3] = make_uint4(c23, c23 ^ c0, c23 ^ c1, c23 ^ c01);
+}
+__device__ __forceinline__ uint32_t gf32_dbl(uint32_t c) { return (c << 1) ^ ((0u - (c >> 31)) & GF32_POLY); }
+// nibble position k of the multiply-by-lam table: c_b = lam * x^(4k+b)
+__device__ __forceinline__ void gf32_tab_build(uint32_t* t, uint32_t lam, int k) {
+    const uint32_t c0 = gf32_reduce64((uint64_t)lam << (4 * k));
+    const uint32_t c1 = gf32_dbl(c0), c2 = gf32_dbl(c1), c3 = gf32_dbl(c2);
+    gf32_tab_write(t, k, c0, c1, c2, c3);
+}
+// nibble position k of the squaring table: c_b = x^(8k+2b)
+__device__ __forceinline__ void gf32_tab_build_sqr(uint32_t* t, int k) {
+    const uint32_t c0 = gf32_reduce64(1ull << (8 * k)), c1 = gf32_reduce64(1ull << (8 * k + 2));
+    const uint32_t c2 = gf32_reduce64(1ull << (8 * k + 4)), c3 = gf32_reduce64(1ull << (8 * k + 6));
+    gf32_tab_write(t, k, c0, c1, c2, c3);
 }
 
 // ------------------------------------------------------------------ tables --
@@ -1135,6 +1145,53 @@ __global__ __launch_bounds__(1024) void k_merkle_subtree4(uint32_t* __restrict__
     }
 }
 
+// Three levels per launch for wide layers: thread t turns nodes 8t..8t+7 of
+// the layer at in_off into their 4 parents, 2 grandparents and 1
+// great-grandparent (7 compressions, no idle lanes, one launch instead of
+// three).  n_out = number of great-grandparents = n_in / 8.
+__global__ __launch_bounds__(256) void k_merkle_block3(uint32_t* __restrict__ tree, size_t in_off, size_t n_in) {
+    const size_t n_out = n_in / 8;
+    const size_t l1 = in_off + n_in, l2 = l1 + n_in / 2, l3 = l2 + n_in / 4;  // first node of each layer
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n_out; t += (size_t)gridDim.x * blockDim.x) {
+        uint32_t c[64];  // 8 children, then reused for the upper nodes
+        const uint4* src = reinterpret_cast<const uint4*>(tree + (in_off + 8 * t) * 8);
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint4 v = src[k];
+            c[4 * k] = v.x;
+            c[4 * k + 1] = v.y;
+            c[4 * k + 2] = v.z;
+            c[4 * k + 3] = v.w;
+        }
+        // level 1: parent p of children 2p, 2p+1 (words 16p..16p+15) into words 8p..8p+7
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            uint32_t out[8];
+            b3::hash64(c + 16 * p, out);
+#pragma unroll
+            for (int w = 0; w < 8; w++) c[8 * p + w] = out[w];
+            uint4* d = reinterpret_cast<uint4*>(tree + (l1 + 4 * t + p) * 8);
+            d[0] = make_uint4(out[0], out[1], out[2], out[3]);
+            d[1] = make_uint4(out[4], out[5], out[6], out[7]);
+        }
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            uint32_t out[8];
+            b3::hash64(c + 16 * p, out);
+#pragma unroll
+            for (int w = 0; w < 8; w++) c[8 * p + w] = out[w];
+            uint4* d = reinterpret_cast<uint4*>(tree + (l2 + 2 * t + p) * 8);
+            d[0] = make_uint4(out[0], out[1], out[2], out[3]);
+            d[1] = make_uint4(out[4], out[5], out[6], out[7]);
+        }
+        uint32_t out[8];
+        b3::hash64(c, out);
+        uint4* d = reinterpret_cast<uint4*>(tree + (l3 + t) * 8);
+        d[0] = make_uint4(out[0], out[1], out[2], out[3]);
+        d[1] = make_uint4(out[4], out[5], out[6], out[7]);
+    }
+}
+
 // Wide levels one launch each, one lane per parent (throughput-bound);
 // once a level has fewer than MERKLE_WIDE parents, k_merkle_subtree4
 // finishes the top with four lanes per parent (latency-bound).
@@ -1145,6 +1202,11 @@ constexpr int SUB4_G = 7, SUB4_TOP = 10;
 
 void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out) {
     size_t in_off = 0, nodes = n_leaves;
+    while (nodes / 8 >= MERKLE_WIDE) {
+        hipLaunchKernelGGL(k_merkle_block3, dim3(grid_for(nodes / 8, 256)), dim3(256), 0, s, tree, in_off, nodes);
+        in_off += nodes + nodes / 2 + nodes / 4;
+        nodes /= 8;
+    }
     while (nodes / 2 >= MERKLE_WIDE) {
         const size_t n_out = nodes / 2;
         hipLaunchKernelGGL(k_merkle_level, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, s, tree, in_off, n_out);
@@ -1455,30 +1517,24 @@ __global__ __launch_bounds__(256) void k_induce_prep32(const uint64_t* __restric
     // table 0: squaring; 1..ell-1: times sks[t-1]; ell..ell+7: times alpha^(2^k)
     __shared__ __attribute__((aligned(256))) uint32_t tabs[(PREP_MAXELL + 8) * 128];
     __shared__ uint32_t apow[8];
+    // squaring and sks tables, one nibble position per work item
+    for (int idx = threadIdx.x; idx < ell * 8; idx += blockDim.x) {
+        const int tbl = idx >> 3, k = idx & 7;
+        if (tbl == 0)
+            gf32_tab_build_sqr(tabs, k);
+        else
+            gf32_tab_build(tabs + tbl * 128, sks[tbl - 1], k);
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t a = alpha[0];
         for (int k = 0; k < 8; k++) {
             apow[k] = a;
-            a = gf32_reduce64(clmul32_portable(a, a));
+            a = mul_tab_lds(tabs, 0u, a);  // squaring
         }
     }
     __syncthreads();
-    const int ntab = ell + 8;
-    for (int idx = threadIdx.x; idx < ntab * 128; idx += blockDim.x) {
-        const int tbl = idx >> 7, e = idx & 127;
-        uint32_t val;
-        if (tbl == 0) {
-            // squaring table in the gf32_tab_word layout
-            const int k = e >> 4;
-            const uint32_t y = (uint32_t)(e & 15) << (4 * k);
-            val = gf32_reduce64(clmul32_portable(y, y));
-        } else if (tbl < ell) {
-            val = gf32_tab_word(sks[tbl - 1], e);
-        } else {
-            val = gf32_tab_word(apow[tbl - ell], e);
-        }
-        tabs[idx] = val;
-    }
+    for (int idx = threadIdx.x; idx < 64; idx += blockDim.x) gf32_tab_build(tabs + (ell + (idx >> 3)) * 128, apow[idx >> 3], idx & 7);
     __syncthreads();
     for (int i = threadIdx.x; i < nq; i += blockDim.x) {
         uint32_t x = (uint32_t)q[i];
@@ -1711,7 +1767,10 @@ __global__ __launch_bounds__(64) void k_fs_step(DevTranscript* __restrict__ st, 
     using T = typename F::T;
     using A = typename F::Acc;
     dfs::State s;
-    s.load(use_init ? &init : st);
+    if (use_init)
+        s.load_value(init);
+    else
+        s.load(st);
     uint32_t err = 0;
     if (res) {
         T t[3];

@@ -17,7 +17,13 @@
 namespace lig {
 namespace dfs {
 
-__device__ __forceinline__ uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+__device__ __forceinline__ uint32_t rotr(uint32_t x, int n) { return __builtin_amdgcn_alignbit(x, x, n); }
+// a ^ b ^ c in one instruction (v_bitop3_b32, truth table 0x96)
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96); }
+__device__ __forceinline__ uint32_t big_s0(uint32_t a) { return xor3(rotr(a, 2), rotr(a, 13), rotr(a, 22)); }
+__device__ __forceinline__ uint32_t big_s1(uint32_t e) { return xor3(rotr(e, 6), rotr(e, 11), rotr(e, 25)); }
+__device__ __forceinline__ uint32_t small_s0(uint32_t w) { return xor3(rotr(w, 7), rotr(w, 18), w >> 3); }
+__device__ __forceinline__ uint32_t small_s1(uint32_t w) { return xor3(rotr(w, 17), rotr(w, 19), w >> 10); }
 
 // Values every lane loaded from one address, moved to scalar registers so
 // the transcript arithmetic runs on the scalar unit.
@@ -50,13 +56,11 @@ __device__ __forceinline__ void compress(uint32_t h[8], const uint32_t w_in[16])
         if (i < 16) {
             wi = w[i];
         } else {
-            const uint32_t w15 = w[(i - 15) & 15], w2 = w[(i - 2) & 15];
-            wi = w[i & 15] + (rotr(w15, 7) ^ rotr(w15, 18) ^ (w15 >> 3)) + w[(i - 7) & 15] +
-                 (rotr(w2, 17) ^ rotr(w2, 19) ^ (w2 >> 10));
+            wi = w[i & 15] + small_s0(w[(i - 15) & 15]) + w[(i - 7) & 15] + small_s1(w[(i - 2) & 15]);
             w[i & 15] = wi;
         }
-        const uint32_t t1 = hh + (rotr(e, 6) ^ rotr(e, 11) ^ rotr(e, 25)) + ((e & f) ^ (~e & g)) + K[i] + wi;
-        const uint32_t t2 = (rotr(a, 2) ^ rotr(a, 13) ^ rotr(a, 22)) + ((a & b) ^ (a & c) ^ (b & c));
+        const uint32_t t1 = hh + big_s1(e) + ((e & f) ^ (~e & g)) + K[i] + wi;
+        const uint32_t t2 = big_s0(a) + ((a & b) ^ (a & c) ^ (b & c));
         hh = g;
         g = f;
         f = e;
@@ -84,6 +88,16 @@ struct State {
     uint64_t total;
     uint32_t counter;
 
+    // from a state held in kernel arguments (member reads, no address taken)
+    __device__ void load_value(const DevTranscript& v) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) h[i] = v.h[i];
+#pragma unroll
+        for (int i = 0; i < 16; i++) buf[i] = v.buf[i];
+        blen = v.blen;
+        total = (uint64_t)v.total_lo | ((uint64_t)v.total_hi << 32);
+        counter = v.counter;
+    }
     __device__ void load(const DevTranscript* s) {
 #pragma unroll
         for (int i = 0; i < 8; i++) h[i] = uniform(s->h[i]);
@@ -109,9 +123,11 @@ struct State {
     // 32-bit words, so the pending length stays a multiple of 4.
     __device__ void word_le(uint32_t v) {
         const uint32_t be = __builtin_bswap32(v);
+        // words past the pending length are zero: OR the word in with a
+        // per-word mask (a conditional store here becomes an indexed store,
+        // which puts buf in scratch memory)
 #pragma unroll
-        for (int i = 0; i < 16; i++)
-            if ((uint32_t)i == (blen >> 2)) buf[i] = be;
+        for (int i = 0; i < 16; i++) buf[i] |= be & (0u - (uint32_t)((uint32_t)i == (blen >> 2)));
         blen += 4;
         total += 4;
         if (blen == 64) {
@@ -133,8 +149,7 @@ struct State {
         for (int i = 0; i < 16; i++) bc[i] = buf[i];
         // padding: 0x80, zeros, 64-bit big-endian bit length
 #pragma unroll
-        for (int i = 0; i < 16; i++)
-            if ((uint32_t)i == (blen >> 2)) bc[i] = 0x80000000u;
+        for (int i = 0; i < 16; i++) bc[i] |= 0x80000000u & (0u - (uint32_t)((uint32_t)i == (blen >> 2)));
         if (blen >= 56) {
             compress(hc, bc);
 #pragma unroll
