@@ -248,7 +248,9 @@ namespace {
 // to pinned host memory are in flight on the context stream.
 template <class H> struct PendingOpen {
     Rlp<H>* out = nullptr;
-    PBuf<uint64_t> d_q;
+    // query then Merkle node indices in pinned memory, read by the kernels
+    // through the mapping (valid until the stream is next synchronized)
+    const uint64_t* d_q = nullptr;
     PBuf<typename H::T> d_rows;
     size_t n_nodes = 0;
     uint8_t* h_rows = nullptr;
@@ -270,8 +272,7 @@ void open_launch(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, Pendi
     uint64_t* h_idx = (uint64_t*)(pin + pin_off);
     for (size_t i = 0; i < q.size(); i++) h_idx[i] = q[i];
     memcpy(h_idx + q.size(), nodes.data(), nodes.size() * 8);
-    po.d_q = PBuf<uint64_t>(&ctx->pool, q.size() + std::max<size_t>(nodes.size(), 1));
-    LIG_HIP(hipMemcpyAsync(po.d_q.p, h_idx, idx_bytes, hipMemcpyHostToDevice, s));
+    po.d_q = reinterpret_cast<const uint64_t*>(ctx->pin_dev + pin_off);
     po.d_rows = PBuf<T>(&ctx->pool, q.size() * w.n);
     po.h_rows = pin + pin_off + idx_bytes;
     po.h_nodes = po.h_rows + rows_bytes;
@@ -279,8 +280,8 @@ void open_launch(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, Pendi
     uint8_t* dev_rows = ctx->pin_dev + pin_off + idx_bytes;
     {
         PhaseTimer pt(ctx, PH_OPEN);
-        launch_gather_rows<F>(w.mat.p, w.n, po.d_q.p, q.size(), po.d_rows.p, (T*)dev_rows, s);
-        launch_gather_nodes(w.tree.p, po.d_q.p + q.size(), nodes.size(), (uint32_t*)(dev_rows + rows_bytes), s);
+        launch_gather_rows<F>(w.mat.p, w.n, po.d_q, q.size(), po.d_rows.p, (T*)dev_rows, s);
+        launch_gather_nodes(w.tree.p, po.d_q + q.size(), nodes.size(), (uint32_t*)(dev_rows + rows_bytes), s);
     }
     out.n_rows = q.size();
     out.n_cols = w.n;
@@ -320,7 +321,8 @@ void induce_launch(Ctx* ctx, int ell, const PendingOpen<H>& po, size_t n_cols, c
     const size_t nq = q.size();
     LIG_REQUIRE(gr.size() == n_cols, LIG_E_ASSERT, "opened row length != 2^k");
     Vec<H> sks = eval_sk_at_vks<H>((size_t)1 << ell);
-    // one pinned upload: sks | gr | alpha
+    // sks | gr | alpha in pinned memory, read by the kernels through the
+    // mapping (valid until the stream is next synchronized)
     const size_t nvals = sks.size() + gr.size() + 1;
     LIG_REQUIRE(nvals * sizeof(T) <= 16384 - 256, LIG_E_INVALID, "induce constants exceed the staging region");
     T* h = (T*)ctx->pinned(pin_off + nvals * sizeof(T));
@@ -328,15 +330,15 @@ void induce_launch(Ctx* ctx, int ell, const PendingOpen<H>& po, size_t n_cols, c
     memcpy(h, sks.data(), sks.size() * sizeof(T));
     memcpy(h + sks.size(), gr.data(), gr.size() * sizeof(T));
     h[nvals - 1] = alpha;
+    const T* d_vals = (const T*)(ctx->pin_dev + pin_off);
     const int clo = induce_low_bits(ell);
-    PBuf<T> d_vals(&ctx->pool, nvals), d_c(&ctx->pool, nq * ell), d_ap(&ctx->pool, nq);
+    PBuf<T> d_c(&ctx->pool, nq * ell), d_ap(&ctx->pool, nq);
     PBuf<T> d_P(&ctx->pool, nq << clo), d_K(&ctx->pool, nq << (ell - clo));
-    LIG_HIP(hipMemcpyAsync(d_vals.p, h, nvals * sizeof(T), hipMemcpyHostToDevice, s));
     d_basis = PBuf<T>(&ctx->pool, (size_t)1 << ell);
     {
         PhaseTimer pt(ctx, PH_INDUCE);
-        launch_induce_prep<F>(po.d_q.p, d_vals.p, d_vals.p + nvals - 1, (int)nq, ell, d_c.p, d_ap.p, s);
-        launch_enforced<F>(po.d_rows.p, (int)nq, (int)n_cols, d_vals.p + sks.size(), d_ap.p, d_enf, s);
+        launch_induce_prep<F>(po.d_q, d_vals, d_vals + nvals - 1, (int)nq, ell, d_c.p, d_ap.p, s);
+        launch_enforced<F>(po.d_rows.p, (int)nq, (int)n_cols, d_vals + sks.size(), d_ap.p, d_enf, s);
         *n_enf = enforced_parts((int)nq, (int)n_cols);
         launch_induce<F>(d_c.p, d_ap.p, (int)nq, ell, d_P.p, d_K.p, d_basis.p, s);
     }
@@ -499,7 +501,10 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         if (i == cfg.recursive_steps - 1) {
             // final step (prover.rs:104-142): yr, queries, openings
             T* hy = (T*)(ctx->pin + PIN_YR);
-            LIG_HIP(hipMemcpyAsync(hy, d_f.p, flen * sizeof(T), hipMemcpyDeviceToHost, s));
+            LIG_REQUIRE(PIN_YR + flen * sizeof(T) <= ((size_t)4 << 20), LIG_E_INVALID, "yr exceeds the staging region");
+            // written by a kernel straight into the mapped pinned buffer
+            launch_copy_words((uint32_t*)(ctx->pin_dev + PIN_YR), (const uint32_t*)d_f.p, flen * (sizeof(T) / 4), s);
+            LIG_HIP(hipGetLastError());
             ctx->sync();
             replay_section("sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
             open_finish<H>(*pending);

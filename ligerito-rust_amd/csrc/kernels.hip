@@ -1229,6 +1229,15 @@ void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32
     }
 }
 
+__global__ void k_copy_words(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+void launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_copy_words, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, dst, src, n);
+}
+
 __global__ void k_gather_nodes(const uint32_t* __restrict__ tree, const uint64_t* __restrict__ idx, size_t count,
                                uint32_t* __restrict__ out) {
     size_t total = count * 8;

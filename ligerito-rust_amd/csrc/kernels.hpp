@@ -94,6 +94,9 @@ void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32
 // out[i] = node at byte offset offs[i] of the tree (32 bytes each).
 void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t count, uint32_t* out, hipStream_t s);
 
+// dst[i] = src[i] for i < n words (dst may be mapped pinned host memory).
+void launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s);
+
 // ------------------------------------------------------------ folds -------
 // out[i] = sum_j gr[j] * poly[j*m + i], j < 2^kk (partial_eval over the top
 // kk variables, multilinear_poly.rs:66-87).  gr_tabs: 2^kk tables.
