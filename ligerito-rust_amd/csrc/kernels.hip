@@ -774,29 +774,28 @@ static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t 
     const int npass = (ell + KMAX - 1) / KMAX;
     const size_t nchunks = (n + 63) / 64;
     const int cc = n < 64 ? (int)n : 64;
-    // Large quarters run one at a time (a quarter stays cache-resident
-    // between its passes); small ones run all four per launch.
-    const bool by_quarter = m * n * 4 >= ((size_t)16 << 20);
-    for (int qi = 0; qi < (by_quarter ? 4 : 1); qi++) {
-        int d0 = 0;
-        for (int p = 0; p < npass; p++) {
-            const int K = ell / npass + (p < ell % npass ? 1 : 0);
-            const int K2 = K / 2;
-            const size_t tiles = (size_t)1 << (ell - K);
-            const size_t S = (size_t)1 << (ell - d0 - K);
-            // tiles sharing B (hence tables) are grouped per workgroup
-            size_t jobs = tiles * nchunks;
-            int tpw = 1;
-            while (tpw < 4 && (size_t)(tpw * 2) <= S * nchunks && jobs / (tpw * 2) >= 512) tpw *= 2;
-            const uint32_t* src = (p == 0 && msg) ? msg : mat;
-            const int shared = (p == 0 && msg) ? 1 : 0;
-            dim3 grid((unsigned)(jobs / tpw), by_quarter ? 1 : 4);
-            const size_t elems = m * n * (by_quarter ? 1 : 4);
-            if (obs) obs->before(K, elems);
-            launch_pass32(K, grid, cc << K2, s, src, mat, m, n, ell, d0, by_quarter ? qi : -1, shared, tpw, tw);
-            if (obs) obs->after(K, elems);
-            d0 += K;
-        }
+    // All four quarters per launch (grid.y): one quarter per launch leaves a
+    // partly filled last round of workgroups at the end of every launch,
+    // which costs more than the quarter's cache residency between passes
+    // saves (measured: 495 us against 534 us for the 2^18 x 64 encode).
+    int d0 = 0;
+    for (int p = 0; p < npass; p++) {
+        const int K = ell / npass + (p < ell % npass ? 1 : 0);
+        const int K2 = K / 2;
+        const size_t tiles = (size_t)1 << (ell - K);
+        const size_t S = (size_t)1 << (ell - d0 - K);
+        // tiles sharing B (hence tables) are grouped per workgroup
+        size_t jobs = tiles * nchunks;
+        int tpw = 1;
+        while (tpw < 4 && (size_t)(tpw * 2) <= S * nchunks && jobs / (tpw * 2) >= 512) tpw *= 2;
+        const uint32_t* src = (p == 0 && msg) ? msg : mat;
+        const int shared = (p == 0 && msg) ? 1 : 0;
+        dim3 grid((unsigned)(jobs / tpw), 4);
+        const size_t elems = m * n * 4;
+        if (obs) obs->before(K, elems);
+        launch_pass32(K, grid, cc << K2, s, src, mat, m, n, ell, d0, -1, shared, tpw, tw);
+        if (obs) obs->after(K, elems);
+        d0 += K;
     }
 }
 
