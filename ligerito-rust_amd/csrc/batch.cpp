@@ -22,6 +22,7 @@ template <class H> Worker<H>::Worker(int dev) {
     ctx.device = dev;
     LIG_HIP(hipSetDevice(dev));
     LIG_HIP(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
+    ctx.spin_wait = false;  // many workers share the host cores
 }
 
 template <class H> Worker<H>::~Worker() {

@@ -245,46 +245,69 @@ namespace {
 
 
 // Openings (prover.rs:72-76 / 114-125, merkle_tree.rs:98-111) whose copies
-// to pinned host memory are in flight on the context stream.
+// to pinned host memory are in flight on the context stream.  Pinned layout
+// from the region's offset: query indices | opened rows | Merkle node
+// indices | Merkle nodes, the node regions sized for the most nodes the
+// queries can need (one per query and tree level).
 template <class H> struct PendingOpen {
     Rlp<H>* out = nullptr;
-    // query then Merkle node indices in pinned memory, read by the kernels
-    // through the mapping (valid until the stream is next synchronized)
+    // query indices in pinned memory, read by the kernels through the
+    // mapping (valid until the stream is next synchronized)
     const uint64_t* d_q = nullptr;
     PBuf<typename H::T> d_rows;
-    size_t n_nodes = 0;
+    size_t n_nodes = 0, max_nodes = 0, nodes_off = 0;
     uint8_t* h_rows = nullptr;
     uint8_t* h_nodes = nullptr;
 };
 
+constexpr size_t pin_round(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// The opened rows: query indices to pinned memory, the row gather launched
+// (its output feeds induce).  open_launch_nodes adds the Merkle proof.
 template <class H>
-void open_launch(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, PendingOpen<H>& po, size_t pin_off) {
+void open_launch_rows(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, PendingOpen<H>& po, size_t pin_off) {
     using F = typename DevField<H>::F;
     using T = typename H::T;
     Ctx* ctx = w.ctx;
     hipStream_t s = ctx->stream;
-    std::vector<uint64_t> nodes = merkle_proof_nodes(w.M, q);
     po.out = &out;
-    po.n_nodes = nodes.size();
-    const size_t rows_bytes = q.size() * w.n * sizeof(T), nodes_bytes = nodes.size() * 32;
-    const size_t idx_bytes = (q.size() + nodes.size()) * 8;
-    uint8_t* pin = ctx->pinned(pin_off + idx_bytes + rows_bytes + nodes_bytes + 256);
+    const size_t q_bytes = pin_round(q.size() * 8), rows_bytes = pin_round(q.size() * w.n * sizeof(T));
+    po.max_nodes = q.size() * (size_t)ilog2(w.M);
+    po.nodes_off = pin_off + q_bytes + rows_bytes;
+    uint8_t* pin = ctx->pinned(po.nodes_off + pin_round(po.max_nodes * 8) + po.max_nodes * 32);
     uint64_t* h_idx = (uint64_t*)(pin + pin_off);
     for (size_t i = 0; i < q.size(); i++) h_idx[i] = q[i];
-    memcpy(h_idx + q.size(), nodes.data(), nodes.size() * 8);
     po.d_q = reinterpret_cast<const uint64_t*>(ctx->pin_dev + pin_off);
     po.d_rows = PBuf<T>(&ctx->pool, q.size() * w.n);
-    po.h_rows = pin + pin_off + idx_bytes;
-    po.h_nodes = po.h_rows + rows_bytes;
-    // the gathers write the proof's rows and nodes straight to pinned memory
-    uint8_t* dev_rows = ctx->pin_dev + pin_off + idx_bytes;
+    po.h_rows = pin + pin_off + q_bytes;
     {
+        // the gather writes the proof's rows straight to pinned memory too
         PhaseTimer pt(ctx, PH_OPEN);
-        launch_gather_rows<F>(w.mat.p, w.n, po.d_q, q.size(), po.d_rows.p, (T*)dev_rows, s);
-        launch_gather_nodes(w.tree.p, po.d_q + q.size(), nodes.size(), (uint32_t*)(dev_rows + rows_bytes), s);
+        launch_gather_rows<F>(w.mat.p, w.n, po.d_q, q.size(), po.d_rows.p, (T*)(ctx->pin_dev + pin_off + q_bytes), s);
     }
     out.n_rows = q.size();
     out.n_cols = w.n;
+}
+
+// The Merkle proof of the opened rows (merkle_tree.rs:98-111): node indices
+// on the host, the gather of the nodes into pinned memory on the stream.
+template <class H> void open_launch_nodes(Witness<H>& w, const std::vector<size_t>& q, PendingOpen<H>& po) {
+    Ctx* ctx = w.ctx;
+    std::vector<uint64_t> nodes = merkle_proof_nodes(w.M, q);
+    LIG_REQUIRE(nodes.size() <= po.max_nodes, LIG_E_ASSERT, "Merkle proof needs more nodes than queries x depth");
+    po.n_nodes = nodes.size();
+    const size_t nodes_at = po.nodes_off + pin_round(po.max_nodes * 8);
+    memcpy(ctx->pin + po.nodes_off, nodes.data(), nodes.size() * 8);
+    po.h_nodes = ctx->pin + nodes_at;
+    PhaseTimer pt(ctx, PH_OPEN);
+    launch_gather_nodes(w.tree.p, reinterpret_cast<const uint64_t*>(ctx->pin_dev + po.nodes_off), nodes.size(),
+                        (uint32_t*)(ctx->pin_dev + nodes_at), ctx->stream);
+}
+
+template <class H>
+void open_launch(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, PendingOpen<H>& po, size_t pin_off) {
+    open_launch_rows<H>(w, q, out, po, pin_off);
+    open_launch_nodes<H>(w, q, po);
 }
 
 // Call after the stream has been synchronized.
@@ -484,8 +507,10 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         }
     };
 
+    // the GPU starts on the rows, induce and the sumcheck section before the
+    // host lists the Merkle proof nodes
     std::unique_ptr<PendingOpen<H>> pending(new PendingOpen<H>());
-    open_launch<H>(w0, q, proof.initial, *pending, PIN_OPEN);
+    open_launch_rows<H>(w0, q, proof.initial, *pending, PIN_OPEN);
     induce_launch<H>(ctx, nv, *pending, w0.n, gr0, q, alpha, d_b, d_enf, &n_enf, PIN_UP);
     {
         PhaseTimer pt(ctx, PH_SUMCHECK);
@@ -495,6 +520,7 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     // SumcheckProverInstance::new (ligerito.rs:32-41) and the rounds of the
     // first step (prover.rs:96-102)
     launch_section(1, cfg.ks[0], nullptr);
+    open_launch_nodes<H>(w0, q, *pending);
     w0 = Witness<H>();  // the first witness is released once its kernels ran
 
     for (int i = 0; i < cfg.recursive_steps; i++) {
@@ -531,7 +557,7 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         proof.rec_proofs.emplace_back();
         PBuf<T> d_bnew;
         pending.reset(new PendingOpen<H>());
-        open_launch<H>(wprev, q, proof.rec_proofs.back(), *pending, PIN_OPEN);
+        open_launch_rows<H>(wprev, q, proof.rec_proofs.back(), *pending, PIN_OPEN);
         Vec<H> gr = lagrange_basis<H>(fold_rs);
         induce_launch<H>(ctx, nv, *pending, wprev.n, gr, q, a, d_bnew, d_enf, &n_enf, PIN_UP);
         // introduce_new (ligerito.rs:50-57): triple of f with the new basis
@@ -544,6 +570,7 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             std::swap(res_cur, res_next);
         }
         launch_section(2, cfg.ks[i + 1], &d_bnew);
+        open_launch_nodes<H>(wprev, q, *pending);
         wprev = std::move(wnext);
     }
     throw Error(LIG_E_ASSERT, "Prover completed without entering final step (prover.rs:198)");

@@ -60,7 +60,23 @@ struct Ctx {
         }
         return pin;
     }
-    void sync() { LIG_HIP(hipStreamSynchronize(stream)); }
+    // Waits for the stream.  With spin_wait the host polls the stream (a
+    // blocking wait wakes the thread several microseconds after the last
+    // kernel ends, which the prover's host steps then add to every sync);
+    // contexts that share the host with many others (batch and serve
+    // workers) block instead.
+    bool spin_wait = true;
+    void sync() {
+        if (spin_wait) {
+            for (;;) {
+                const hipError_t e = hipStreamQuery(stream);
+                if (e == hipSuccess) return;
+                if (e != hipErrorNotReady) LIG_HIP(e);
+                __builtin_ia32_pause();
+            }
+        }
+        LIG_HIP(hipStreamSynchronize(stream));
+    }
     // Resolves recorded event pairs into per-phase totals.
     void collect();
     ~Ctx() {

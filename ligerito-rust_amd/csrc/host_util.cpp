@@ -88,6 +88,63 @@ __attribute__((target("sha,sse4.1,ssse3"))) void sha_blocks_ni(uint32_t st[8], c
     _mm_storeu_si128((__m128i*)&st[4], s1);
 }
 
+// Two independent one-block compressions interleaved instruction by
+// instruction: a single chain of sha256rnds2 waits on each round's latency,
+// two chains keep the unit busy.
+__attribute__((target("sha,sse4.1,ssse3"))) void sha_ni_x2(uint32_t sa[8], const uint8_t* pa, uint32_t sb[8],
+                                                           const uint8_t* pb) {
+    const __m128i MASK = _mm_set_epi64x(0x0c0d0e0f08090a0bULL, 0x0405060700010203ULL);
+    __m128i a0, a1, b0, b1;
+    {
+        __m128i t = _mm_shuffle_epi32(_mm_loadu_si128((const __m128i*)&sa[0]), 0xB1);
+        a1 = _mm_shuffle_epi32(_mm_loadu_si128((const __m128i*)&sa[4]), 0x1B);
+        a0 = _mm_alignr_epi8(t, a1, 8);
+        a1 = _mm_blend_epi16(a1, t, 0xF0);
+        t = _mm_shuffle_epi32(_mm_loadu_si128((const __m128i*)&sb[0]), 0xB1);
+        b1 = _mm_shuffle_epi32(_mm_loadu_si128((const __m128i*)&sb[4]), 0x1B);
+        b0 = _mm_alignr_epi8(t, b1, 8);
+        b1 = _mm_blend_epi16(b1, t, 0xF0);
+    }
+    const __m128i sa0 = a0, sa1 = a1, sb0 = b0, sb1 = b1;
+    __m128i wa[4], wb[4];
+    for (int g = 0; g < 16; g++) {
+        __m128i ga, gb;
+        if (g < 4) {
+            ga = _mm_shuffle_epi8(_mm_loadu_si128((const __m128i*)(pa + 16 * g)), MASK);
+            gb = _mm_shuffle_epi8(_mm_loadu_si128((const __m128i*)(pb + 16 * g)), MASK);
+        } else {
+            __m128i ta = _mm_sha256msg1_epu32(wa[(g - 4) & 3], wa[(g - 3) & 3]);
+            __m128i tb = _mm_sha256msg1_epu32(wb[(g - 4) & 3], wb[(g - 3) & 3]);
+            ta = _mm_add_epi32(ta, _mm_alignr_epi8(wa[(g - 1) & 3], wa[(g - 2) & 3], 4));
+            tb = _mm_add_epi32(tb, _mm_alignr_epi8(wb[(g - 1) & 3], wb[(g - 2) & 3], 4));
+            ga = _mm_sha256msg2_epu32(ta, wa[(g - 1) & 3]);
+            gb = _mm_sha256msg2_epu32(tb, wb[(g - 1) & 3]);
+        }
+        wa[g & 3] = ga;
+        wb[g & 3] = gb;
+        const __m128i k = _mm_loadu_si128((const __m128i*)&kSha[4 * g]);
+        __m128i ma = _mm_add_epi32(ga, k), mb = _mm_add_epi32(gb, k);
+        a1 = _mm_sha256rnds2_epu32(a1, a0, ma);
+        b1 = _mm_sha256rnds2_epu32(b1, b0, mb);
+        ma = _mm_shuffle_epi32(ma, 0x0E);
+        mb = _mm_shuffle_epi32(mb, 0x0E);
+        a0 = _mm_sha256rnds2_epu32(a0, a1, ma);
+        b0 = _mm_sha256rnds2_epu32(b0, b1, mb);
+    }
+    a0 = _mm_add_epi32(a0, sa0);
+    a1 = _mm_add_epi32(a1, sa1);
+    b0 = _mm_add_epi32(b0, sb0);
+    b1 = _mm_add_epi32(b1, sb1);
+    __m128i t = _mm_shuffle_epi32(a0, 0x1B);
+    a1 = _mm_shuffle_epi32(a1, 0xB1);
+    _mm_storeu_si128((__m128i*)&sa[0], _mm_blend_epi16(t, a1, 0xF0));
+    _mm_storeu_si128((__m128i*)&sa[4], _mm_alignr_epi8(a1, t, 8));
+    t = _mm_shuffle_epi32(b0, 0x1B);
+    b1 = _mm_shuffle_epi32(b1, 0xB1);
+    _mm_storeu_si128((__m128i*)&sb[0], _mm_blend_epi16(t, b1, 0xF0));
+    _mm_storeu_si128((__m128i*)&sb[4], _mm_alignr_epi8(b1, t, 8));
+}
+
 bool have_sha_ni() {
     static int cached = -1;
     if (cached < 0) {
@@ -267,24 +324,83 @@ bool have_avx2() {
 }
 }  // namespace
 
-// get_distinct_queries (emulated_fs.rs:61-75).  Without the SHA extensions,
-// squeezes are generated in groups of eight: the running state absorbs each
-// counter in turn, the finalizations that fit one block run eight at a time
-// (AVX2), the others one by one; afterwards the transcript is rewound to
-// exactly the squeezes the rejection sampling consumed.  With the SHA
-// extensions one squeeze at a time is faster.
+// get_distinct_queries (emulated_fs.rs:61-75).  Squeezes are generated in
+// groups: the running state absorbs each counter in turn and the
+// finalizations that fit one block run together (two interleaved with the
+// SHA extensions, eight at a time with AVX2 without them), the others one by
+// one; afterwards the transcript is rewound to exactly the squeezes the
+// rejection sampling consumed.
 std::vector<size_t> Transcript::get_distinct_queries(size_t n, size_t s) {
     LIG_REQUIRE(s <= n, LIG_E_INVALID, "get_distinct_queries: s > n never terminates");
     std::vector<size_t> q;
     q.reserve(s);
     std::vector<uint8_t> seen((n + 7) / 8, 0);
-    if (!have_avx2() || have_sha_ni()) {
+    auto take = [&](uint64_t lo) {
+        const size_t v = (size_t)(lo % (uint64_t)n);
+        if (!(seen[v >> 3] & (1u << (v & 7)))) {
+            seen[v >> 3] |= (uint8_t)(1u << (v & 7));
+            q.push_back(v);
+        }
+    };
+    if (have_sha_ni()) {
+        // Squeezes in pairs: the running state absorbs both counters, the two
+        // one-block finalizations run interleaved; if the first completes
+        // the set, the transcript is rewound to just after it.
         while (q.size() < s) {
-            size_t v = get_query(n);
-            if (!(seen[v >> 3] & (1u << (v & 7)))) {
-                seen[v >> 3] |= (uint8_t)(1u << (v & 7));
-                q.push_back(v);
+            uint64_t lo[2];
+            uint32_t st[2][8];
+            alignas(16) uint8_t blk[2][64];
+            bool one[2];
+            Sha256 h_mid;
+            uint32_t c_mid = 0;
+            for (int k = 0; k < 2; k++) {
+                if (k == 1) {
+                    h_mid = hasher;
+                    c_mid = counter;
+                }
+                uint8_t b[4];
+                memcpy(b, &counter, 4);
+                hasher.update(b, 4);
+                counter++;
+                const uint32_t nb = hasher.blen;
+                one[k] = nb < 56;
+                if (one[k]) {
+                    const uint64_t bits = hasher.total * 8;
+                    memcpy(st[k], hasher.h, 32);
+                    memcpy(blk[k], hasher.buf, nb);
+                    blk[k][nb] = 0x80;
+                    memset(blk[k] + nb + 1, 0, 56 - nb - 1);
+                    for (int i = 0; i < 8; i++) blk[k][56 + i] = (uint8_t)(bits >> (56 - 8 * i));
+                } else {
+                    uint8_t d[32];
+                    hasher.digest(d);
+                    memcpy(&lo[k], d, 8);
+                }
             }
+            if (one[0] && one[1]) {
+                sha_ni_x2(st[0], blk[0], st[1], blk[1]);
+            } else {
+                for (int k = 0; k < 2; k++)
+                    if (one[k]) sha_blocks_ni(st[k], blk[k], 1);
+            }
+            for (int k = 0; k < 2; k++)
+                if (one[k]) lo[k] = (uint64_t)__builtin_bswap32(st[k][0]) | ((uint64_t)__builtin_bswap32(st[k][1]) << 32);
+            take(lo[0]);
+            if (q.size() == s) {
+                hasher = h_mid;
+                counter = c_mid;
+                break;
+            }
+            take(lo[1]);
+        }
+        std::sort(q.begin(), q.end());
+        return q;
+    }
+    if (!have_avx2()) {
+        while (q.size() < s) {
+            uint64_t lo, hi;
+            squeeze(&lo, &hi);
+            take(lo);
         }
         std::sort(q.begin(), q.end());
         return q;

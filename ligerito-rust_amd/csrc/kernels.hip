@@ -492,26 +492,23 @@ static void fft_quarters128(const u128v* msg, u128v* mat, size_t m, size_t n, co
     const size_t nchunks = (n + 63) / 64;
     const int cc = n < 64 ? (int)n : 64;
     (void)cc;
-    const bool by_quarter = m * n * 16 >= ((size_t)16 << 20);
-    for (int qi = 0; qi < (by_quarter ? 4 : 1); qi++) {
-        int d0 = 0;
-        for (int p = 0; p < npass; p++) {
-            const int K = ell / npass + (p < ell % npass ? 1 : 0);
-            const size_t S = (size_t)1 << (ell - d0 - K);
-            const size_t jobs = ((size_t)1 << (ell - K)) * nchunks;
-            int J = 1;
-            while (J < 4 && (size_t)(J * 2) <= S * nchunks && jobs % (size_t)(J * 2) == 0) J *= 2;
-            const u128v* src = (p == 0 && msg) ? msg : mat;
-            const int shared = (p == 0 && msg) ? 1 : 0;
-            dim3 grid((unsigned)(jobs / J), by_quarter ? 1 : 4);
-            const int q = by_quarter ? qi : -1;
-            switch (K) {
-                case 1: hipLaunchKernelGGL(k_fft_pass128<1>, grid, dim3(64 * J), 0, s, src, mat, m, n, ell, d0, q, shared, tw); break;
-                case 2: hipLaunchKernelGGL(k_fft_pass128<2>, grid, dim3(64 * J), 0, s, src, mat, m, n, ell, d0, q, shared, tw); break;
-                default: hipLaunchKernelGGL(k_fft_pass128<3>, grid, dim3(64 * J), 0, s, src, mat, m, n, ell, d0, q, shared, tw); break;
-            }
-            d0 += K;
+    // all four quarters per launch (grid.y), as for GF(2^32)
+    int d0 = 0;
+    for (int p = 0; p < npass; p++) {
+        const int K = ell / npass + (p < ell % npass ? 1 : 0);
+        const size_t S = (size_t)1 << (ell - d0 - K);
+        const size_t jobs = ((size_t)1 << (ell - K)) * nchunks;
+        int J = 1;
+        while (J < 4 && (size_t)(J * 2) <= S * nchunks && jobs % (size_t)(J * 2) == 0) J *= 2;
+        const u128v* src = (p == 0 && msg) ? msg : mat;
+        const int shared = (p == 0 && msg) ? 1 : 0;
+        dim3 grid((unsigned)(jobs / J), 4);
+        switch (K) {
+            case 1: hipLaunchKernelGGL(k_fft_pass128<1>, grid, dim3(64 * J), 0, s, src, mat, m, n, ell, d0, -1, shared, tw); break;
+            case 2: hipLaunchKernelGGL(k_fft_pass128<2>, grid, dim3(64 * J), 0, s, src, mat, m, n, ell, d0, -1, shared, tw); break;
+            default: hipLaunchKernelGGL(k_fft_pass128<3>, grid, dim3(64 * J), 0, s, src, mat, m, n, ell, d0, -1, shared, tw); break;
         }
+        d0 += K;
     }
 }
 
@@ -1397,36 +1394,44 @@ __global__ __launch_bounds__(256) void k_induce_accum(const typename F::T* __res
     }
 }
 
-// GF(2^32) variant: all query tables of a workgroup are built once in LDS
-// (up to 160 queries), then each lane streams P over the queries.
-constexpr int INDUCE_QCH = 32;  // queries per workgroup
-__global__ void k_zero_u32(uint32_t* p, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
-}
-// GF(2^32) variant: a workgroup owns one X and a chunk of up to 32 queries;
-// their tables are built once in LDS and the partial sums are XORed into
-// basis atomically (XOR is order-independent).
-__global__ __launch_bounds__(256) void k_induce_accum32(const uint32_t* __restrict__ P, const uint32_t* __restrict__ Kt, int nq,
-                                                        int ell, int clo, uint32_t* __restrict__ basis) {
-    __shared__ __attribute__((aligned(256))) uint32_t tabs[INDUCE_QCH * 128];
-    __shared__ uint32_t kv[INDUCE_QCH];
+// GF(2^32) variant: a workgroup owns one X and holds the multiply tables of
+// up to ACC_QMAX queries in LDS (built per nibble position).  Lane
+// (s, x) with x < 2^clo accumulates basis entry x over the queries
+// i = s, s + QS, s + 2QS, ... in registers (QS = blockDim.x / 2^clo query
+// slices, so narrow X rows still fill the workgroup); the QS partial sums of
+// an entry are XORed through LDS and the entry is written once (no atomics,
+// no zeroing).  More queries than ACC_QMAX are taken in chunks.
+constexpr int ACC_QMAX = 160;
+__global__ __launch_bounds__(1024) void k_induce_accum32(const uint32_t* __restrict__ P, const uint32_t* __restrict__ Kt, int nq,
+                                                         int ell, int clo, uint32_t* __restrict__ basis) {
+    __shared__ __attribute__((aligned(256))) uint32_t tabs[ACC_QMAX * 128];
+    __shared__ uint32_t part[1024];
     const size_t np = (size_t)1 << clo, nk = (size_t)1 << (ell - clo);
     const size_t X = blockIdx.x;
-    const int q0 = blockIdx.y * INDUCE_QCH;
-    const int qn = nq - q0 < INDUCE_QCH ? nq - q0 : INDUCE_QCH;
-    if ((int)threadIdx.x < qn) kv[threadIdx.x] = Kt[(size_t)(q0 + threadIdx.x) * nk + X];
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < qn * 128; idx += blockDim.x) {
-        const int i = idx >> 7, e = idx & 127;
-        tabs[idx] = gf32_tab_word(kv[i], e);
-    }
-    __syncthreads();
-    for (size_t x = threadIdx.x; x < np; x += blockDim.x) {
-        uint32_t acc = 0;
+    const int qs = (int)(blockDim.x >> clo);  // query slices (blockDim.x = qs * 2^clo)
+    const int sl = (int)(threadIdx.x >> clo);
+    const size_t x = threadIdx.x & (np - 1);
+    uint32_t acc = 0;
+    for (int q0 = 0; q0 < nq; q0 += ACC_QMAX) {
+        const int qn = nq - q0 < ACC_QMAX ? nq - q0 : ACC_QMAX;
+        if (q0 > 0) LIG_LDS_BARRIER();  // the previous chunk's lookups are done
+        for (int idx = threadIdx.x; idx < qn * 8; idx += blockDim.x)
+            gf32_tab_build(tabs + (idx >> 3) * 128, Kt[(size_t)(q0 + (idx >> 3)) * nk + X], idx & 7);
+        LIG_LDS_BARRIER();
         const uint32_t* Pq = P + (size_t)q0 * np + x;
 #pragma unroll 8
-        for (int i = 0; i < qn; i++) acc ^= mul_tab_lds(tabs, (uint32_t)(i * 512), Pq[(size_t)i * np]);
-        atomicXor(&basis[X * np + x], acc);
+        for (int i = sl; i < qn; i += qs) acc ^= mul_tab_lds(tabs, (uint32_t)(i * 512), Pq[(size_t)i * np]);
+    }
+    if (qs == 1) {
+        basis[X * np + x] = acc;
+        return;
+    }
+    part[threadIdx.x] = acc;
+    LIG_LDS_BARRIER();
+    if (sl == 0) {
+        uint32_t t = acc;
+        for (int k = 1; k < qs; k++) t ^= part[k * np + x];
+        basis[X * np + x] = t;
     }
 }
 
@@ -1447,10 +1452,7 @@ __global__ __launch_bounds__(256) void k_induce_tensor32(const uint32_t* __restr
     __shared__ uint32_t kb[TENSOR_MAXK];
     const int i = blockIdx.x;
     const uint32_t* c = cvals + (size_t)i * ell;
-    for (int idx = threadIdx.x; idx < ell * 128; idx += blockDim.x) {
-        const int e = idx & 127;
-        tabs[idx] = gf32_tab_word(c[idx >> 7], e);
-    }
+    for (int idx = threadIdx.x; idx < ell * 8; idx += blockDim.x) gf32_tab_build(tabs + (idx >> 3) * 128, c[idx >> 3], idx & 7);
     if (threadIdx.x == 0) {
         pb[0] = 1u;
         kb[0] = ap[i];
@@ -1485,11 +1487,11 @@ void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, 
         hipLaunchKernelGGL(k_induce_tensor<F>, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K);
     }
     if constexpr (F::BITS == 32) {
-        size_t total = (size_t)1 << ell;
-        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(total, 256, 2048)), dim3(256), 0, s, basis, total);
-        dim3 grid((unsigned)nk, (unsigned)((nq + INDUCE_QCH - 1) / INDUCE_QCH));
-        hipLaunchKernelGGL(k_induce_accum32, grid, dim3(256), 0, s, P, K, nq, ell, clo, basis);
-        return;
+        // 1024 lanes per X row: 2^clo entries times 1024 / 2^clo query slices
+        if (clo >= 6 && clo <= 10) {
+            hipLaunchKernelGGL(k_induce_accum32, dim3((unsigned)nk), dim3(1024), 0, s, P, K, nq, ell, clo, basis);
+            return;
+        }
     }
     hipLaunchKernelGGL(k_induce_accum<F>, dim3((unsigned)nk), dim3(256), 0, s, P, K, nq, ell, clo, basis);
 }
