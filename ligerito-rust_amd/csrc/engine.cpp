@@ -436,17 +436,30 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     // from the identical state.
     int maxk = 0;
     for (int k : cfg.ks) maxk = std::max(maxk, k);
-    PBuf<DevTranscript> d_fs(&ctx->pool, 1);
+    // two transcript-state slots: a fold that runs the preceding step reads
+    // one and writes the other (its workgroups read while workgroup 0 writes)
+    PBuf<DevTranscript> d_fs(&ctx->pool, 2);
+    int fsi = 0;
     PBuf<T> d_tlog(&ctx->pool, 3 * (size_t)(maxk + 1)), d_r(&ctx->pool, (size_t)maxk + 2);
-    // two triple accumulators used alternately: each triple kernel adds into
-    // res_next and clears res_cur, then the names swap, so res_cur holds the
-    // latest triple and res_next is clear; plus the enforced sum
-    PBuf<A> d_acc(&ctx->pool, 6 + ENFORCED_MAX_PARTS);
+    // three triple accumulators in rotation: a triple kernel reads the latest
+    // triple (res_cur, as the transcript input of a fold step), adds into
+    // res_next (clear) and clears res_free; then the roles rotate, so
+    // res_cur again holds the latest triple and res_next is clear; plus the
+    // enforced sum
+    PBuf<A> d_acc(&ctx->pool, 9 + ENFORCED_MAX_PARTS);
     A* res_cur = d_acc.p;
     A* res_next = d_acc.p + 3;
-    A* d_enf = d_acc.p + 6;
+    A* res_free = d_acc.p + 6;
+    A* d_enf = d_acc.p + 9;
+    auto rotate = [&]() {
+        A* t = res_cur;
+        res_cur = res_next;
+        res_next = res_free;
+        res_free = t;
+    };
     int n_enf = 0;
     launch_zero_acc<F>(res_next, s);
+    launch_zero_acc<F>(res_free, s);
     PBuf<T> d_b, d_b2(&ctx->pool, flen / 2);
 
     struct Section {
@@ -461,23 +474,34 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     uint32_t* dev_err = (uint32_t*)(dev_rlog + n_r);
     // Launches: leading step (absorb the triple in res_cur, check it against
     // the enforced sum, squeeze n0), optional glue with the first squeeze,
-    // then k fold rounds, each followed by its transcript step.
+    // then k fold rounds; the transcript step after round j-1 (absorb its
+    // triple, squeeze round j's challenge) runs inside round j's fold, and a
+    // last step absorbs the final triple.
     auto launch_section = [&](int n0, int k, PBuf<T>* d_bnew) {
         DevTranscript init;
         export_transcript(fs, init);
         *(uint32_t*)(h_fsdn + (n_tlog + n_r) * sizeof(T)) = 0;
         PhaseTimer pt(ctx, PH_SUMCHECK);
-        launch_fs_step<F>(d_fs.p, &init, res_cur, d_enf, n_enf, dev_tlog, d_r.p, dev_rlog, n0, dev_err, s);
+        launch_fs_step<F>(d_fs.p + fsi, &init, res_cur, d_enf, n_enf, dev_tlog, d_r.p, dev_rlog, n0, dev_err, s);
         if (d_bnew) launch_glue<F>(d_b.p, d_bnew->p, flen, d_r.p, s);
         for (int j = 0; j < k; j++) {
-            launch_sumcheck_fold<F>(d_f.p, d_b.p, flen, d_r.p + (n0 - 1) + j, d_f2.p, d_b2.p, res_next, res_cur, s);
-            std::swap(res_cur, res_next);
-            launch_fs_step<F>(d_fs.p, nullptr, res_cur, nullptr, 0, dev_tlog + 3 * (j + 1), d_r.p + n0 + j,
-                              dev_rlog + n0 + j, j + 1 < k ? 1 : 0, dev_err, s);
+            if (j == 0) {
+                launch_sumcheck_fold<F>(d_f.p, d_b.p, flen, d_r.p + (n0 - 1), d_f2.p, d_b2.p, res_next, res_free, s);
+            } else {
+                // the step after round j-1 (absorb its triple, squeeze r) runs
+                // inside round j's fold
+                FoldStep<F> step{d_fs.p + fsi, d_fs.p + (fsi ^ 1), res_cur, dev_tlog + 3 * j, dev_rlog + n0 + (j - 1)};
+                launch_sumcheck_fold_step<F>(d_f.p, d_b.p, flen, step, d_f2.p, d_b2.p, res_next, res_free, s);
+                fsi ^= 1;
+            }
+            rotate();
             std::swap(d_f, d_f2);
             std::swap(d_b, d_b2);
             flen /= 2;
         }
+        // the last triple is absorbed with no challenge after it
+        launch_fs_step<F>(d_fs.p + fsi, nullptr, res_cur, nullptr, 0, dev_tlog + 3 * k, d_r.p + n0 + k, dev_rlog + n0 + k, 0,
+                          dev_err, s);
         LIG_HIP(hipGetLastError());
         sec.n0 = n0;
         sec.k = k;
@@ -514,8 +538,8 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     induce_launch<H>(ctx, nv, *pending, w0.n, gr0, q, alpha, d_b, d_enf, &n_enf, PIN_UP);
     {
         PhaseTimer pt(ctx, PH_SUMCHECK);
-        launch_eval013<F>(d_f.p, d_b.p, flen, res_next, res_cur, s);
-        std::swap(res_cur, res_next);
+        launch_eval013<F>(d_f.p, d_b.p, flen, res_next, res_free, s);
+        rotate();
     }
     // SumcheckProverInstance::new (ligerito.rs:32-41) and the rounds of the
     // first step (prover.rs:96-102)
@@ -566,8 +590,8 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         // are linear in the basis polys); then the rounds of step i+1
         {
             PhaseTimer pt(ctx, PH_SUMCHECK);
-            launch_eval013<F>(d_f.p, d_bnew.p, flen, res_next, res_cur, s);
-            std::swap(res_cur, res_next);
+            launch_eval013<F>(d_f.p, d_bnew.p, flen, res_next, res_free, s);
+            rotate();
         }
         launch_section(2, cfg.ks[i + 1], &d_bnew);
         open_launch_nodes<H>(wprev, q, *pending);

@@ -1688,9 +1688,9 @@ template <class F> void launch_zero_acc(typename F::Acc* res, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_acc<F>, dim3(1), dim3(64), 0, s, res);
 }
 
-template <class F>
+template <class F, bool STEP>
 __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __restrict__ f, const typename F::T* __restrict__ b,
-                                                       size_t len, const typename F::T* __restrict__ rp,
+                                                       size_t len, const typename F::T* __restrict__ rp, FoldStep<F> step,
                                                        typename F::T* __restrict__ f2, typename F::T* __restrict__ b2,
                                                        typename F::Acc* res, typename F::Acc* res_clear) {
     using T = typename F::T;
@@ -1698,14 +1698,45 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
     if (blockIdx.x == 0 && threadIdx.x < 3) res_clear[threadIdx.x] = A{};
     const size_t L2 = len / 2, h2 = L2 / 2;
     const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
-    // the first element group is loaded while the challenge's table is built
+    // the first element group is loaded while the challenge (and its table)
+    // is produced
     T fa{}, fb{}, fc{}, fd{}, ba{}, bb{}, bc{}, bd{};
     if (i0 < h2) {
         fa = f[i0], fb = f[i0 + L2], fc = f[i0 + h2], fd = f[i0 + h2 + L2];
         ba = b[i0], bb = b[i0 + L2], bc = b[i0 + h2], bd = b[i0 + h2 + L2];
     }
-    const T r = *rp;
     __shared__ T tab[F::TAB];
+    T r;
+    if constexpr (STEP) {
+        // the preceding transcript step, computed by wave 0 of every workgroup
+        __shared__ T r_sh;
+        if (threadIdx.x < 64) {
+            dfs::State st;
+            st.load(step.st_in);
+            T t[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) t[k] = Ops<F>::acc_reduce(dfs::uniform_acc(step.res_in[k]));
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int i = 0; i < Ops<F>::WORDS; i++) st.word_le(Ops<F>::word(t[k], i));
+            uint32_t o[4];
+            st.squeeze(o);
+            const T rv = Ops<F>::from_squeeze(o);
+            if (threadIdx.x == 0) {
+                r_sh = rv;
+                if (blockIdx.x == 0) {
+                    st.store(step.st_out);
+                    for (int k = 0; k < 3; k++) step.tlog[k] = t[k];
+                    step.rlog[0] = rv;
+                }
+            }
+        }
+        LIG_LDS_BARRIER();
+        r = r_sh;
+    } else {
+        r = *rp;
+    }
     for (int e = threadIdx.x; e < F::TAB; e += blockDim.x) tab[e] = Ops<F>::table_entry(r, e);
     LIG_LDS_BARRIER();
     A s0 = Ops<F>::acc_zero(), s1 = Ops<F>::acc_zero(), s2 = Ops<F>::acc_zero();
@@ -1744,8 +1775,19 @@ void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t
                           typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
                           hipStream_t s) {
     size_t h2 = len / 4;
-    hipLaunchKernelGGL(k_sumcheck_fold<F>, dim3(grid_for(h2 ? h2 : 1, 256, 1024)), dim3(256), 0, s, f, b, len, r, f2, b2, res,
-                       res_clear);
+    hipLaunchKernelGGL((k_sumcheck_fold<F, false>), dim3(grid_for(h2 ? h2 : 1, 256, 1024)), dim3(256), 0, s, f, b, len, r,
+                       FoldStep<F>{}, f2, b2, res, res_clear);
+}
+
+template <class F>
+void launch_sumcheck_fold_step(const typename F::T* f, const typename F::T* b, size_t len, const FoldStep<F>& step,
+                               typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
+                               hipStream_t s) {
+    if (step.res_in == res || step.res_in == res_clear || step.st_in == step.st_out)
+        throw std::invalid_argument("fold step: the triple or state read must not be written by the same kernel");
+    size_t h2 = len / 4;
+    hipLaunchKernelGGL((k_sumcheck_fold<F, true>), dim3(grid_for(h2 ? h2 : 1, 256, 1024)), dim3(256), 0, s, f, b, len,
+                       (const typename F::T*)nullptr, step, f2, b2, res, res_clear);
 }
 
 template <class F>
@@ -1841,6 +1883,8 @@ void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename
     template void launch_zero_acc<F>(F::Acc*, hipStream_t);                                                                 \
     template void launch_sumcheck_fold<F>(const F::T*, const F::T*, size_t, const F::T*, F::T*, F::T*, F::Acc*, F::Acc*,  \
                                           hipStream_t);                                                                     \
+    template void launch_sumcheck_fold_step<F>(const F::T*, const F::T*, size_t, const FoldStep<F>&, F::T*, F::T*, F::Acc*, \
+                                               F::Acc*, hipStream_t);                                                       \
     template void launch_fs_step<F>(DevTranscript*, const DevTranscript*, const F::Acc*, const F::Acc*, int, F::T*, F::T*,  \
                                     F::T*, int, uint32_t*, hipStream_t);                                                    \
     template void launch_glue<F>(F::T*, const F::T*, size_t, const F::T*, hipStream_t);                                           \

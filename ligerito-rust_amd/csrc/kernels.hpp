@@ -149,6 +149,23 @@ template <class F>
 void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t len, const typename F::T* r,
                           typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
                           hipStream_t s);
+// The transcript step that precedes a fold, run by the fold kernel itself:
+// every workgroup loads the state at st_in, absorbs the triple at res_in
+// (reduced) and squeezes the fold's challenge r; workgroup 0 also stores the
+// new state at st_out (!= st_in), logs the triple to tlog[0..3) and r to
+// rlog[0].  res_in must differ from res and res_clear (all workgroups read
+// it while the kernel writes those).
+template <class F> struct FoldStep {
+    const DevTranscript* st_in;
+    DevTranscript* st_out;
+    const typename F::Acc* res_in;
+    typename F::T* tlog;
+    typename F::T* rlog;
+};
+template <class F>
+void launch_sumcheck_fold_step(const typename F::T* f, const typename F::T* b, size_t len, const FoldStep<F>& step,
+                               typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
+                               hipStream_t s);
 // One transcript step on the device, on the state at st (or, if init is
 // given, on *init, passed by value): if res is given, reduce the triple
 // (three accumulators) and absorb it (QuadraticEvals::to_bytes,
