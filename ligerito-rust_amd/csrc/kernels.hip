@@ -977,48 +977,6 @@ void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t*
 }
 
 // ----------------------------------------------------------- Merkle levels --
-// A workgroup turns 2^g nodes of layer `lin` into the g layers above them.
-__global__ __launch_bounds__(256) void k_merkle_subtree(uint32_t* __restrict__ tree, size_t in_off, size_t n_in, int g,
-                                                        uint32_t* __restrict__ root_out) {
-    __shared__ uint32_t buf[2][256][8];
-    const size_t base = blockIdx.x * ((size_t)1 << g);  // first input node of this group
-    size_t lay_off = in_off, lay_n = n_in;
-    int cur = 0;
-    for (int l = 1; l <= g; l++) {
-        const int cnt = 1 << (g - l);
-        const size_t out_off = lay_off + lay_n;
-        const size_t out_base = base >> l;
-        for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
-            uint32_t msg[16];
-            if (l == 1) {
-                const uint32_t* src = tree + (lay_off + base + 2 * (size_t)i) * 8;
-#pragma unroll
-                for (int w = 0; w < 16; w++) msg[w] = src[w];
-            } else {
-#pragma unroll
-                for (int w = 0; w < 8; w++) {
-                    msg[w] = buf[cur][2 * i][w];
-                    msg[8 + w] = buf[cur][2 * i + 1][w];
-                }
-            }
-            uint32_t out[8];
-            b3::hash64(msg, out);
-            uint32_t* dst = tree + (out_off + out_base + i) * 8;
-#pragma unroll
-            for (int w = 0; w < 8; w++) {
-                dst[w] = out[w];
-                buf[cur ^ 1][i][w] = out[w];
-            }
-            if (root_out && cnt == 1 && lay_n == 2)
-                for (int w = 0; w < 8; w++) root_out[w] = out[w];
-        }
-        __syncthreads();
-        cur ^= 1;
-        lay_off = out_off;
-        lay_n >>= 1;
-    }
-}
-
 // One tree level: each thread hashes one pair of children.
 __global__ __launch_bounds__(256) void k_merkle_level(uint32_t* __restrict__ tree, size_t in_off, size_t n_out) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_out; i += (size_t)gridDim.x * blockDim.x) {
