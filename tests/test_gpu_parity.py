@@ -80,7 +80,23 @@ def test_build_merkle_tree_matches_oracle(O, ctx, leaf_len):
     assert got == want
 
 
+@pytest.mark.parametrize("log_leaves", [17, 20])
+def test_build_merkle_tree_wide_levels(O, ctx, log_leaves):
+    # 2^20 leaves take the three-levels-per-thread kernel, then the
+    # four-lane subtree kernels; 2^17 leaves only the latter
+    n = 1 << log_leaves
+    say("merkle wide", n)
+    rng = np.random.default_rng(log_leaves)
+    leaves = rng.integers(0, 256, size=(n, 16), dtype=np.uint8)
+    got = lh.build_merkle_tree(ctx, leaves)
+    want = O.build_merkle_tree([bytes(r) for r in leaves])
+    assert got == want
+
+
+# GF(2^32) shapes cover the transpose-and-scale tiles of 4, 8, 16, 32 and 64
+# columns, two column tiles (128), and the generic path (m < 64 or n < 4)
 @pytest.mark.parametrize("bits,m,n", [(32, 1 << 10, 64), (32, 1 << 8, 16), (32, 64, 4), (32, 16, 1),
+                                      (32, 1 << 7, 8), (32, 1 << 6, 32), (32, 32, 8), (32, 1 << 8, 2),
                                       (128, 1 << 8, 16), (128, 64, 64), (32, 1 << 9, 128)])
 def test_ligero_commit_matches_oracle(O, ctx, bits, m, n):
     say("commit", bits, m, n)
