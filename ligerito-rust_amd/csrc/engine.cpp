@@ -440,7 +440,6 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     // one and writes the other (its workgroups read while workgroup 0 writes)
     PBuf<DevTranscript> d_fs(&ctx->pool, 2);
     int fsi = 0;
-    PBuf<T> d_tlog(&ctx->pool, 3 * (size_t)(maxk + 1)), d_r(&ctx->pool, (size_t)maxk + 2);
     // three triple accumulators in rotation: a triple kernel reads the latest
     // triple (res_cur, as the transcript input of a fold step), adds into
     // res_next (clear) and clears res_free; then the roles rotate, so
@@ -472,36 +471,49 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     T* dev_tlog = (T*)(ctx->pin_dev + PIN_FSDN);
     T* dev_rlog = dev_tlog + n_tlog;
     uint32_t* dev_err = (uint32_t*)(dev_rlog + n_r);
-    // Launches: leading step (absorb the triple in res_cur, check it against
-    // the enforced sum, squeeze n0), optional glue with the first squeeze,
-    // then k fold rounds; the transcript step after round j-1 (absorb its
-    // triple, squeeze round j's challenge) runs inside round j's fold, and a
-    // last step absorbs the final triple.
+    // Launches: k fold rounds, each running the transcript step before it:
+    // round 0 the section's leading step (absorb the triple in res_cur from
+    // the host's exported state, check it against the enforced sum, squeeze
+    // n0 challenges, glue b + beta * bnew when n0 == 2), round j the step
+    // after round j-1 (absorb its triple, squeeze round j's challenge); then
+    // a last step absorbs the final triple.
     auto launch_section = [&](int n0, int k, PBuf<T>* d_bnew) {
         DevTranscript init;
         export_transcript(fs, init);
         *(uint32_t*)(h_fsdn + (n_tlog + n_r) * sizeof(T)) = 0;
         PhaseTimer pt(ctx, PH_SUMCHECK);
-        launch_fs_step<F>(d_fs.p + fsi, &init, res_cur, d_enf, n_enf, dev_tlog, d_r.p, dev_rlog, n0, dev_err, s);
-        if (d_bnew) launch_glue<F>(d_b.p, d_bnew->p, flen, d_r.p, s);
         for (int j = 0; j < k; j++) {
+            FoldStep<F> step{};
+            step.st_out = d_fs.p + (fsi ^ 1);
+            step.res_in = res_cur;
             if (j == 0) {
-                launch_sumcheck_fold<F>(d_f.p, d_b.p, flen, d_r.p + (n0 - 1), d_f2.p, d_b2.p, res_next, res_free, s);
+                // the leading step: from the host's state, with the
+                // enforced-sum check, n0 challenges and (n0 == 2) the glue
+                step.st_in = nullptr;
+                step.init = init;
+                step.enf = d_enf;
+                step.n_enf = n_enf;
+                step.err = dev_err;
+                step.n_squeeze = n0;
+                step.bnew = d_bnew ? d_bnew->p : nullptr;
+                step.tlog = dev_tlog;
+                step.rlog = dev_rlog;
             } else {
-                // the step after round j-1 (absorb its triple, squeeze r) runs
-                // inside round j's fold
-                FoldStep<F> step{d_fs.p + fsi, d_fs.p + (fsi ^ 1), res_cur, dev_tlog + 3 * j, dev_rlog + n0 + (j - 1)};
-                launch_sumcheck_fold_step<F>(d_f.p, d_b.p, flen, step, d_f2.p, d_b2.p, res_next, res_free, s);
-                fsi ^= 1;
+                // the step after round j-1 (absorb its triple, squeeze r)
+                step.st_in = d_fs.p + fsi;
+                step.n_squeeze = 1;
+                step.tlog = dev_tlog + 3 * j;
+                step.rlog = dev_rlog + n0 + (j - 1);
             }
+            launch_sumcheck_fold_step<F>(d_f.p, d_b.p, flen, step, d_f2.p, d_b2.p, res_next, res_free, s);
+            fsi ^= 1;
             rotate();
             std::swap(d_f, d_f2);
             std::swap(d_b, d_b2);
             flen /= 2;
         }
         // the last triple is absorbed with no challenge after it
-        launch_fs_step<F>(d_fs.p + fsi, nullptr, res_cur, nullptr, 0, dev_tlog + 3 * k, d_r.p + n0 + k, dev_rlog + n0 + k, 0,
-                          dev_err, s);
+        launch_fs_step<F>(d_fs.p + fsi, nullptr, res_cur, nullptr, 0, dev_tlog + 3 * k, nullptr, nullptr, 0, dev_err, s);
         LIG_HIP(hipGetLastError());
         sec.n0 = n0;
         sec.k = k;

@@ -1664,34 +1664,59 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
         ba = b[i0], bb = b[i0 + L2], bc = b[i0 + h2], bd = b[i0 + h2 + L2];
     }
     __shared__ T tab[F::TAB];
+    __shared__ T tab_beta[STEP ? F::TAB : 1];
     T r;
+    bool glue = false;
     if constexpr (STEP) {
         // the preceding transcript step, computed by wave 0 of every workgroup
-        __shared__ T r_sh;
+        __shared__ T r_sh[2];
         if (threadIdx.x < 64) {
             dfs::State st;
-            st.load(step.st_in);
+            if (step.st_in)
+                st.load(step.st_in);
+            else
+                st.load_value(step.init);
             T t[3];
 #pragma unroll
             for (int k = 0; k < 3; k++) t[k] = Ops<F>::acc_reduce(dfs::uniform_acc(step.res_in[k]));
+            uint32_t bad = 0;
+            if (step.enf) {
+                A e = Ops<F>::acc_zero();
+                for (int i = threadIdx.x; i < step.n_enf; i += 64) e ^= step.enf[i];
+                e = wave_xor(e);
+                bad = Ops<F>::eq(t[0] ^ t[1], Ops<F>::acc_reduce(dfs::uniform_acc(e))) ? 0u : 1u;
+            }
 #pragma unroll
             for (int k = 0; k < 3; k++)
 #pragma unroll
                 for (int i = 0; i < Ops<F>::WORDS; i++) st.word_le(Ops<F>::word(t[k], i));
             uint32_t o[4];
             st.squeeze(o);
-            const T rv = Ops<F>::from_squeeze(o);
+            const T rv0 = Ops<F>::from_squeeze(o);
+            T rv1 = rv0;
+            if (step.n_squeeze == 2) {
+                st.squeeze(o);
+                rv1 = Ops<F>::from_squeeze(o);
+            }
             if (threadIdx.x == 0) {
-                r_sh = rv;
+                r_sh[0] = rv0;
+                r_sh[1] = rv1;
                 if (blockIdx.x == 0) {
                     st.store(step.st_out);
                     for (int k = 0; k < 3; k++) step.tlog[k] = t[k];
-                    step.rlog[0] = rv;
+                    step.rlog[0] = rv0;
+                    if (step.n_squeeze == 2) step.rlog[1] = rv1;
+                    if (bad) *step.err = 1;
                 }
             }
         }
         LIG_LDS_BARRIER();
-        r = r_sh;
+        r = r_sh[1];
+        glue = step.bnew != nullptr;
+        if (glue) {
+            const T beta = r_sh[0];
+            for (int e = threadIdx.x; e < F::TAB; e += blockDim.x) tab_beta[e] = Ops<F>::table_entry(beta, e);
+        }
     } else {
         r = *rp;
     }
@@ -1700,14 +1725,30 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
     A s0 = Ops<F>::acc_zero(), s1 = Ops<F>::acc_zero(), s2 = Ops<F>::acc_zero();
     if (h2 == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
+            T b0 = b[0], b1 = b[1];
+            if constexpr (STEP) {
+                if (glue) {
+                    b0 ^= Ops<F>::mul_tab(tab_beta, step.bnew[0]);
+                    b1 ^= Ops<F>::mul_tab(tab_beta, step.bnew[1]);
+                }
+            }
             f2[0] = f[0] ^ Ops<F>::mul_tab(tab, f[0] ^ f[1]);
-            b2[0] = b[0] ^ Ops<F>::mul_tab(tab, b[0] ^ b[1]);
+            b2[0] = b0 ^ Ops<F>::mul_tab(tab, b0 ^ b1);
         }
     } else {
         for (size_t i = i0; i < h2; i += stride) {
             if (i != i0) {
                 fa = f[i], fb = f[i + L2], fc = f[i + h2], fd = f[i + h2 + L2];
                 ba = b[i], bb = b[i + L2], bc = b[i + h2], bd = b[i + h2 + L2];
+            }
+            if constexpr (STEP) {
+                if (glue) {
+                    const T* bn = step.bnew;
+                    ba ^= Ops<F>::mul_tab(tab_beta, bn[i]);
+                    bb ^= Ops<F>::mul_tab(tab_beta, bn[i + L2]);
+                    bc ^= Ops<F>::mul_tab(tab_beta, bn[i + h2]);
+                    bd ^= Ops<F>::mul_tab(tab_beta, bn[i + h2 + L2]);
+                }
             }
             // (1+r)*e[i] + r*e[i+L2] = e[i] + r*(e[i] + e[i+L2])  (multilinear_poly.rs:71-73)
             T f0 = fa ^ Ops<F>::mul_tab(tab, fa ^ fb);
@@ -1743,6 +1784,8 @@ void launch_sumcheck_fold_step(const typename F::T* f, const typename F::T* b, s
                                hipStream_t s) {
     if (step.res_in == res || step.res_in == res_clear || step.st_in == step.st_out)
         throw std::invalid_argument("fold step: the triple or state read must not be written by the same kernel");
+    if (step.n_squeeze < 1 || step.n_squeeze > 2 || (step.bnew && step.n_squeeze != 2))
+        throw std::invalid_argument("fold step: one challenge, or two with glue");
     size_t h2 = len / 4;
     hipLaunchKernelGGL((k_sumcheck_fold<F, true>), dim3(grid_for(h2 ? h2 : 1, 256, 1024)), dim3(256), 0, s, f, b, len,
                        (const typename F::T*)nullptr, step, f2, b2, res, res_clear);

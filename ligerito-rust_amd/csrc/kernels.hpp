@@ -150,15 +150,26 @@ void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t
                           typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
                           hipStream_t s);
 // The transcript step that precedes a fold, run by the fold kernel itself:
-// every workgroup loads the state at st_in, absorbs the triple at res_in
-// (reduced) and squeezes the fold's challenge r; workgroup 0 also stores the
-// new state at st_out (!= st_in), logs the triple to tlog[0..3) and r to
-// rlog[0].  res_in must differ from res and res_clear (all workgroups read
-// it while the kernel writes those).
+// every workgroup takes the state at st_in (or, if st_in is null, the state
+// init, passed by value), absorbs the triple at res_in (reduced) and
+// squeezes n_squeeze challenges; the fold's r is the last one.  Workgroup 0
+// also stores the new state at st_out (!= st_in), logs the triple to
+// tlog[0..3) and the challenges to rlog[0..n_squeeze), and, if enf is given,
+// sets *err to 1 when t0 + t1 differs from the enforced sum (the XOR of the
+// n_enf parts at enf, ligerito.rs:35,52).  With bnew, the fold reads
+// b + beta * bnew instead of b (glue, ligerito.rs:59-67), beta being the
+// first of two challenges.  res_in must differ from res and res_clear (all
+// workgroups read it while the kernel writes those).
 template <class F> struct FoldStep {
     const DevTranscript* st_in;
+    DevTranscript init;
     DevTranscript* st_out;
     const typename F::Acc* res_in;
+    const typename F::Acc* enf;
+    int n_enf;
+    uint32_t* err;
+    int n_squeeze;
+    const typename F::T* bnew;
     typename F::T* tlog;
     typename F::T* rlog;
 };
