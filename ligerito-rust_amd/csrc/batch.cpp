@@ -29,6 +29,7 @@ template <class H> Worker<H>::~Worker() {
     if (!ctx.stream) return;
     (void)hipSetDevice(ctx.device);
     (void)hipStreamSynchronize(ctx.stream);
+    ctx.release_side();
     cfgs.clear();
     ctx.pool.drain();
     (void)hipStreamDestroy(ctx.stream);

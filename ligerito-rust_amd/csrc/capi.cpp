@@ -118,6 +118,7 @@ void lig_ctx_destroy(lig_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->c.device);
     (void)hipStreamSynchronize(ctx->c.stream);
+    ctx->c.release_side();
     try {
         ctx->c.collect();
     } catch (...) {

@@ -238,6 +238,10 @@ template <class H> void Config<H>::init(Ctx* c, int L) {
         rs.emplace_back(new RsPlan<H>());
         rs.back()->init(c, dims_m[i], 4 * dims_m[i]);
     }
+    // induce over 2^ell for ell = L - 6 (initial fold) down by 4 per step
+    sks_by_ell.assign(L + 1, Vec<H>());
+    for (int ell = L - 6; ell >= 0 && ell >= L - 6 - 4 * recursive_steps; ell -= 4)
+        sks_by_ell[ell] = eval_sk_at_vks<H>((size_t)1 << ell);
 }
 
 // ----------------------------------------------------------------- prover --
@@ -265,7 +269,8 @@ constexpr size_t pin_round(size_t b) { return (b + 255) & ~(size_t)255; }
 // The opened rows: query indices to pinned memory, the row gather launched
 // (its output feeds induce).  open_launch_nodes adds the Merkle proof.
 template <class H>
-void open_launch_rows(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, PendingOpen<H>& po, size_t pin_off) {
+void open_launch_rows(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, PendingOpen<H>& po, size_t pin_off,
+                      size_t pin_end) {
     using F = typename DevField<H>::F;
     using T = typename H::T;
     Ctx* ctx = w.ctx;
@@ -274,7 +279,9 @@ void open_launch_rows(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, 
     const size_t q_bytes = pin_round(q.size() * 8), rows_bytes = pin_round(q.size() * w.n * sizeof(T));
     po.max_nodes = q.size() * (size_t)ilog2(w.M);
     po.nodes_off = pin_off + q_bytes + rows_bytes;
-    uint8_t* pin = ctx->pinned(po.nodes_off + pin_round(po.max_nodes * 8) + po.max_nodes * 32);
+    const size_t end = po.nodes_off + pin_round(po.max_nodes * 8) + po.max_nodes * 32;
+    LIG_REQUIRE(end <= pin_end, LIG_E_INVALID, "opening exceeds its staging region");
+    uint8_t* pin = ctx->pinned(end);
     uint64_t* h_idx = (uint64_t*)(pin + pin_off);
     for (size_t i = 0; i < q.size(); i++) h_idx[i] = q[i];
     po.d_q = reinterpret_cast<const uint64_t*>(ctx->pin_dev + pin_off);
@@ -304,10 +311,25 @@ template <class H> void open_launch_nodes(Witness<H>& w, const std::vector<size_
                         (uint32_t*)(ctx->pin_dev + nodes_at), ctx->stream);
 }
 
+// An opening read on the host from a copy of the witness: the matrix
+// (M rows of n elements) and the Merkle tree (2M - 1 nodes of 32 bytes,
+// leaves first), as the gathers above read them on the device.
 template <class H>
-void open_launch(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, PendingOpen<H>& po, size_t pin_off) {
-    open_launch_rows<H>(w, q, out, po, pin_off);
-    open_launch_nodes<H>(w, q, po);
+void open_host(const uint8_t* mat, const uint8_t* tree, size_t M, size_t n, const std::vector<size_t>& q, Rlp<H>& out) {
+    using T = typename H::T;
+    out.n_rows = q.size();
+    out.n_cols = n;
+    out.rows.resize(q.size() * n);
+    for (size_t i = 0; i < q.size(); i++) {
+        LIG_REQUIRE(q[i] < M, LIG_E_ASSERT, "query index beyond the witness");
+        memcpy(out.rows.data() + i * n, mat + q[i] * n * sizeof(T), n * sizeof(T));
+    }
+    const std::vector<uint64_t> nodes = merkle_proof_nodes(M, q);
+    out.nodes.resize(nodes.size() * 32);
+    for (size_t i = 0; i < nodes.size(); i++) {
+        LIG_REQUIRE(nodes[i] < 2 * M - 1, LIG_E_ASSERT, "Merkle node index beyond the tree");
+        memcpy(out.nodes.data() + i * 32, tree + nodes[i] * 32, 32);
+    }
 }
 
 // Call after the stream has been synchronized.
@@ -335,15 +357,15 @@ void export_transcript(const Transcript& fs, DevTranscript& d) {
 // basis polynomial into d_basis, enforced sum (unreduced) as *n_enf parts
 // at d_enf.
 template <class H>
-void induce_launch(Ctx* ctx, int ell, const PendingOpen<H>& po, size_t n_cols, const Vec<H>& gr, const std::vector<size_t>& q,
-                   typename H::T alpha, PBuf<typename H::T>& d_basis, typename DevField<H>::F::Acc* d_enf, int* n_enf,
-                   size_t pin_off) {
+void induce_launch(Ctx* ctx, int ell, const Vec<H>& sks, const PendingOpen<H>& po, size_t n_cols, const Vec<H>& gr,
+                   const std::vector<size_t>& q, typename H::T alpha, PBuf<typename H::T>& d_basis,
+                   typename DevField<H>::F::Acc* d_enf, int* n_enf, size_t pin_off) {
     using F = typename DevField<H>::F;
     using T = typename H::T;
     hipStream_t s = ctx->stream;
     const size_t nq = q.size();
     LIG_REQUIRE(gr.size() == n_cols, LIG_E_ASSERT, "opened row length != 2^k");
-    Vec<H> sks = eval_sk_at_vks<H>((size_t)1 << ell);
+    LIG_REQUIRE(sks.size() == (size_t)ell + 1, LIG_E_ASSERT, "subspace constants of another size");
     // sks | gr | alpha in pinned memory, read by the kernels through the
     // mapping (valid until the stream is next synchronized)
     const size_t nvals = sks.size() + gr.size() + 1;
@@ -383,9 +405,15 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     // Pinned staging regions (disjoint, so nothing in flight touches a
     // region the host is refilling): [0,256) commit roots, [256,16K) uploads
     // of induce constants and fold weights, [16K,32K) device transcript
-    // logs, [32K,2M) openings, [2M,4M) yr.
-    constexpr size_t PIN_UP = 256, PIN_FSDN = 16384, PIN_OPEN = 32768, PIN_YR = (size_t)2 << 20;
-    ctx->pinned((size_t)4 << 20);
+    // logs, [32K,1M) and [1M,2M) openings (alternating: the host copies one
+    // step's opening out while the next step's gathers fill the other),
+    // [2M,4M) yr, from 4M the last witness (matrix, then Merkle tree), whose
+    // openings the host reads there.
+    constexpr size_t PIN_UP = 256, PIN_FSDN = 16384, PIN_YR = (size_t)2 << 20, PIN_FINAL = (size_t)4 << 20;
+    const size_t pin_open[3] = {32768, (size_t)1 << 20, (size_t)2 << 20};
+    int open_slot = 0;
+    const size_t final_M = 4 * cfg.dims_m.back(), final_mat_bytes = final_M * cfg.dims_n.back() * sizeof(T);
+    ctx->pinned(PIN_FINAL + final_mat_bytes + (2 * final_M - 1) * 32);
     Witness<H> w0;
     commit<H>(*cfg.initial_rs, d_poly, cfg.initial_m, cfg.initial_n, w0, PH_ENCODE0);
     memcpy(proof.initial_root.data(), w0.root, 32);
@@ -418,16 +446,6 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         LIG_HIP(hipGetLastError());
     }
 
-    Witness<H> wprev;
-    commit<H>(*cfg.rs[0], d_f.p, cfg.dims_m[0], cfg.dims_n[0], wprev);
-    proof.rec_roots.emplace_back();
-    memcpy(proof.rec_roots.back().data(), wprev.root, 32);
-    fs.absorb(wprev.root, 32);
-
-    std::vector<size_t> q = fs.get_distinct_queries(w0.M, S);
-    T alpha = fs.get_field<H>();
-    int nv = ilog2(flen);
-
     // Sumcheck sections run on the device transcript: the host hands its
     // transcript over after the section's host-side steps, a chain of
     // kernels absorbs each triple and squeezes each challenge, and after the
@@ -457,8 +475,31 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         res_free = t;
     };
     int n_enf = 0;
-    launch_zero_acc<F>(res_next, s);
-    launch_zero_acc<F>(res_free, s);
+    // res_next and res_free are adjacent: one launch, queued ahead of the
+    // recursive commit rather than after the synchronisation that ends it
+    launch_zero_acc<F>(res_next, 6, s);
+
+    // The last witness is copied to pinned memory on the side stream while
+    // the last sumcheck section runs; the final openings then need no GPU
+    // round trip.
+    auto stage_final = [&](const Witness<H>& w) {
+        LIG_REQUIRE(w.M == final_M && w.M * w.n * sizeof(T) == final_mat_bytes, LIG_E_ASSERT, "last witness shape");
+        hipStream_t ss = ctx->side_stream();
+        LIG_HIP(hipMemcpyAsync(ctx->pin + PIN_FINAL, w.mat.p, final_mat_bytes, hipMemcpyDeviceToHost, ss));
+        LIG_HIP(hipMemcpyAsync(ctx->pin + PIN_FINAL + final_mat_bytes, w.tree.p, (2 * w.M - 1) * 32, hipMemcpyDeviceToHost, ss));
+    };
+
+    Witness<H> wprev;
+    commit<H>(*cfg.rs[0], d_f.p, cfg.dims_m[0], cfg.dims_n[0], wprev);  // synchronizes
+    if (cfg.recursive_steps == 1) stage_final(wprev);
+    proof.rec_roots.emplace_back();
+    memcpy(proof.rec_roots.back().data(), wprev.root, 32);
+    fs.absorb(wprev.root, 32);
+
+    std::vector<size_t> q = fs.get_distinct_queries(w0.M, S);
+    T alpha = fs.get_field<H>();
+    int nv = ilog2(flen);
+
     PBuf<T> d_b, d_b2(&ctx->pool, flen / 2);
 
     struct Section {
@@ -546,8 +587,10 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     // the GPU starts on the rows, induce and the sumcheck section before the
     // host lists the Merkle proof nodes
     std::unique_ptr<PendingOpen<H>> pending(new PendingOpen<H>());
-    open_launch_rows<H>(w0, q, proof.initial, *pending, PIN_OPEN);
-    induce_launch<H>(ctx, nv, *pending, w0.n, gr0, q, alpha, d_b, d_enf, &n_enf, PIN_UP);
+    // pending openings point into rec_proofs until they are copied out
+    proof.rec_proofs.reserve(cfg.recursive_steps);
+    open_launch_rows<H>(w0, q, proof.initial, *pending, pin_open[0], pin_open[1]);
+    induce_launch<H>(ctx, nv, cfg.sks_by_ell.at(nv), *pending, w0.n, gr0, q, alpha, d_b, d_enf, &n_enf, PIN_UP);
     {
         PhaseTimer pt(ctx, PH_SUMCHECK);
         launch_eval013<F>(d_f.p, d_b.p, flen, res_next, res_free, s);
@@ -569,20 +612,18 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             LIG_HIP(hipGetLastError());
             ctx->sync();
             replay_section("sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
-            open_finish<H>(*pending);
             proof.yr.assign(hy, hy + flen);
             fs.absorb_elems<H>(proof.yr.data(), flen);
             q = fs.get_distinct_queries(wprev.M, S);
-            PendingOpen<H> po;
-            open_launch<H>(wprev, q, proof.final_, po, PIN_OPEN);
-            ctx->sync();
-            open_finish<H>(po);
+            open_finish<H>(*pending);
+            ctx->wait(ctx->side_stream());  // the last witness copy, long done
+            open_host<H>(ctx->pin + PIN_FINAL, ctx->pin + PIN_FINAL + final_mat_bytes, wprev.M, wprev.n, q, proof.final_);
             return;
         }
         Witness<H> wnext;
         commit<H>(*cfg.rs[i + 1], d_f.p, cfg.dims_m[i + 1], cfg.dims_n[i + 1], wnext);  // synchronizes
+        if (i + 1 == cfg.recursive_steps - 1) stage_final(wnext);
         replay_section("sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
-        open_finish<H>(*pending);
         proof.rec_roots.emplace_back();
         memcpy(proof.rec_roots.back().data(), wnext.root, 32);
         fs.absorb(wnext.root, 32);
@@ -592,10 +633,11 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         nv = ilog2(flen);
         proof.rec_proofs.emplace_back();
         PBuf<T> d_bnew;
-        pending.reset(new PendingOpen<H>());
-        open_launch_rows<H>(wprev, q, proof.rec_proofs.back(), *pending, PIN_OPEN);
+        std::unique_ptr<PendingOpen<H>> next(new PendingOpen<H>());
+        open_slot ^= 1;
+        open_launch_rows<H>(wprev, q, proof.rec_proofs.back(), *next, pin_open[open_slot], pin_open[open_slot + 1]);
         Vec<H> gr = lagrange_basis<H>(fold_rs);
-        induce_launch<H>(ctx, nv, *pending, wprev.n, gr, q, a, d_bnew, d_enf, &n_enf, PIN_UP);
+        induce_launch<H>(ctx, nv, cfg.sks_by_ell.at(nv), *next, wprev.n, gr, q, a, d_bnew, d_enf, &n_enf, PIN_UP);
         // introduce_new (ligerito.rs:50-57): triple of f with the new basis
         // alone; then glue (ligerito.rs:59-67) with the next challenge: the
         // summed basis absorbs beta * b_new (folds and the 0/1/3 evaluations
@@ -606,7 +648,9 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             rotate();
         }
         launch_section(2, cfg.ks[i + 1], &d_bnew);
-        open_launch_nodes<H>(wprev, q, *pending);
+        open_launch_nodes<H>(wprev, q, *next);
+        open_finish<H>(*pending);  // the previous opening, while this section runs
+        pending = std::move(next);
         wprev = std::move(wnext);
     }
     throw Error(LIG_E_ASSERT, "Prover completed without entering final step (prover.rs:198)");

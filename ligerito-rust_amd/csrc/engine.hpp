@@ -51,6 +51,7 @@ struct Ctx {
     uint8_t* pinned(size_t bytes) {
         if (bytes > pin_cap) {
             sync();
+            if (side) wait(side);
             if (pin) (void)hipHostFree(pin);
             pin = pin_dev = nullptr;
             size_t cap = std::max<size_t>(bytes, (size_t)1 << 20);
@@ -66,16 +67,31 @@ struct Ctx {
     // contexts that share the host with many others (batch and serve
     // workers) block instead.
     bool spin_wait = true;
-    void sync() {
+    void wait(hipStream_t st) {
         if (spin_wait) {
             for (;;) {
-                const hipError_t e = hipStreamQuery(stream);
+                const hipError_t e = hipStreamQuery(st);
                 if (e == hipSuccess) return;
                 if (e != hipErrorNotReady) LIG_HIP(e);
                 __builtin_ia32_pause();
             }
         }
-        LIG_HIP(hipStreamSynchronize(stream));
+        LIG_HIP(hipStreamSynchronize(st));
+    }
+    void sync() { wait(stream); }
+    // A second stream (default priority) for device-to-host copies that
+    // overlap the work on the context stream; created on first use, ended
+    // by release_side().
+    hipStream_t side = nullptr;
+    hipStream_t side_stream() {
+        if (!side) LIG_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        return side;
+    }
+    void release_side() {
+        if (!side) return;
+        (void)hipStreamSynchronize(side);
+        (void)hipStreamDestroy(side);
+        side = nullptr;
     }
     // Resolves recorded event pairs into per-phase totals.
     void collect();
@@ -181,6 +197,8 @@ template <class H> struct Config {
     std::vector<int> ks;
     std::unique_ptr<RsPlan<H>> initial_rs;
     std::vector<std::unique_ptr<RsPlan<H>>> rs;
+    // eval_sk_at_vks(2^ell) for every ell a prove's sumcheck inductions use
+    std::vector<Vec<H>> sks_by_ell;
     void init(Ctx* c, int L);
 };
 

@@ -769,8 +769,13 @@ static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t 
     while (((size_t)1 << ell) < m) ell++;
     const int KMAX = 6;
     const int npass = (ell + KMAX - 1) / KMAX;
-    const size_t nchunks = (n + 63) / 64;
-    const int cc = n < 64 ? (int)n : 64;
+    // Narrow matrices (n < 64): the 2^v = 64/n consecutive rows of an
+    // aligned group are contiguous, so a layer that pairs rows 2^v or more
+    // apart is the same layer (same twiddle blocks) of the (m >> v) x 64
+    // matrix over the same words.  Passes made only of such layers run on
+    // that view: 64-wide tiles, one set of tables per 64 columns.
+    int v = 0;
+    while ((n << v) < 64 && v < ell) v++;
     // All four quarters per launch (grid.y): one quarter per launch leaves a
     // partly filled last round of workgroups at the end of every launch,
     // which costs more than the quarter's cache residency between passes
@@ -779,8 +784,13 @@ static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t 
     for (int p = 0; p < npass; p++) {
         const int K = ell / npass + (p < ell % npass ? 1 : 0);
         const int K2 = K / 2;
-        const size_t tiles = (size_t)1 << (ell - K);
-        const size_t S = (size_t)1 << (ell - d0 - K);
+        const bool wide = d0 + K <= ell - v;
+        const size_t pm = wide ? m >> v : m, pn = wide ? n << v : n;
+        const int pell = wide ? ell - v : ell;
+        const size_t nchunks = (pn + 63) / 64;
+        const int cc = pn < 64 ? (int)pn : 64;
+        const size_t tiles = (size_t)1 << (pell - K);
+        const size_t S = (size_t)1 << (pell - d0 - K);
         // tiles sharing B (hence tables) are grouped per workgroup
         size_t jobs = tiles * nchunks;
         int tpw = 1;
@@ -790,7 +800,7 @@ static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t 
         dim3 grid((unsigned)(jobs / tpw), 4);
         const size_t elems = m * n * 4;
         if (obs) obs->before(K, elems);
-        launch_pass32(K, grid, cc << K2, s, src, mat, m, n, ell, d0, -1, shared, tpw, tw);
+        launch_pass32(K, grid, cc << K2, s, src, mat, pm, pn, pell, d0, -1, shared, tpw, tw);
         if (obs) obs->after(K, elems);
         d0 += K;
     }
@@ -1477,45 +1487,65 @@ __global__ void k_induce_prep(const uint64_t* __restrict__ q, const typename F::
     ap[i] = r;
 }
 
-// GF(2^32) variant: squaring and multiplication by each sks[t] and each
-// alpha^(2^k) are table multiplies (squaring is GF(2)-linear).
+// GF(2^32) variant, one workgroup: squaring and multiplication by each
+// sks[t] and each alpha^(2^k) are table multiplies (squaring is
+// GF(2)-linear).  The inputs sit in host memory (query indices, sks, alpha);
+// every thread issues its loads first, so the kernel waits for one round
+// trip.  With at most 192 queries the fourth wave builds the alpha tables
+// and the powers alpha^i while the first three run the next_s chains.
 __global__ __launch_bounds__(256) void k_induce_prep32(const uint64_t* __restrict__ q, const uint32_t* __restrict__ sks,
                                                        const uint32_t* __restrict__ alpha, int nq, int ell,
                                                        uint32_t* __restrict__ cvals, uint32_t* __restrict__ ap) {
     // table 0: squaring; 1..ell-1: times sks[t-1]; ell..ell+7: times alpha^(2^k)
     __shared__ __attribute__((aligned(256))) uint32_t tabs[(PREP_MAXELL + 8) * 128];
-    __shared__ uint32_t apow[8];
-    // squaring and sks tables, one nibble position per work item
-    for (int idx = threadIdx.x; idx < ell * 8; idx += blockDim.x) {
-        const int tbl = idx >> 3, k = idx & 7;
+    const int tid = threadIdx.x;
+    const int tbl = tid >> 3, k = tid & 7;  // ell * 8 <= 208 work items
+    uint32_t lam = 0u;
+    if (tid >= 8 && tid < ell * 8) lam = sks[tbl - 1];
+    const uint32_t a0 = alpha[0];
+    uint32_t x = tid < nq ? (uint32_t)q[tid] : 0u;
+    if (tid < ell * 8) {
         if (tbl == 0)
             gf32_tab_build_sqr(tabs, k);
         else
-            gf32_tab_build(tabs + tbl * 128, sks[tbl - 1], k);
+            gf32_tab_build(tabs + tbl * 128, lam, k);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t a = alpha[0];
-        for (int k = 0; k < 8; k++) {
-            apow[k] = a;
-            a = mul_tab_lds(tabs, 0u, a);  // squaring
-        }
+    // lane j of 64: table ell + j/8, nibble position j%8, of alpha^(2^(j/8))
+    auto alpha_tables = [&](int j) {
+        uint32_t a = a0;
+        for (int t = 0; t < (j >> 3); t++) a = mul_tab_lds(tabs, 0u, a);  // squaring
+        gf32_tab_build(tabs + (ell + (j >> 3)) * 128, a, j & 7);
+    };
+    auto alpha_power = [&](int i) {
+        uint32_t r = 1u;
+        for (int b = 0; b < 8; b++)
+            if ((i >> b) & 1) r = mul_tab_lds(tabs, (uint32_t)((ell + b) * 512), r);
+        ap[i] = r;
+    };
+    const bool spare = nq <= 192;
+    if (spare && tid >= 192) {
+        alpha_tables(tid - 192);
+        // only this wave reads the alpha tables: its own LDS writes done
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        for (int i = tid - 192; i < nq; i += 64) alpha_power(i);
     }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < 64; idx += blockDim.x) gf32_tab_build(tabs + (ell + (idx >> 3)) * 128, apow[idx >> 3], idx & 7);
-    __syncthreads();
-    for (int i = threadIdx.x; i < nq; i += blockDim.x) {
-        uint32_t x = (uint32_t)q[i];
-        uint32_t* c = cvals + (size_t)i * ell;
+    if (tid < nq) {
+        uint32_t* c = cvals + (size_t)tid * ell;
         c[0] = x;
         for (int t = 1; t < ell; t++) {
             x = mul_tab_lds(tabs, 0u, x) ^ mul_tab_lds(tabs, (uint32_t)(t * 512), x);  // next_s
             c[t] = x;
         }
-        uint32_t r = 1u;
-        for (int k = 0; k < 8; k++)
-            if ((i >> k) & 1) r = mul_tab_lds(tabs, (uint32_t)((ell + k) * 512), r);
-        ap[i] = r;
+    }
+    if (!spare) {
+        __syncthreads();
+        if (tid < 64) alpha_tables(tid);
+        __syncthreads();
+        if (tid < nq) alpha_power(tid);
     }
 }
 
@@ -1612,9 +1642,9 @@ template <class F> __device__ inline void block_xor3(typename F::Acc a0, typenam
     }
 }
 
-template <class F> __global__ void k_zero_acc(typename F::Acc* res) {
+template <class F> __global__ void k_zero_acc(typename F::Acc* res, int count) {
     using A = typename F::Acc;
-    if (threadIdx.x < 3) res[threadIdx.x] = A{};
+    if ((int)threadIdx.x < count) res[threadIdx.x] = A{};
 }
 
 template <class F>
@@ -1642,8 +1672,9 @@ void launch_eval013(const typename F::T* f, const typename F::T* g, size_t len, 
     hipLaunchKernelGGL(k_eval013<F>, dim3(grid_for(len / 2, 256, 1024)), dim3(256), 0, s, f, g, len, res, res_clear);
 }
 
-template <class F> void launch_zero_acc(typename F::Acc* res, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_acc<F>, dim3(1), dim3(64), 0, s, res);
+template <class F> void launch_zero_acc(typename F::Acc* res, int count, hipStream_t s) {
+    if (count < 1 || count > 64) throw std::invalid_argument("launch_zero_acc: 1..64 accumulators");
+    hipLaunchKernelGGL(k_zero_acc<F>, dim3(1), dim3(64), 0, s, res, count);
 }
 
 template <class F, bool STEP>
@@ -1881,7 +1912,7 @@ void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename
     template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, F::T*, hipStream_t);                  \
     template void launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t);                   \
     template void launch_eval013<F>(const F::T*, const F::T*, size_t, F::Acc*, F::Acc*, hipStream_t);                       \
-    template void launch_zero_acc<F>(F::Acc*, hipStream_t);                                                                 \
+    template void launch_zero_acc<F>(F::Acc*, int, hipStream_t);                                                                 \
     template void launch_sumcheck_fold<F>(const F::T*, const F::T*, size_t, const F::T*, F::T*, F::T*, F::Acc*, F::Acc*,  \
                                           hipStream_t);                                                                     \
     template void launch_sumcheck_fold_step<F>(const F::T*, const F::T*, size_t, const FoldStep<F>&, F::T*, F::T*, F::Acc*, \

@@ -137,11 +137,12 @@ void launch_enforced(const typename F::T* rows, int nrows, int ncols, const type
 // ------------------------------------------------------------ sumcheck ----
 // res[0..3) = (sum f0*g0, sum f1*g1, sum f2*g2) unreduced (eval_013_product).
 // res_clear (3 accumulators) is zeroed for the next accumulation; res must
-// have been zeroed by the previous call (or launch_zero_acc).
+// have been zeroed by the previous call (or launch_zero_acc, which zeroes
+// count consecutive accumulators).
 template <class F>
 void launch_eval013(const typename F::T* f, const typename F::T* g, size_t len, typename F::Acc* res,
                     typename F::Acc* res_clear, hipStream_t s);
-template <class F> void launch_zero_acc(typename F::Acc* res, hipStream_t s);
+template <class F> void launch_zero_acc(typename F::Acc* res, int count, hipStream_t s);
 // Folds f and b by r (length len -> len/2) into f2, b2 and writes the
 // triple of the folded pair (ligerito.rs:119-127).
 // r is read from device memory (written by launch_fs_step).
