@@ -172,6 +172,7 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
     w.m = m;
     w.n = n;
     w.M = M;
+    uint32_t seq = 0;
     {
         PBuf<T> msg(&ctx->pool, m * n);
         w.mat = PBuf<T>(&ctx->pool, M * n);
@@ -205,12 +206,13 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
         PhaseTimer pt(ctx, PH_MERKLE);
         PBuf<uint32_t> scratch(&ctx->pool, leaf_hash_scratch_words(M, n * sizeof(T)));
         launch_leaf_hash<F>(w.mat.p, M, n, w.tree.p, scratch.n ? scratch.p : nullptr, s);
-        uint8_t* pin = ctx->pinned(32);
+        uint8_t* pin = ctx->pinned(256);
         (void)pin;
-        launch_merkle_levels(w.tree.p, M, s, (uint32_t*)ctx->pin_dev);
+        seq = ctx->next_flag();
+        launch_merkle_levels(w.tree.p, M, s, (uint32_t*)ctx->pin_dev, ctx->flag_dev(), seq);
     }
     LIG_HIP(hipGetLastError());
-    ctx->sync();  // the last tree kernel wrote the root to pinned offset 0
+    ctx->wait_flag(seq);  // the last tree kernel wrote the root to pinned offset 0, then the flag
     memcpy(w.root, ctx->pin, 32);
 }
 
@@ -609,8 +611,10 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             LIG_REQUIRE(PIN_YR + flen * sizeof(T) <= ((size_t)4 << 20), LIG_E_INVALID, "yr exceeds the staging region");
             // written by a kernel straight into the mapped pinned buffer
             launch_copy_words((uint32_t*)(ctx->pin_dev + PIN_YR), (const uint32_t*)d_f.p, flen * (sizeof(T) / 4), s);
+            const uint32_t seq = ctx->next_flag();
+            launch_signal(ctx->flag_dev(), seq, s);
             LIG_HIP(hipGetLastError());
-            ctx->sync();
+            ctx->wait_flag(seq);  // yr and the section's logs are in pinned memory
             replay_section("sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
             proof.yr.assign(hy, hy + flen);
             fs.absorb_elems<H>(proof.yr.data(), flen);

@@ -4,6 +4,7 @@
 #pragma once
 #include <algorithm>
 #include <array>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -57,6 +58,7 @@ struct Ctx {
             size_t cap = std::max<size_t>(bytes, (size_t)1 << 20);
             LIG_HIP(hipHostMalloc((void**)&pin, cap, hipHostMallocMapped | hipHostMallocCoherent));
             LIG_HIP(hipHostGetDevicePointer((void**)&pin_dev, pin, 0));
+            memset(pin, 0, 256);  // the completion flag starts below any sequence number
             pin_cap = cap;
         }
         return pin;
@@ -79,6 +81,37 @@ struct Ctx {
         LIG_HIP(hipStreamSynchronize(st));
     }
     void sync() { wait(stream); }
+    // Completion flag: a kernel stores a sequence number at byte PIN_FLAG of
+    // the pinned buffer once the results the host needs are written (in
+    // mapped memory, system scope).  Polling it sees the results several
+    // microseconds before the stream reports the kernel complete.
+    static constexpr size_t PIN_FLAG = 128;
+    uint32_t flag_seq = 0;
+    uint32_t* flag_dev() { return reinterpret_cast<uint32_t*>(pin_dev + PIN_FLAG); }
+    uint32_t next_flag() { return ++flag_seq; }
+    void wait_flag(uint32_t seq) {
+        const uint32_t* f = reinterpret_cast<const uint32_t*>(pin + PIN_FLAG);
+        if (!spin_wait) {
+            sync();
+            LIG_REQUIRE(__atomic_load_n(f, __ATOMIC_ACQUIRE) == seq, LIG_E_ASSERT, "stream done without its completion flag");
+            return;
+        }
+        // every 1024 polls the stream is asked too, so that an error or a
+        // stream that ends without the flag stops the wait
+        for (uint32_t it = 1;; it++) {
+            if (__atomic_load_n(f, __ATOMIC_ACQUIRE) == seq) return;
+            if ((it & 1023) == 0) {
+                const hipError_t e = hipStreamQuery(stream);
+                if (e == hipSuccess) {
+                    LIG_REQUIRE(__atomic_load_n(f, __ATOMIC_ACQUIRE) == seq, LIG_E_ASSERT,
+                                "stream done without its completion flag");
+                    return;
+                }
+                if (e != hipErrorNotReady) LIG_HIP(e);
+            }
+            __builtin_ia32_pause();
+        }
+    }
     // A second stream (default priority) for device-to-host copies that
     // overlap the work on the context stream; created on first use, ended
     // by release_side().

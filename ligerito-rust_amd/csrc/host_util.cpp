@@ -324,6 +324,35 @@ bool have_avx2() {
 }
 }  // namespace
 
+namespace {
+// The first finalization block of a hash whose pending bytes are buf[0, nb)
+// (nb a multiple of 4): those bytes, 0x80, zeros to the end of the block
+// (the caller adds the length).  Fixed-size masks instead of copies and
+// fills of variable length.
+struct PadMasks {
+    alignas(16) uint8_t keep[16][64];
+    alignas(16) uint8_t pad[16][64];
+    PadMasks() {
+        for (int i = 0; i < 16; i++)
+            for (int b = 0; b < 64; b++) {
+                keep[i][b] = b < 4 * i ? 0xFF : 0x00;
+                pad[i][b] = b == 4 * i ? 0x80 : 0x00;
+            }
+    }
+};
+const PadMasks pad_masks;
+
+inline void pad_block(uint8_t* out, const uint8_t* buf, uint32_t nb) {
+    const int i = (int)(nb >> 2);
+    for (int j = 0; j < 4; j++) {
+        __m128i v = _mm_loadu_si128((const __m128i*)(buf + 16 * j));
+        v = _mm_and_si128(v, _mm_load_si128((const __m128i*)(pad_masks.keep[i] + 16 * j)));
+        v = _mm_or_si128(v, _mm_load_si128((const __m128i*)(pad_masks.pad[i] + 16 * j)));
+        _mm_storeu_si128((__m128i*)(out + 16 * j), v);
+    }
+}
+}  // namespace
+
 // get_distinct_queries (emulated_fs.rs:61-75).  Squeezes are generated in
 // groups: the running state absorbs each counter in turn and the
 // finalizations that fit one block run together (two interleaved with the
@@ -334,23 +363,48 @@ std::vector<size_t> Transcript::get_distinct_queries(size_t n, size_t s) {
     LIG_REQUIRE(s <= n, LIG_E_INVALID, "get_distinct_queries: s > n never terminates");
     std::vector<size_t> q;
     q.reserve(s);
-    std::vector<uint8_t> seen((n + 7) / 8, 0);
+    // Indices seen so far: an open-addressing set of at least 4s slots
+    // (value + 1, 0 = empty), on the stack for the usual s, rather than a
+    // bitmap over all n indices.
+    size_t cap = 64;
+    while (cap < 4 * s) cap <<= 1;
+    uint64_t local[1024];
+    std::vector<uint64_t> heap;
+    uint64_t* slots = local;
+    if (cap > 1024) {
+        heap.assign(cap, 0);
+        slots = heap.data();
+    } else {
+        memset(local, 0, cap * sizeof(uint64_t));
+    }
+    const int shift = 64 - __builtin_ctzll(cap);
+    // the row counts are powers of two: a mask instead of a 64-bit division
+    const bool pow2 = (n & (n - 1)) == 0;
     auto take = [&](uint64_t lo) {
-        const size_t v = (size_t)(lo % (uint64_t)n);
-        if (!(seen[v >> 3] & (1u << (v & 7)))) {
-            seen[v >> 3] |= (uint8_t)(1u << (v & 7));
-            q.push_back(v);
+        const size_t v = pow2 ? (size_t)(lo & (uint64_t)(n - 1)) : (size_t)(lo % (uint64_t)n);
+        size_t h = (size_t)(((uint64_t)v * 0x9E3779B97F4A7C15ull) >> shift);
+        for (;;) {
+            if (slots[h] == 0) {
+                slots[h] = (uint64_t)v + 1;
+                q.push_back(v);
+                return;
+            }
+            if (slots[h] == (uint64_t)v + 1) return;
+            h = (h + 1) & (cap - 1);
         }
     };
-    if (have_sha_ni()) {
-        // Squeezes in pairs: the running state absorbs both counters, the two
-        // one-block finalizations run interleaved; if the first completes
-        // the set, the transcript is rewound to just after it.
+    if (have_sha_ni() && hasher.blen % 4 == 0) {
+        // Squeezes in pairs: each counter goes straight into the running
+        // block (a full block is compressed into the running state), each
+        // squeeze's finalization is built from a copy of that block, and the
+        // pair's first finalization blocks run interleaved (a pending length
+        // of 56 or 60 bytes needs a second block, run alone).  If the first
+        // squeeze of a pair completes the set, the transcript is rewound to
+        // just after it.
         while (q.size() < s) {
-            uint64_t lo[2];
             uint32_t st[2][8];
-            alignas(16) uint8_t blk[2][64];
-            bool one[2];
+            alignas(16) uint8_t blk[2][2][64];
+            bool two[2];
             Sha256 h_mid;
             uint32_t c_mid = 0;
             for (int k = 0; k < 2; k++) {
@@ -358,33 +412,34 @@ std::vector<size_t> Transcript::get_distinct_queries(size_t n, size_t s) {
                     h_mid = hasher;
                     c_mid = counter;
                 }
-                uint8_t b[4];
-                memcpy(b, &counter, 4);
-                hasher.update(b, 4);
+                // the pending length stays a multiple of 4
+                memcpy(hasher.buf + hasher.blen, &counter, 4);
+                hasher.blen += 4;
+                hasher.total += 4;
+                if (hasher.blen == 64) {
+                    compress(hasher.h, hasher.buf, 1);
+                    hasher.blen = 0;
+                }
                 counter++;
                 const uint32_t nb = hasher.blen;
-                one[k] = nb < 56;
-                if (one[k]) {
-                    const uint64_t bits = hasher.total * 8;
-                    memcpy(st[k], hasher.h, 32);
-                    memcpy(blk[k], hasher.buf, nb);
-                    blk[k][nb] = 0x80;
-                    memset(blk[k] + nb + 1, 0, 56 - nb - 1);
-                    for (int i = 0; i < 8; i++) blk[k][56 + i] = (uint8_t)(bits >> (56 - 8 * i));
-                } else {
-                    uint8_t d[32];
-                    hasher.digest(d);
-                    memcpy(&lo[k], d, 8);
+                memcpy(st[k], hasher.h, 32);
+                uint8_t* b0 = blk[k][0];
+                pad_block(b0, hasher.buf, nb);
+                two[k] = nb >= 56;
+                uint8_t* last = b0;
+                if (two[k]) {
+                    last = blk[k][1];
+                    memset(last, 0, 56);
                 }
+                const uint64_t bits_be = __builtin_bswap64(hasher.total * 8);
+                memcpy(last + 56, &bits_be, 8);
             }
-            if (one[0] && one[1]) {
-                sha_ni_x2(st[0], blk[0], st[1], blk[1]);
-            } else {
-                for (int k = 0; k < 2; k++)
-                    if (one[k]) sha_blocks_ni(st[k], blk[k], 1);
+            sha_ni_x2(st[0], blk[0][0], st[1], blk[1][0]);
+            uint64_t lo[2];
+            for (int k = 0; k < 2; k++) {
+                if (two[k]) sha_blocks_ni(st[k], blk[k][1], 1);
+                lo[k] = (uint64_t)__builtin_bswap32(st[k][0]) | ((uint64_t)__builtin_bswap32(st[k][1]) << 32);
             }
-            for (int k = 0; k < 2; k++)
-                if (one[k]) lo[k] = (uint64_t)__builtin_bswap32(st[k][0]) | ((uint64_t)__builtin_bswap32(st[k][1]) << 32);
             take(lo[0]);
             if (q.size() == s) {
                 hasher = h_mid;
@@ -452,11 +507,7 @@ std::vector<size_t> Transcript::get_distinct_queries(size_t n, size_t s) {
         int used = 0;
         for (int k = 0; k < 8 && q.size() < s; k++) {
             used = k + 1;
-            const size_t v = (size_t)(lo[k] % (uint64_t)n);
-            if (!(seen[v >> 3] & (1u << (v & 7)))) {
-                seen[v >> 3] |= (uint8_t)(1u << (v & 7));
-                q.push_back(v);
-            }
+            take(lo[k]);
         }
         if (used < 8) {
             // rewind to the state after exactly `used` squeezes
@@ -498,6 +549,11 @@ std::vector<uint64_t> merkle_proof_nodes(size_t n_leaves, const std::vector<size
     std::vector<uint64_t> out;
     std::vector<size_t> q = sorted_queries;
     size_t cnt = q.size();
+    {
+        size_t depth = 0;
+        while (((size_t)1 << depth) < n_leaves) depth++;
+        out.reserve(cnt * depth);
+    }
     size_t layer_off = 0, layer_n = n_leaves;
     while (layer_n > 1) {
         size_t next = 0, i = 0;

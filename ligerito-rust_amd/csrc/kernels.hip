@@ -1044,9 +1044,12 @@ __device__ __forceinline__ uint32_t rotr(uint32_t x, int n) { return (x >> n) | 
 // layer at in_off become the g layers above them.  Dynamic LDS holds two
 // layers of at most 2^g nodes (2^(g+6) bytes).  Each level's nodes go to LDS
 // for the next level and to the tree in global memory; the barrier between
-// levels does not wait for the global stores.
+// levels does not wait for the global stores.  With a flag (the one
+// workgroup of the top launch), once every node is written the kernel stores
+// seq there (host memory; system scope, after the nodes and the root).
 __global__ __launch_bounds__(1024) void k_merkle_subtree4(uint32_t* __restrict__ tree, size_t in_off, size_t n_in, int g,
-                                                          uint32_t* __restrict__ root_out) {
+                                                          uint32_t* __restrict__ root_out, uint32_t* __restrict__ flag,
+                                                          uint32_t seq) {
     using namespace b3x4;
     extern __shared__ uint32_t sub4_lds[];
     uint32_t* lv[2] = {sub4_lds, sub4_lds + ((size_t)8 << g)};
@@ -1107,6 +1110,13 @@ __global__ __launch_bounds__(1024) void k_merkle_subtree4(uint32_t* __restrict__
         lay_off = out_off;
         lay_n >>= 1;
     }
+    if (flag) {
+        __syncthreads();  // every wave's global stores done
+        if (threadIdx.x == 0) {
+            __threadfence_system();
+            __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
 }
 
 // Three levels per launch for wide layers: thread t turns nodes 8t..8t+7 of
@@ -1164,7 +1174,7 @@ constexpr size_t MERKLE_WIDE = (size_t)1 << 17;
 // it, 256-thread workgroups of depth SUB4_G.
 constexpr int SUB4_G = 7, SUB4_TOP = 10;
 
-void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out) {
+void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out, uint32_t* flag, uint32_t seq) {
     size_t in_off = 0, nodes = n_leaves;
     while (nodes / 8 >= MERKLE_WIDE) {
         hipLaunchKernelGGL(k_merkle_block3, dim3(grid_for(nodes / 8, 256)), dim3(256), 0, s, tree, in_off, nodes);
@@ -1185,12 +1195,25 @@ void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32
         const size_t groups = nodes >> g;
         const size_t lds = (size_t)64 << g;  // two layers of 2^g nodes
         hipLaunchKernelGGL(k_merkle_subtree4, dim3((unsigned)groups), dim3(top ? 1024 : 256), lds, s, tree, in_off, nodes, g,
-                           root_out);
+                           root_out, top ? flag : nullptr, seq);
         for (int l = 0; l < g; l++) {
             in_off += nodes;
             nodes >>= 1;
         }
     }
+}
+
+// One thread stores seq at flag (host memory, system scope) once the
+// kernels before it on the stream are done.
+__global__ void k_signal(uint32_t* flag, uint32_t seq) {
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+void launch_signal(uint32_t* flag, uint32_t seq, hipStream_t s) {
+    hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, s, flag, seq);
 }
 
 __global__ void k_copy_words(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, size_t n) {

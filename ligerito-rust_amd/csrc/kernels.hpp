@@ -89,8 +89,12 @@ void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t*
 // Builds layers 1.. in place after layer 0 (contiguous node layout,
 // merkle_tree.rs:22-60).  n_leaves must be a power of two, at least 2.
 // If root_out is given, the root's 8 words are also written there (for
-// instance to mapped host memory).
-void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out = nullptr);
+// instance to mapped host memory); if flag is given, the last launch stores
+// seq there after every node and the root (for a host polling mapped memory).
+void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out = nullptr,
+                          uint32_t* flag = nullptr, uint32_t seq = 0);
+// Stores seq at flag (mapped host memory) after the stream's earlier kernels.
+void launch_signal(uint32_t* flag, uint32_t seq, hipStream_t s);
 // out[i] = node at byte offset offs[i] of the tree (32 bytes each).
 void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t count, uint32_t* out, hipStream_t s);
 
