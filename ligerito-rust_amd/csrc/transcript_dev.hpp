@@ -24,6 +24,18 @@ __device__ __forceinline__ uint32_t big_s0(uint32_t a) { return xor3(rotr(a, 2),
 __device__ __forceinline__ uint32_t big_s1(uint32_t e) { return xor3(rotr(e, 6), rotr(e, 11), rotr(e, 25)); }
 __device__ __forceinline__ uint32_t small_s0(uint32_t w) { return xor3(rotr(w, 7), rotr(w, 18), w >> 3); }
 __device__ __forceinline__ uint32_t small_s1(uint32_t w) { return xor3(rotr(w, 17), rotr(w, 19), w >> 10); }
+// Ch and Maj as one three-input bit operation each (truth tables over
+// (x, y, z) = (0xF0, 0xCC, 0xAA)).
+__device__ __forceinline__ uint32_t ch(uint32_t x, uint32_t y, uint32_t z) { return __builtin_amdgcn_bitop3_b32(x, y, z, 0xCA); }
+__device__ __forceinline__ uint32_t maj(uint32_t x, uint32_t y, uint32_t z) { return __builtin_amdgcn_bitop3_b32(x, y, z, 0xE8); }
+// Three-operand add; written out because the compiler keeps these sums as
+// chains of two-operand adds, and one wave's SHA-256 latency is its
+// instruction count.
+__device__ __forceinline__ uint32_t add3(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_add3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
 
 // Values every lane loaded from one address, moved to scalar registers so
 // the transcript arithmetic runs on the scalar unit.
@@ -56,11 +68,11 @@ __device__ __forceinline__ void compress(uint32_t h[8], const uint32_t w_in[16])
         if (i < 16) {
             wi = w[i];
         } else {
-            wi = w[i & 15] + small_s0(w[(i - 15) & 15]) + w[(i - 7) & 15] + small_s1(w[(i - 2) & 15]);
+            wi = add3(w[i & 15], small_s0(w[(i - 15) & 15]), w[(i - 7) & 15]) + small_s1(w[(i - 2) & 15]);
             w[i & 15] = wi;
         }
-        const uint32_t t1 = hh + big_s1(e) + ((e & f) ^ (~e & g)) + K[i] + wi;
-        const uint32_t t2 = big_s0(a) + ((a & b) ^ (a & c) ^ (b & c));
+        const uint32_t t1 = add3(hh + K[i], big_s1(e), ch(e, f, g)) + wi;
+        const uint32_t na = add3(t1, big_s0(a), maj(a, b, c));
         hh = g;
         g = f;
         f = e;
@@ -68,7 +80,7 @@ __device__ __forceinline__ void compress(uint32_t h[8], const uint32_t w_in[16])
         d = c;
         c = b;
         b = a;
-        a = t1 + t2;
+        a = na;
     }
     h[0] += a;
     h[1] += b;
