@@ -32,11 +32,28 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def _one_l3_cpus():
+    """The CPUs of this thread's allowed set that share the last-level cache
+    with the first of them (None when the topology is not readable)."""
+    allowed = sorted(os.sched_getaffinity(0))
+    try:
+        with open(f"/sys/devices/system/cpu/cpu{allowed[0]}/cache/index3/shared_cpu_list") as fh:
+            spec = fh.read().strip()
+    except (OSError, IndexError):
+        return None
+    cpus = set()
+    for part in spec.split(","):
+        a, _, b = part.partition("-")
+        cpus.update(range(int(a), int(b or a) + 1))
+    cpus &= set(allowed)
+    return sorted(cpus) or None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--log-n", type=int, default=LOG_N)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-prove measurement")
@@ -80,9 +97,16 @@ def main():
     torch.cuda.synchronize()
     ptr = d_poly.data_ptr()
 
+    # The calling thread (it drives the prove and polls for its results)
+    # stays on one last-level-cache domain while the plain proves run.
+    prev_cpus = os.sched_getaffinity(0)
+    host_cpus = _one_l3_cpus()
+    if host_cpus:
+        os.sched_setaffinity(0, host_cpus)
+        log(f"calling thread on CPUs {host_cpus[0]}-{host_cpus[-1]} (one L3) for the plain proves")
     for i in range(args.warmup):
         lh.prover_dev(cfg, ptr, FS_SEED)
-        log(f"warmup {i + 1}/{args.warmup} done")
+    log(f"{args.warmup} warmup proves done")
 
     ctx.synchronize()
     torch.cuda.synchronize()
@@ -95,6 +119,7 @@ def main():
     t1 = time.perf_counter()
     elapsed = t1 - t0
     log(f"timed {args.steps} proves in {elapsed * 1e3:.2f} ms")
+    os.sched_setaffinity(0, prev_cpus)
 
     # Instrumented proves after the timed region: HIP events per phase and
     # around each radix-2^6 FFT pass launch (events add gaps between
@@ -177,6 +202,7 @@ def main():
             "log_n": L,
             "field": "GF(2^32)",
             "path": "plain single-proof GPU drop-in",
+            "host_cpus": f"{host_cpus[0]}-{host_cpus[-1]}" if host_cpus else None,
         },
         "roofline": {
             "bound": "hbm",
