@@ -261,6 +261,8 @@ template <class H> struct PendingOpen {
     // mapping (valid until the stream is next synchronized)
     const uint64_t* d_q = nullptr;
     PBuf<typename H::T> d_rows;
+    // the row gather, launched with the induce constants
+    RowGather<typename DevField<H>::F> gather;
     size_t n_nodes = 0, max_nodes = 0, nodes_off = 0;
     uint8_t* h_rows = nullptr;
     uint8_t* h_nodes = nullptr;
@@ -268,15 +270,14 @@ template <class H> struct PendingOpen {
 
 constexpr size_t pin_round(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// The opened rows: query indices to pinned memory, the row gather launched
-// (its output feeds induce).  open_launch_nodes adds the Merkle proof.
+// The opened rows: query indices to pinned memory and the row gather set up
+// (induce_launch launches it with the induce constants; its output feeds the
+// enforced sum).  open_launch_nodes adds the Merkle proof.
 template <class H>
 void open_launch_rows(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, PendingOpen<H>& po, size_t pin_off,
                       size_t pin_end) {
-    using F = typename DevField<H>::F;
     using T = typename H::T;
     Ctx* ctx = w.ctx;
-    hipStream_t s = ctx->stream;
     po.out = &out;
     const size_t q_bytes = pin_round(q.size() * 8), rows_bytes = pin_round(q.size() * w.n * sizeof(T));
     po.max_nodes = q.size() * (size_t)ilog2(w.M);
@@ -289,11 +290,11 @@ void open_launch_rows(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, 
     po.d_q = reinterpret_cast<const uint64_t*>(ctx->pin_dev + pin_off);
     po.d_rows = PBuf<T>(&ctx->pool, q.size() * w.n);
     po.h_rows = pin + pin_off + q_bytes;
-    {
-        // the gather writes the proof's rows straight to pinned memory too
-        PhaseTimer pt(ctx, PH_OPEN);
-        launch_gather_rows<F>(w.mat.p, w.n, po.d_q, q.size(), po.d_rows.p, (T*)(ctx->pin_dev + pin_off + q_bytes), s);
-    }
+    // the gather writes the proof's rows straight to pinned memory too
+    po.gather.mat = w.mat.p;
+    po.gather.n = w.n;
+    po.gather.out = po.d_rows.p;
+    po.gather.out2 = (T*)(ctx->pin_dev + pin_off + q_bytes);
     out.n_rows = q.size();
     out.n_cols = w.n;
 }
@@ -384,10 +385,14 @@ void induce_launch(Ctx* ctx, int ell, const Vec<H>& sks, const PendingOpen<H>& p
     d_basis = PBuf<T>(&ctx->pool, (size_t)1 << ell);
     {
         PhaseTimer pt(ctx, PH_INDUCE);
-        launch_induce_prep<F>(po.d_q, d_vals, d_vals + nvals - 1, (int)nq, ell, d_c.p, d_ap.p, s);
-        launch_enforced<F>(po.d_rows.p, (int)nq, (int)n_cols, d_vals + sks.size(), d_ap.p, d_enf, s);
-        *n_enf = enforced_parts((int)nq, (int)n_cols);
-        launch_induce<F>(d_c.p, d_ap.p, (int)nq, ell, d_P.p, d_K.p, d_basis.p, s);
+        launch_induce_prep<F>(po.d_q, d_vals, d_vals + nvals - 1, (int)nq, ell, d_c.p, d_ap.p, s, &po.gather);
+        EnforcedIn<F> e;
+        e.rows = po.d_rows.p;
+        e.ncols = (int)n_cols;
+        e.gr = d_vals + sks.size();
+        e.parts = d_enf;
+        LIG_REQUIRE(nq <= (size_t)ENFORCED_MAX_PARTS, LIG_E_INVALID, "more queries than enforced-sum parts");
+        *n_enf = launch_induce<F>(d_c.p, d_ap.p, (int)nq, ell, d_P.p, d_K.p, d_basis.p, s, &e);
     }
     LIG_HIP(hipGetLastError());
 }

@@ -1434,14 +1434,29 @@ int induce_low_bits(int ell) {
 }
 
 // GF(2^32) variant: P and K by doubling, P[x + 2^t] = P[x] * c_t, with
-// the per-query tables of every c_t in LDS.
+// the per-query tables of every c_t in LDS.  With parts, workgroup i also
+// writes query i's part of the enforced sum: (row_i . gr) times ap[i],
+// unreduced (the parts XOR to sum_i ap[i] * (row_i . gr)).
 constexpr int TENSOR_MAXK = 4096;
 __global__ __launch_bounds__(256) void k_induce_tensor32(const uint32_t* __restrict__ cvals, const uint32_t* __restrict__ ap,
-                                                         int ell, int clo, uint32_t* __restrict__ P, uint32_t* __restrict__ Kt) {
+                                                         int ell, int clo, uint32_t* __restrict__ P, uint32_t* __restrict__ Kt,
+                                                         const uint32_t* __restrict__ rows, int ncols,
+                                                         const uint32_t* __restrict__ gr, uint64_t* __restrict__ parts) {
     __shared__ __attribute__((aligned(256))) uint32_t tabs[PREP_MAXELL * 128];
     __shared__ uint32_t pb[1024];
     __shared__ uint32_t kb[TENSOR_MAXK];
     const int i = blockIdx.x;
+    if (parts && threadIdx.x < 64) {
+        uint32_t d = 0u;
+        for (int j = threadIdx.x; j < ncols; j += 64) d ^= Ops<F32>::mul(rows[(size_t)i * ncols + j], gr[j]);
+#pragma unroll
+        for (int sh = 32; sh >= 1; sh >>= 1) d ^= (uint32_t)__shfl_xor((int)d, sh, 64);
+        if (threadIdx.x == 0) {
+            uint64_t acc = 0;
+            Ops<F32>::acc_mul(acc, d, ap[i]);
+            parts[i] = acc;
+        }
+    }
     const uint32_t* c = cvals + (size_t)i * ell;
     for (int idx = threadIdx.x; idx < ell * 8; idx += blockDim.x) gf32_tab_build(tabs + (idx >> 3) * 128, c[idx >> 3], idx & 7);
     if (threadIdx.x == 0) {
@@ -1465,26 +1480,36 @@ __global__ __launch_bounds__(256) void k_induce_tensor32(const uint32_t* __restr
 }
 
 template <class F>
-void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
-                   typename F::T* basis, hipStream_t s) {
+int launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
+                  typename F::T* basis, hipStream_t s, const EnforcedIn<F>* e) {
     const int clo = induce_low_bits(ell);
     size_t nk = (size_t)1 << (ell - clo);
+    int n_parts = 0;
+    bool fused = false;
     if constexpr (F::BITS == 32) {
-        if (ell <= PREP_MAXELL && nk <= (size_t)TENSOR_MAXK && clo <= 10)
-            hipLaunchKernelGGL(k_induce_tensor32, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K);
-        else
-            hipLaunchKernelGGL(k_induce_tensor<F>, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K);
-    } else {
+        if (ell <= PREP_MAXELL && nk <= (size_t)TENSOR_MAXK && clo <= 10 && (!e || nq <= ENFORCED_MAX_PARTS)) {
+            hipLaunchKernelGGL(k_induce_tensor32, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K,
+                               e ? e->rows : nullptr, e ? e->ncols : 0, e ? e->gr : nullptr, e ? e->parts : nullptr);
+            fused = true;
+            if (e) n_parts = nq;
+        }
+    }
+    if (!fused) {
+        if (e) {
+            launch_enforced<F>(e->rows, nq, e->ncols, e->gr, ap, e->parts, s);
+            n_parts = enforced_parts(nq, e->ncols);
+        }
         hipLaunchKernelGGL(k_induce_tensor<F>, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K);
     }
     if constexpr (F::BITS == 32) {
         // 1024 lanes per X row: 2^clo entries times 1024 / 2^clo query slices
         if (clo >= 6 && clo <= 10) {
             hipLaunchKernelGGL(k_induce_accum32, dim3((unsigned)nk), dim3(1024), 0, s, P, K, nq, ell, clo, basis);
-            return;
+            return n_parts;
         }
     }
     hipLaunchKernelGGL(k_induce_accum<F>, dim3((unsigned)nk), dim3(256), 0, s, P, K, nq, ell, clo, basis);
+    return n_parts;
 }
 
 template <class F>
@@ -1516,12 +1541,25 @@ __global__ void k_induce_prep(const uint64_t* __restrict__ q, const typename F::
 // every thread issues its loads first, so the kernel waits for one round
 // trip.  With at most 192 queries the fourth wave builds the alpha tables
 // and the powers alpha^i while the first three run the next_s chains.
+// Workgroups 1.. (if any) gather the opened rows instead: row q[i] of mat
+// (n per row) to out and, if given, out2.
 __global__ __launch_bounds__(256) void k_induce_prep32(const uint64_t* __restrict__ q, const uint32_t* __restrict__ sks,
                                                        const uint32_t* __restrict__ alpha, int nq, int ell,
-                                                       uint32_t* __restrict__ cvals, uint32_t* __restrict__ ap) {
+                                                       uint32_t* __restrict__ cvals, uint32_t* __restrict__ ap,
+                                                       const uint32_t* __restrict__ mat, size_t n, uint32_t* __restrict__ out,
+                                                       uint32_t* __restrict__ out2) {
     // table 0: squaring; 1..ell-1: times sks[t-1]; ell..ell+7: times alpha^(2^k)
     __shared__ __attribute__((aligned(256))) uint32_t tabs[(PREP_MAXELL + 8) * 128];
     const int tid = threadIdx.x;
+    if (blockIdx.x > 0) {
+        const size_t total = (size_t)nq * n, stride = (size_t)(gridDim.x - 1) * blockDim.x;
+        for (size_t i = (size_t)(blockIdx.x - 1) * blockDim.x + tid; i < total; i += stride) {
+            const uint32_t v = mat[q[i / n] * n + (i % n)];
+            out[i] = v;
+            if (out2) out2[i] = v;
+        }
+        return;
+    }
     const int tbl = tid >> 3, k = tid & 7;  // ell * 8 <= 208 work items
     uint32_t lam = 0u;
     if (tid >= 8 && tid < ell * 8) lam = sks[tbl - 1];
@@ -1574,13 +1612,17 @@ __global__ __launch_bounds__(256) void k_induce_prep32(const uint64_t* __restric
 
 template <class F>
 void launch_induce_prep(const uint64_t* q, const typename F::T* sks, const typename F::T* alpha, int nq, int ell,
-                        typename F::T* cvals, typename F::T* ap, hipStream_t s) {
+                        typename F::T* cvals, typename F::T* ap, hipStream_t s, const RowGather<F>* g) {
     if constexpr (F::BITS == 32) {
         if (nq <= 256 && ell <= PREP_MAXELL) {
-            hipLaunchKernelGGL(k_induce_prep32, dim3(1), dim3(256), 0, s, q, sks, alpha, nq, ell, cvals, ap);
+            // workgroup 0 the constants, the rest the row gather
+            const unsigned ng = g && nq ? (unsigned)grid_for((size_t)nq * g->n, 256) : 0u;
+            hipLaunchKernelGGL(k_induce_prep32, dim3(1 + ng), dim3(256), 0, s, q, sks, alpha, nq, ell, cvals, ap,
+                               g ? g->mat : nullptr, g ? g->n : (size_t)0, g ? g->out : nullptr, g ? g->out2 : nullptr);
             return;
         }
     }
+    if (g) launch_gather_rows<F>(g->mat, g->n, q, (size_t)nq, g->out, g->out2, s);
     hipLaunchKernelGGL(k_induce_prep<F>, dim3((nq + 63) / 64), dim3(64), 0, s, q, sks, alpha, nq, ell, cvals, ap);
 }
 
@@ -1933,7 +1975,8 @@ void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename
     template void launch_leaf_hash<F>(const F::T*, size_t, size_t, uint32_t*, uint32_t*, hipStream_t);                                 \
     template void launch_fold<F>(const F::T*, size_t, int, const F::T*, F::T*, hipStream_t);                                \
     template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, F::T*, hipStream_t);                  \
-    template void launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t);                   \
+    template int launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t,                     \
+                                  const EnforcedIn<F>*);                                                                    \
     template void launch_eval013<F>(const F::T*, const F::T*, size_t, F::Acc*, F::Acc*, hipStream_t);                       \
     template void launch_zero_acc<F>(F::Acc*, int, hipStream_t);                                                                 \
     template void launch_sumcheck_fold<F>(const F::T*, const F::T*, size_t, const F::T*, F::T*, F::T*, F::Acc*, F::Acc*,  \
@@ -1943,7 +1986,8 @@ void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename
     template void launch_fs_step<F>(DevTranscript*, const DevTranscript*, const F::Acc*, const F::Acc*, int, F::T*, F::T*,  \
                                     F::T*, int, uint32_t*, hipStream_t);                                                    \
     template void launch_glue<F>(F::T*, const F::T*, size_t, const F::T*, hipStream_t);                                           \
-    template void launch_induce_prep<F>(const uint64_t*, const F::T*, const F::T*, int, int, F::T*, F::T*, hipStream_t);   \
+    template void launch_induce_prep<F>(const uint64_t*, const F::T*, const F::T*, int, int, F::T*, F::T*, hipStream_t,    \
+                                        const RowGather<F>*);                                                               \
     template void launch_enforced<F>(const F::T*, int, int, const F::T*, const F::T*, F::Acc*, hipStream_t);
 
 LIG_INST(F32)

@@ -120,19 +120,40 @@ void launch_gather_rows(const typename F::T* mat, size_t n, const uint64_t* rows
 // The basis index splits into clo low bits (P tables, nq << clo elements)
 // and ell - clo high bits (K tables, nq << (ell - clo) elements).
 int induce_low_bits(int ell);
+// Inputs of the enforced sum of sumcheck_polys.rs:249-256: the opened rows
+// (nq x ncols), the row weights gr (ncols) and the parts buffer.
+template <class F> struct EnforcedIn {
+    const typename F::T* rows = nullptr;
+    int ncols = 0;
+    const typename F::T* gr = nullptr;
+    typename F::Acc* parts = nullptr;
+};
 // cvals[i*ell + t] = W_t(q_i); ap[i] = alpha^i.  Produces basis[2^ell].
+// With e, also the enforced sum as parts whose XOR is the sum (unreduced):
+// one per query, from the same launch, where the GF(2^32) kernel applies,
+// else enforced_parts(nq, ncols) from launch_enforced.  Returns the number
+// of parts written (0 without e).
 template <class F>
-void launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
-                   typename F::T* basis, hipStream_t s);
+int launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
+                  typename F::T* basis, hipStream_t s, const EnforcedIn<F>* e = nullptr);
 
+// The opened rows of a step: out (and out2, if given) receive the rows of
+// mat (row-major, n per row) listed in the query indices.
+template <class F> struct RowGather {
+    const typename F::T* mat = nullptr;
+    size_t n = 0;
+    typename F::T* out = nullptr;
+    typename F::T* out2 = nullptr;
+};
 // cvals[i*ell + t] = W_t(q_i) (the sks_at_x recursion, binary_fft.rs:289-292)
-// and ap[i] = alpha^i for the nq queries.
+// and ap[i] = alpha^i for the nq queries.  With g, also the row gather of
+// the same queries (one launch for both where the GF(2^32) kernel applies).
 template <class F>
 void launch_induce_prep(const uint64_t* q, const typename F::T* sks, const typename F::T* alpha, int nq, int ell,
-                        typename F::T* cvals, typename F::T* ap, hipStream_t s);
+                        typename F::T* cvals, typename F::T* ap, hipStream_t s, const RowGather<F>* g = nullptr);
 // The enforced sum of sumcheck_polys.rs:249-256, sum_i ap[i] * (row_i . gr)
 // (unreduced), as enforced_parts(nrows, ncols) parts whose XOR is the sum.
-constexpr int ENFORCED_MAX_PARTS = 64;
+constexpr int ENFORCED_MAX_PARTS = 256;
 int enforced_parts(int nrows, int ncols);
 template <class F>
 void launch_enforced(const typename F::T* rows, int nrows, int ncols, const typename F::T* gr, const typename F::T* ap,
