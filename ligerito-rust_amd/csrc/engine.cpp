@@ -525,7 +525,9 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     // n0 challenges, glue b + beta * bnew when n0 == 2), round j the step
     // after round j-1 (absorb its triple, squeeze round j's challenge); then
     // a last step absorbs the final triple.
-    auto launch_section = [&](int n0, int k, PBuf<T>* d_bnew) {
+    // last: the prove's last section, whose final triple is logged together
+    // with the completion flag of the final step
+    auto launch_section = [&](int n0, int k, PBuf<T>* d_bnew, bool last) {
         DevTranscript init;
         export_transcript(fs, init);
         *(uint32_t*)(h_fsdn + (n_tlog + n_r) * sizeof(T)) = 0;
@@ -560,8 +562,9 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             std::swap(d_b, d_b2);
             flen /= 2;
         }
-        // the last triple is absorbed with no challenge after it
-        launch_fs_step<F>(d_fs.p + fsi, nullptr, res_cur, nullptr, 0, dev_tlog + 3 * k, nullptr, nullptr, 0, dev_err, s);
+        // the last triple: only its log is needed (the next section starts
+        // from the host's exported transcript, and the host replays this one)
+        if (!last) launch_log_signal<F>(res_cur, dev_tlog + 3 * k, nullptr, 0, s);
         LIG_HIP(hipGetLastError());
         sec.n0 = n0;
         sec.k = k;
@@ -605,7 +608,7 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     }
     // SumcheckProverInstance::new (ligerito.rs:32-41) and the rounds of the
     // first step (prover.rs:96-102)
-    launch_section(1, cfg.ks[0], nullptr);
+    launch_section(1, cfg.ks[0], nullptr, cfg.recursive_steps == 1);
     open_launch_nodes<H>(w0, q, *pending);
     w0 = Witness<H>();  // the first witness is released once its kernels ran
 
@@ -617,7 +620,8 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             // written by a kernel straight into the mapped pinned buffer
             launch_copy_words((uint32_t*)(ctx->pin_dev + PIN_YR), (const uint32_t*)d_f.p, flen * (sizeof(T) / 4), s);
             const uint32_t seq = ctx->next_flag();
-            launch_signal(ctx->flag_dev(), seq, s);
+            // the last section's final triple into its log, then the flag
+            launch_log_signal<F>(res_cur, dev_tlog + 3 * sec.k, ctx->flag_dev(), seq, s);
             LIG_HIP(hipGetLastError());
             ctx->wait_flag(seq);  // yr and the section's logs are in pinned memory
             replay_section("sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
@@ -656,7 +660,7 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             launch_eval013<F>(d_f.p, d_bnew.p, flen, res_next, res_free, s);
             rotate();
         }
-        launch_section(2, cfg.ks[i + 1], &d_bnew);
+        launch_section(2, cfg.ks[i + 1], &d_bnew, i + 2 == cfg.recursive_steps);
         open_launch_nodes<H>(wprev, q, *next);
         open_finish<H>(*pending);  // the previous opening, while this section runs
         pending = std::move(next);

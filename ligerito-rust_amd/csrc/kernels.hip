@@ -1953,6 +1953,25 @@ __global__ __launch_bounds__(64) void k_fs_step(DevTranscript* __restrict__ st, 
     }
 }
 
+// Thread 0: the reduced triple at res into tlog (if res); then, if flag,
+// seq at flag (mapped host memory, system scope) after everything before.
+template <class F>
+__global__ void k_log_signal(const typename F::Acc* __restrict__ res, typename F::T* __restrict__ tlog, uint32_t* flag,
+                             uint32_t seq) {
+    if (threadIdx.x != 0) return;
+    if (res)
+        for (int k = 0; k < 3; k++) tlog[k] = Ops<F>::acc_reduce(res[k]);
+    if (flag) {
+        __threadfence_system();
+        __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+template <class F>
+void launch_log_signal(const typename F::Acc* res, typename F::T* tlog, uint32_t* flag, uint32_t seq, hipStream_t s) {
+    hipLaunchKernelGGL(k_log_signal<F>, dim3(1), dim3(64), 0, s, res, tlog, flag, seq);
+}
+
 template <class F>
 void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename F::Acc* res, const typename F::Acc* enf,
                     int n_enf, typename F::T* tlog, typename F::T* rout, typename F::T* rout2, int n_squeeze,
@@ -1986,6 +2005,7 @@ void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename
     template void launch_fs_step<F>(DevTranscript*, const DevTranscript*, const F::Acc*, const F::Acc*, int, F::T*, F::T*,  \
                                     F::T*, int, uint32_t*, hipStream_t);                                                    \
     template void launch_glue<F>(F::T*, const F::T*, size_t, const F::T*, hipStream_t);                                           \
+    template void launch_log_signal<F>(const F::Acc*, F::T*, uint32_t*, uint32_t, hipStream_t);                             \
     template void launch_induce_prep<F>(const uint64_t*, const F::T*, const F::T*, int, int, F::T*, F::T*, hipStream_t,    \
                                         const RowGather<F>*);                                                               \
     template void launch_enforced<F>(const F::T*, int, int, const F::T*, const F::T*, F::Acc*, hipStream_t);

@@ -95,6 +95,10 @@ void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32
                           uint32_t* flag = nullptr, uint32_t seq = 0);
 // Stores seq at flag (mapped host memory) after the stream's earlier kernels.
 void launch_signal(uint32_t* flag, uint32_t seq, hipStream_t s);
+// One thread: the reduced triple at res (unreduced accumulators) into tlog
+// if res is given, then seq at flag if flag is given.
+template <class F>
+void launch_log_signal(const typename F::Acc* res, typename F::T* tlog, uint32_t* flag, uint32_t seq, hipStream_t s);
 // out[i] = node at byte offset offs[i] of the tree (32 bytes each).
 void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t count, uint32_t* out, hipStream_t s);
 
