@@ -362,7 +362,7 @@ void export_transcript(const Transcript& fs, DevTranscript& d) {
 template <class H>
 void induce_launch(Ctx* ctx, int ell, const Vec<H>& sks, const PendingOpen<H>& po, size_t n_cols, const Vec<H>& gr,
                    const std::vector<size_t>& q, typename H::T alpha, PBuf<typename H::T>& d_basis,
-                   typename DevField<H>::F::Acc* d_enf, int* n_enf, size_t pin_off) {
+                   typename DevField<H>::F::Acc* d_enf, int* n_enf, size_t pin_off, const Eval013In<typename DevField<H>::F>& ev) {
     using F = typename DevField<H>::F;
     using T = typename H::T;
     hipStream_t s = ctx->stream;
@@ -392,7 +392,7 @@ void induce_launch(Ctx* ctx, int ell, const Vec<H>& sks, const PendingOpen<H>& p
         e.gr = d_vals + sks.size();
         e.parts = d_enf;
         LIG_REQUIRE(nq <= (size_t)ENFORCED_MAX_PARTS, LIG_E_INVALID, "more queries than enforced-sum parts");
-        *n_enf = launch_induce<F>(d_c.p, d_ap.p, (int)nq, ell, d_P.p, d_K.p, d_basis.p, s, &e);
+        *n_enf = launch_induce<F>(d_c.p, d_ap.p, (int)nq, ell, d_P.p, d_K.p, d_basis.p, s, &e, &ev);
     }
     LIG_HIP(hipGetLastError());
 }
@@ -600,10 +600,13 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     // pending openings point into rec_proofs until they are copied out
     proof.rec_proofs.reserve(cfg.recursive_steps);
     open_launch_rows<H>(w0, q, proof.initial, *pending, pin_open[0], pin_open[1]);
-    induce_launch<H>(ctx, nv, cfg.sks_by_ell.at(nv), *pending, w0.n, gr0, q, alpha, d_b, d_enf, &n_enf, PIN_UP);
     {
-        PhaseTimer pt(ctx, PH_SUMCHECK);
-        launch_eval013<F>(d_f.p, d_b.p, flen, res_next, res_free, s);
+        // the basis, and with it the 0/1/3 triple of (f, basis)
+        Eval013In<F> ev;
+        ev.f = d_f.p;
+        ev.res = res_next;
+        ev.res_clear = res_free;
+        induce_launch<H>(ctx, nv, cfg.sks_by_ell.at(nv), *pending, w0.n, gr0, q, alpha, d_b, d_enf, &n_enf, PIN_UP, ev);
         rotate();
     }
     // SumcheckProverInstance::new (ligerito.rs:32-41) and the rounds of the
@@ -650,14 +653,17 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         open_slot ^= 1;
         open_launch_rows<H>(wprev, q, proof.rec_proofs.back(), *next, pin_open[open_slot], pin_open[open_slot + 1]);
         Vec<H> gr = lagrange_basis<H>(fold_rs);
-        induce_launch<H>(ctx, nv, cfg.sks_by_ell.at(nv), *next, wprev.n, gr, q, a, d_bnew, d_enf, &n_enf, PIN_UP);
-        // introduce_new (ligerito.rs:50-57): triple of f with the new basis
-        // alone; then glue (ligerito.rs:59-67) with the next challenge: the
-        // summed basis absorbs beta * b_new (folds and the 0/1/3 evaluations
-        // are linear in the basis polys); then the rounds of step i+1
+        // introduce_new (ligerito.rs:50-57): the new basis and the triple of
+        // f with it alone (from the same launches); then glue
+        // (ligerito.rs:59-67) with the next challenge: the summed basis
+        // absorbs beta * b_new (folds and the 0/1/3 evaluations are linear in
+        // the basis polys); then the rounds of step i+1
         {
-            PhaseTimer pt(ctx, PH_SUMCHECK);
-            launch_eval013<F>(d_f.p, d_bnew.p, flen, res_next, res_free, s);
+            Eval013In<F> ev;
+            ev.f = d_f.p;
+            ev.res = res_next;
+            ev.res_clear = res_free;
+            induce_launch<H>(ctx, nv, cfg.sks_by_ell.at(nv), *next, wprev.n, gr, q, a, d_bnew, d_enf, &n_enf, PIN_UP, ev);
             rotate();
         }
         launch_section(2, cfg.ks[i + 1], &d_bnew, i + 2 == cfg.recursive_steps);

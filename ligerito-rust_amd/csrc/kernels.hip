@@ -1392,9 +1392,20 @@ __global__ __launch_bounds__(256) void k_induce_accum(const typename F::T* __res
 // slices, so narrow X rows still fill the workgroup); the QS partial sums of
 // an entry are XORed through LDS and the entry is written once (no atomics,
 // no zeroing).  More queries than ACC_QMAX are taken in chunks.
+//
+// With f, the kernel also adds the triple of eval_013_product(f, basis)
+// into res (and zeroes res_clear): entry j of the basis pairs with j +- h
+// (h = 2^(ell-1)), which another workgroup holds, but the triple is
+// bilinear, so the holder of b0 = basis[j] (j < h) adds f0*b0 to s0 and
+// a2*(2 b0) to s2, the holder of b1 = basis[j] (j >= h) adds f1*b1 to s1
+// and a2*(3 b1) to s2, with a2 = 3 f1 + 2 f0 loaded from f; summed over
+// both, s2 gets a2*(3 b1 + 2 b0) as in eval_013_product.
+template <class F> __device__ inline void block_xor3(typename F::Acc a0, typename F::Acc a1, typename F::Acc a2,
+                                                     typename F::Acc* res);
 constexpr int ACC_QMAX = 160;
 __global__ __launch_bounds__(1024) void k_induce_accum32(const uint32_t* __restrict__ P, const uint32_t* __restrict__ Kt, int nq,
-                                                         int ell, int clo, uint32_t* __restrict__ basis) {
+                                                         int ell, int clo, uint32_t* __restrict__ basis,
+                                                         const uint32_t* __restrict__ f, uint64_t* res, uint64_t* res_clear) {
     __shared__ __attribute__((aligned(256))) uint32_t tabs[ACC_QMAX * 128];
     __shared__ uint32_t part[1024];
     const size_t np = (size_t)1 << clo, nk = (size_t)1 << (ell - clo);
@@ -1413,17 +1424,35 @@ __global__ __launch_bounds__(1024) void k_induce_accum32(const uint32_t* __restr
 #pragma unroll 8
         for (int i = sl; i < qn; i += qs) acc ^= mul_tab_lds(tabs, (uint32_t)(i * 512), Pq[(size_t)i * np]);
     }
-    if (qs == 1) {
-        basis[X * np + x] = acc;
-        return;
+    uint32_t b = acc;
+    if (qs > 1) {
+        part[threadIdx.x] = acc;
+        LIG_LDS_BARRIER();
+        if (sl == 0)
+            for (int k = 1; k < qs; k++) b ^= part[k * np + x];
     }
-    part[threadIdx.x] = acc;
-    LIG_LDS_BARRIER();
+    const size_t j = X * np + x;
+    if (sl == 0) basis[j] = b;
+    if (!f) return;
+    // this entry's share of the triple (see above); X < nk / 2 is uniform
+    uint64_t s0 = 0, s1 = 0, s2 = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 3) res_clear[threadIdx.x] = 0;
     if (sl == 0) {
-        uint32_t t = acc;
-        for (int k = 1; k < qs; k++) t ^= part[k * np + x];
-        basis[X * np + x] = t;
+        const size_t h = (nk * np) / 2;
+        const uint32_t fj = f[j];
+        if (X < nk / 2) {
+            const uint32_t f1 = f[j + h];
+            const uint32_t a2 = Ops<F32>::mul2(f1) ^ f1 ^ Ops<F32>::mul2(fj);
+            Ops<F32>::acc_mul(s0, fj, b);
+            Ops<F32>::acc_mul(s2, a2, Ops<F32>::mul2(b));
+        } else {
+            const uint32_t f0 = f[j - h];
+            const uint32_t a2 = Ops<F32>::mul2(fj) ^ fj ^ Ops<F32>::mul2(f0);
+            Ops<F32>::acc_mul(s1, fj, b);
+            Ops<F32>::acc_mul(s2, a2, Ops<F32>::mul2(b) ^ b);
+        }
     }
+    block_xor3<F32>(s0, s1, s2, res);
 }
 
 int induce_low_bits(int ell) {
@@ -1481,7 +1510,7 @@ __global__ __launch_bounds__(256) void k_induce_tensor32(const uint32_t* __restr
 
 template <class F>
 int launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
-                  typename F::T* basis, hipStream_t s, const EnforcedIn<F>* e) {
+                  typename F::T* basis, hipStream_t s, const EnforcedIn<F>* e, const Eval013In<F>* ev) {
     const int clo = induce_low_bits(ell);
     size_t nk = (size_t)1 << (ell - clo);
     int n_parts = 0;
@@ -1503,12 +1532,14 @@ int launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, i
     }
     if constexpr (F::BITS == 32) {
         // 1024 lanes per X row: 2^clo entries times 1024 / 2^clo query slices
-        if (clo >= 6 && clo <= 10) {
-            hipLaunchKernelGGL(k_induce_accum32, dim3((unsigned)nk), dim3(1024), 0, s, P, K, nq, ell, clo, basis);
+        if (clo >= 6 && clo <= 10 && (!ev || nk >= 2)) {  // the triple pairs rows X and X + nk/2
+            hipLaunchKernelGGL(k_induce_accum32, dim3((unsigned)nk), dim3(1024), 0, s, P, K, nq, ell, clo, basis,
+                               ev ? ev->f : nullptr, ev ? ev->res : nullptr, ev ? ev->res_clear : nullptr);
             return n_parts;
         }
     }
     hipLaunchKernelGGL(k_induce_accum<F>, dim3((unsigned)nk), dim3(256), 0, s, P, K, nq, ell, clo, basis);
+    if (ev) launch_eval013<F>(ev->f, basis, (size_t)1 << ell, ev->res, ev->res_clear, s);
     return n_parts;
 }
 
@@ -1995,7 +2026,7 @@ void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename
     template void launch_fold<F>(const F::T*, size_t, int, const F::T*, F::T*, hipStream_t);                                \
     template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, F::T*, hipStream_t);                  \
     template int launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t,                     \
-                                  const EnforcedIn<F>*);                                                                    \
+                                  const EnforcedIn<F>*, const Eval013In<F>*);                                               \
     template void launch_eval013<F>(const F::T*, const F::T*, size_t, F::Acc*, F::Acc*, hipStream_t);                       \
     template void launch_zero_acc<F>(F::Acc*, int, hipStream_t);                                                                 \
     template void launch_sumcheck_fold<F>(const F::T*, const F::T*, size_t, const F::T*, F::T*, F::T*, F::Acc*, F::Acc*,  \

@@ -132,14 +132,23 @@ template <class F> struct EnforcedIn {
     const typename F::T* gr = nullptr;
     typename F::Acc* parts = nullptr;
 };
+// The triple of eval_013_product(f, basis) (as launch_eval013: added into
+// res, res_clear zeroed), f of the basis length.
+template <class F> struct Eval013In {
+    const typename F::T* f = nullptr;
+    typename F::Acc* res = nullptr;
+    typename F::Acc* res_clear = nullptr;
+};
 // cvals[i*ell + t] = W_t(q_i); ap[i] = alpha^i.  Produces basis[2^ell].
 // With e, also the enforced sum as parts whose XOR is the sum (unreduced):
 // one per query, from the same launch, where the GF(2^32) kernel applies,
-// else enforced_parts(nq, ncols) from launch_enforced.  Returns the number
-// of parts written (0 without e).
+// else enforced_parts(nq, ncols) from launch_enforced.  With ev, also the
+// 0/1/3 triple of (f, basis), inside the accumulation where the GF(2^32)
+// kernel applies, else by launch_eval013.  Returns the number of enforced
+// parts written (0 without e).
 template <class F>
 int launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
-                  typename F::T* basis, hipStream_t s, const EnforcedIn<F>* e = nullptr);
+                  typename F::T* basis, hipStream_t s, const EnforcedIn<F>* e = nullptr, const Eval013In<F>* ev = nullptr);
 
 // The opened rows of a step: out (and out2, if given) receive the rows of
 // mat (row-major, n per row) listed in the query indices.
