@@ -57,7 +57,7 @@ __constant__ static const uint32_t K[64] = {
     0x748f82eeu, 0x78a5636fu, 0x84c87814u, 0x8cc70208u, 0x90befffau, 0xa4506cebu, 0xbef9a3f7u, 0xc67178f2u};
 
 // One compression of the big-endian message words w[16] into h[8].
-__device__ __forceinline__ void compress(uint32_t h[8], const uint32_t w_in[16]) {
+__device__ __forceinline__ void compress_inline(uint32_t h[8], const uint32_t w_in[16]) {
     uint32_t w[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) w[i] = w_in[i];
@@ -90,6 +90,35 @@ __device__ __forceinline__ void compress(uint32_t h[8], const uint32_t w_in[16])
     h[5] += f;
     h[6] += g;
     h[7] += hh;
+}
+
+// The compression as one shared out-of-line function: a transcript step
+// compresses at several sites (an absorbed block, a squeeze's counter, one
+// or two finalization blocks), and inlined copies are each fetched cold the
+// first time a launch reaches them (a section's leading step takes paths
+// the other rounds do not).  Scalars in and out keep every value in
+// registers under the calling convention.
+struct Sha8 {
+    uint32_t v[8];
+};
+__device__ __noinline__ Sha8 compress_shared(uint32_t h0, uint32_t h1, uint32_t h2, uint32_t h3, uint32_t h4, uint32_t h5,
+                                             uint32_t h6, uint32_t h7, uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3,
+                                             uint32_t w4, uint32_t w5, uint32_t w6, uint32_t w7, uint32_t w8, uint32_t w9,
+                                             uint32_t w10, uint32_t w11, uint32_t w12, uint32_t w13, uint32_t w14,
+                                             uint32_t w15) {
+    uint32_t h[8] = {h0, h1, h2, h3, h4, h5, h6, h7};
+    const uint32_t w[16] = {w0, w1, w2, w3, w4, w5, w6, w7, w8, w9, w10, w11, w12, w13, w14, w15};
+    compress_inline(h, w);
+    Sha8 r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = h[i];
+    return r;
+}
+__device__ __forceinline__ void compress(uint32_t h[8], const uint32_t w[16]) {
+    const Sha8 r = compress_shared(h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7], w[0], w[1], w[2], w[3], w[4], w[5], w[6],
+                                   w[7], w[8], w[9], w[10], w[11], w[12], w[13], w[14], w[15]);
+#pragma unroll
+    for (int i = 0; i < 8; i++) h[i] = r.v[i];
 }
 
 // Working copy of a DevTranscript in registers.
