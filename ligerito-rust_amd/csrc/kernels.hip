@@ -1797,6 +1797,15 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
     if constexpr (STEP) {
         // the preceding transcript step, computed by wave 0 of every workgroup
         __shared__ T r_sh[2];
+        if (!step.st_in && threadIdx.x >= 64 && threadIdx.x < 128) {
+            // a section's leading step: wave 1, idle until the challenge is
+            // out, runs one compression on dummy words first, so the shared
+            // compression code is in this CU's instruction cache by the time
+            // wave 0 (still loading the state and triple) reaches it
+            const dfs::Sha8 d = dfs::compress_shared(threadIdx.x, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+                                                      0, 0, 0);
+            asm volatile("" ::"v"(d.v[0]));
+        }
         if (threadIdx.x < 64) {
             dfs::State st;
             if (step.st_in)
