@@ -28,9 +28,9 @@ template <> struct Ops<F32> {
         T r3 = t[96 + ((w >> 24) & 15u)] ^ t[112 + (w >> 28)];
         return (r0 ^ r1) ^ (r2 ^ r3);
     }
-    __device__ static T mul(T a, T b) { return gf32_reduce64(clmul32_portable(a, b)); }
+    __device__ static T mul(T a, T b) { return gf32_reduce64(clmul32_holes(a, b)); }
     __device__ static Acc acc_zero() { return 0ull; }
-    __device__ static void acc_mul(Acc& acc, T a, T b) { acc ^= clmul32_portable(a, b); }
+    __device__ static void acc_mul(Acc& acc, T a, T b) { acc ^= clmul32_holes(a, b); }
     __device__ static T acc_reduce(Acc a) { return gf32_reduce64(a); }
     __device__ static bool eq(T a, T b) { return a == b; }
     // little-endian bytes as 32-bit words (BinaryField::to_bytes)

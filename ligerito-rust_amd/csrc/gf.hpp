@@ -49,6 +49,24 @@ LIG_UNROLL
     return acc;
 }
 
+// Carry-less 32x32 -> 64 from integer multiplies ("multiplication with
+// holes", as in BearSSL's bmul32): each operand splits into the bits of
+// four residues mod 4; an integer product of two such parts has at most 8
+// terms at any position, so its carries reach at most 3 positions up and
+// never the next position of the same residue, which the final masks keep.
+// 16 integer products instead of 32 shift-and-mask steps.
+LIG_HD uint64_t clmul32_holes(uint32_t a, uint32_t b) {
+    const uint32_t a0 = a & 0x11111111u, a1 = a & 0x22222222u, a2 = a & 0x44444444u, a3 = a & 0x88888888u;
+    const uint32_t b0 = b & 0x11111111u, b1 = b & 0x22222222u, b2 = b & 0x44444444u, b3 = b & 0x88888888u;
+    const uint64_t A0 = a0, A1 = a1, A2 = a2, A3 = a3;
+    const uint64_t z0 = (A0 * b0) ^ (A1 * b3) ^ (A2 * b2) ^ (A3 * b1);
+    const uint64_t z1 = (A0 * b1) ^ (A1 * b0) ^ (A2 * b3) ^ (A3 * b2);
+    const uint64_t z2 = (A0 * b2) ^ (A1 * b1) ^ (A2 * b0) ^ (A3 * b3);
+    const uint64_t z3 = (A0 * b3) ^ (A1 * b2) ^ (A2 * b1) ^ (A3 * b0);
+    return (z0 & 0x1111111111111111ull) | (z1 & 0x2222222222222222ull) | (z2 & 0x4444444444444444ull) |
+           (z3 & 0x8888888888888888ull);
+}
+
 // lambda * v * x^(4k) for a 4-bit v: one entry of a multiply-by-lambda table.
 LIG_HD uint32_t gf32_table_entry(uint32_t lambda, int k, uint32_t v) {
     uint64_t p = 0, l = lambda;
