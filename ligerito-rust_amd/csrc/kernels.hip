@@ -1467,6 +1467,7 @@ int induce_low_bits(int ell) {
 // writes query i's part of the enforced sum: (row_i . gr) times ap[i],
 // unreduced (the parts XOR to sum_i ap[i] * (row_i . gr)).
 constexpr int TENSOR_MAXK = 4096;
+__device__ __forceinline__ uint32_t wave_xor32(uint32_t x);
 __global__ __launch_bounds__(256) void k_induce_tensor32(const uint32_t* __restrict__ cvals, const uint32_t* __restrict__ ap,
                                                          int ell, int clo, uint32_t* __restrict__ P, uint32_t* __restrict__ Kt,
                                                          const uint32_t* __restrict__ rows, int ncols,
@@ -1478,8 +1479,7 @@ __global__ __launch_bounds__(256) void k_induce_tensor32(const uint32_t* __restr
     if (parts && threadIdx.x < 64) {
         uint32_t d = 0u;
         for (int j = threadIdx.x; j < ncols; j += 64) d ^= Ops<F32>::mul(rows[(size_t)i * ncols + j], gr[j]);
-#pragma unroll
-        for (int sh = 32; sh >= 1; sh >>= 1) d ^= (uint32_t)__shfl_xor((int)d, sh, 64);
+        d = wave_xor32(d);
         if (threadIdx.x == 0) {
             uint64_t acc = 0;
             Ops<F32>::acc_mul(acc, d, ap[i]);
@@ -1694,11 +1694,17 @@ void launch_enforced(const typename F::T* rows, int nrows, int ncols, const type
 }
 
 // -------------------------------------------------------------- sumcheck ---
-// XOR over the 64 lanes of a wave (every lane gets the total)
+// XOR over the 64 lanes of a wave (every lane gets the total; all 64 lanes
+// active): DPP within each row of 16 (swap neighbours, swap pairs, mirror
+// halves of 8, mirror the row), then the four row totals read from lanes
+// 0, 16, 32 and 48.
 __device__ __forceinline__ uint32_t wave_xor32(uint32_t x) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) x ^= (uint32_t)__shfl_xor((int)x, d, 64);
-    return x;
+    x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true);   // quad_perm [1,0,3,2]
+    x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, true);   // quad_perm [2,3,0,1]
+    x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x141, 0xF, 0xF, true);  // row_half_mirror
+    x ^= (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x140, 0xF, 0xF, true);  // row_mirror
+    return (uint32_t)(__builtin_amdgcn_readlane((int)x, 0) ^ __builtin_amdgcn_readlane((int)x, 16) ^
+                      __builtin_amdgcn_readlane((int)x, 32) ^ __builtin_amdgcn_readlane((int)x, 48));
 }
 __device__ __forceinline__ uint64_t wave_xor(uint64_t x) {
     return (uint64_t)wave_xor32((uint32_t)x) | ((uint64_t)wave_xor32((uint32_t)(x >> 32)) << 32);

@@ -114,6 +114,23 @@ def test_ligero_commit_matches_oracle(O, ctx, bits, m, n):
         assert np.array_equal(w.extract_row(r), mat.reshape(n, 4 * m, -1)[:, r].reshape(w.extract_row(r).shape))
 
 
+def test_commit_tree_2e19_leaves(O, ctx):
+    # 2^19 leaves: two single-level launches to 2^17 nodes, then subtree
+    # launches of depth 7 and the one-workgroup top
+    m, n = 1 << 17, 4
+    say("commit tree top", m, n)
+    rs = lh.ReedSolomonEncoding(ctx, m, 4 * m, 32)
+    poly = col(O, 21, m * n, 32)
+    w = lh.ligero_commit(poly, m, n, rs)
+    root, _, layers = O.ligero_commit(poly, m, n, 32, want_layers=True)
+    assert w.root() == root
+    assert w.tree() == layers
+    # a second commit on the same context
+    w2 = lh.ligero_commit(poly[::-1].copy(), m, n, rs)
+    root2, _, _ = O.ligero_commit(poly[::-1].copy(), m, n, 32)
+    assert w2.root() == root2
+
+
 def test_commit_encode_kernel_shape_gf32(O, ctx):
     # 4096 independent length-2^12 rows encoded to 2^14, 16 KiB leaves
     m, n = 1 << 12, 4096
