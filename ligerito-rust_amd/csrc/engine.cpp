@@ -301,7 +301,8 @@ void open_launch_rows(Witness<H>& w, const std::vector<size_t>& q, Rlp<H>& out, 
 
 // The Merkle proof of the opened rows (merkle_tree.rs:98-111): node indices
 // on the host, the gather of the nodes into pinned memory on the stream.
-template <class H> void open_launch_nodes(Witness<H>& w, const std::vector<size_t>& q, PendingOpen<H>& po) {
+template <class H>
+void open_launch_nodes(Witness<H>& w, const std::vector<size_t>& q, PendingOpen<H>& po, const WordCopy* extra = nullptr) {
     Ctx* ctx = w.ctx;
     std::vector<uint64_t> nodes = merkle_proof_nodes(w.M, q);
     LIG_REQUIRE(nodes.size() <= po.max_nodes, LIG_E_ASSERT, "Merkle proof needs more nodes than queries x depth");
@@ -311,7 +312,7 @@ template <class H> void open_launch_nodes(Witness<H>& w, const std::vector<size_
     po.h_nodes = ctx->pin + nodes_at;
     PhaseTimer pt(ctx, PH_OPEN);
     launch_gather_nodes(w.tree.p, reinterpret_cast<const uint64_t*>(ctx->pin_dev + po.nodes_off), nodes.size(),
-                        (uint32_t*)(ctx->pin_dev + nodes_at), ctx->stream);
+                        (uint32_t*)(ctx->pin_dev + nodes_at), ctx->stream, extra);
 }
 
 // An opening read on the host from a copy of the witness: the matrix
@@ -430,29 +431,6 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     for (auto& r : pe0) r = fs.get_field<H>();
     Vec<H> gr0 = lagrange_basis<H>(pe0);
 
-    // partial_eval over the top initial_k variables == row combination with gr
-    const size_t N = cfg.initial_m * cfg.initial_n;
-    size_t flen = N >> cfg.initial_k;
-    PBuf<T> d_f(&ctx->pool, flen), d_f2(&ctx->pool, flen / 2);
-    bool folded = false;
-    if constexpr (F::BITS == 32) {
-        // the weights travel as a kernel argument
-        PhaseTimer pt(ctx, PH_FOLD);
-        folded = launch_fold32_weights(reinterpret_cast<const uint32_t*>(d_poly), flen, cfg.initial_k,
-                                       reinterpret_cast<const uint32_t*>(gr0.data()), reinterpret_cast<uint32_t*>(d_f.p), s);
-        LIG_HIP(hipGetLastError());
-    }
-    if (!folded) {
-        T* h = (T*)(ctx->pinned(PIN_UP + gr0.size() * sizeof(T)) + PIN_UP);
-        memcpy(h, gr0.data(), gr0.size() * sizeof(T));
-        PBuf<T> d_gr(&ctx->pool, gr0.size()), d_grt(&ctx->pool, gr0.size() * F::TAB);
-        LIG_HIP(hipMemcpyAsync(d_gr.p, h, gr0.size() * sizeof(T), hipMemcpyHostToDevice, s));
-        PhaseTimer pt(ctx, PH_FOLD);
-        launch_build_tables<F>(d_gr.p, gr0.size(), d_grt.p, s);
-        launch_fold<F>(d_poly, flen, cfg.initial_k, d_grt.p, d_f.p, s);
-        LIG_HIP(hipGetLastError());
-    }
-
     // Sumcheck sections run on the device transcript: the host hands its
     // transcript over after the section's host-side steps, a chain of
     // kernels absorbs each triple and squeezes each challenge, and after the
@@ -482,9 +460,35 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         res_free = t;
     };
     int n_enf = 0;
-    // res_next and res_free are adjacent: one launch, queued ahead of the
-    // recursive commit rather than after the synchronisation that ends it
-    launch_zero_acc<F>(res_next, 6, s);
+    // res_next and res_free are adjacent; the fold below zeroes them (or a
+    // launch of its own), ahead of the recursive commit rather than after the
+    // synchronisation that ends it
+
+    // partial_eval over the top initial_k variables == row combination with gr
+    const size_t N = cfg.initial_m * cfg.initial_n;
+    size_t flen = N >> cfg.initial_k;
+    PBuf<T> d_f(&ctx->pool, flen), d_f2(&ctx->pool, flen / 2);
+    bool folded = false;
+    if constexpr (F::BITS == 32) {
+        // the weights travel as a kernel argument
+        PhaseTimer pt(ctx, PH_FOLD);
+        folded = launch_fold32_weights(reinterpret_cast<const uint32_t*>(d_poly), flen, cfg.initial_k,
+                                       reinterpret_cast<const uint32_t*>(gr0.data()), reinterpret_cast<uint32_t*>(d_f.p), s,
+                                       reinterpret_cast<uint64_t*>(res_next), 6);
+        LIG_HIP(hipGetLastError());
+    }
+    if (!folded) {
+        T* h = (T*)(ctx->pinned(PIN_UP + gr0.size() * sizeof(T)) + PIN_UP);
+        memcpy(h, gr0.data(), gr0.size() * sizeof(T));
+        PBuf<T> d_gr(&ctx->pool, gr0.size()), d_grt(&ctx->pool, gr0.size() * F::TAB);
+        LIG_HIP(hipMemcpyAsync(d_gr.p, h, gr0.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        PhaseTimer pt(ctx, PH_FOLD);
+        launch_build_tables<F>(d_gr.p, gr0.size(), d_grt.p, s);
+        launch_fold<F>(d_poly, flen, cfg.initial_k, d_grt.p, d_f.p, s);
+        launch_zero_acc<F>(res_next, 6, s);
+        LIG_HIP(hipGetLastError());
+    }
+
 
     // The last witness is copied to pinned memory on the side stream while
     // the last sumcheck section runs; the final openings then need no GPU
@@ -519,15 +523,18 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     T* dev_tlog = (T*)(ctx->pin_dev + PIN_FSDN);
     T* dev_rlog = dev_tlog + n_tlog;
     uint32_t* dev_err = (uint32_t*)(dev_rlog + n_r);
+    // a section's last triple, as raw accumulators (the host reduces them)
+    constexpr size_t LAST_OFF = 8192;
+    LIG_REQUIRE((n_tlog + n_r) * sizeof(T) + 4 <= LAST_OFF, LIG_E_INVALID, "section logs exceed their staging region");
+    const A* h_last = (const A*)(ctx->pin + PIN_FSDN + LAST_OFF);
+    A* dev_last = (A*)(ctx->pin_dev + PIN_FSDN + LAST_OFF);
     // Launches: k fold rounds, each running the transcript step before it:
     // round 0 the section's leading step (absorb the triple in res_cur from
     // the host's exported state, check it against the enforced sum, squeeze
     // n0 challenges, glue b + beta * bnew when n0 == 2), round j the step
     // after round j-1 (absorb its triple, squeeze round j's challenge); then
     // a last step absorbs the final triple.
-    // last: the prove's last section, whose final triple is logged together
-    // with the completion flag of the final step
-    auto launch_section = [&](int n0, int k, PBuf<T>* d_bnew, bool last) {
+    auto launch_section = [&](int n0, int k, PBuf<T>* d_bnew) {
         DevTranscript init;
         export_transcript(fs, init);
         *(uint32_t*)(h_fsdn + (n_tlog + n_r) * sizeof(T)) = 0;
@@ -562,12 +569,20 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             std::swap(d_b, d_b2);
             flen /= 2;
         }
-        // the last triple: only its log is needed (the next section starts
-        // from the host's exported transcript, and the host replays this one)
-        if (!last) launch_log_signal<F>(res_cur, dev_tlog + 3 * k, nullptr, 0, s);
         LIG_HIP(hipGetLastError());
         sec.n0 = n0;
         sec.k = k;
+    };
+    // The section's last triple needs no transcript step on the device (the
+    // next section starts from the host's exported transcript, and the host
+    // replays this one): its raw accumulators ride with the node gather
+    // launched right after the section.
+    auto last_triple_copy = [&]() {
+        WordCopy c;
+        c.src = reinterpret_cast<const uint32_t*>(res_cur);
+        c.dst = reinterpret_cast<uint32_t*>(dev_last);
+        c.n = (int)(3 * sizeof(A) / 4);
+        return c;
     };
     Vec<H> fold_rs;  // challenges of the section's fold rounds
     // after the stream has been synchronized
@@ -588,6 +603,11 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         for (int i = 0; i < sec.n0; i++) squeeze_check();
         for (int j = 0; j < sec.k; j++) {
             const T* t3 = tl + 3 * (j + 1);
+            T last3[3];
+            if (j + 1 == sec.k) {
+                for (int i = 0; i < 3; i++) last3[i] = acc_reduce_host<H>(h_last[i]);
+                t3 = last3;
+            }
             proof.tr.insert(proof.tr.end(), t3, t3 + 3);
             fs.absorb_elems<H>(t3, 3);
             if (j + 1 < sec.k) squeeze_check();
@@ -611,8 +631,11 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     }
     // SumcheckProverInstance::new (ligerito.rs:32-41) and the rounds of the
     // first step (prover.rs:96-102)
-    launch_section(1, cfg.ks[0], nullptr, cfg.recursive_steps == 1);
-    open_launch_nodes<H>(w0, q, *pending);
+    launch_section(1, cfg.ks[0], nullptr);
+    {
+        const WordCopy lc = last_triple_copy();
+        open_launch_nodes<H>(w0, q, *pending, &lc);
+    }
     w0 = Witness<H>();  // the first witness is released once its kernels ran
 
     for (int i = 0; i < cfg.recursive_steps; i++) {
@@ -623,8 +646,7 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             // written by a kernel straight into the mapped pinned buffer
             launch_copy_words((uint32_t*)(ctx->pin_dev + PIN_YR), (const uint32_t*)d_f.p, flen * (sizeof(T) / 4), s);
             const uint32_t seq = ctx->next_flag();
-            // the last section's final triple into its log, then the flag
-            launch_log_signal<F>(res_cur, dev_tlog + 3 * sec.k, ctx->flag_dev(), seq, s);
+            launch_signal(ctx->flag_dev(), seq, s);
             LIG_HIP(hipGetLastError());
             ctx->wait_flag(seq);  // yr and the section's logs are in pinned memory
             replay_section("sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
@@ -666,8 +688,11 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             induce_launch<H>(ctx, nv, cfg.sks_by_ell.at(nv), *next, wprev.n, gr, q, a, d_bnew, d_enf, &n_enf, PIN_UP, ev);
             rotate();
         }
-        launch_section(2, cfg.ks[i + 1], &d_bnew, i + 2 == cfg.recursive_steps);
-        open_launch_nodes<H>(wprev, q, *next);
+        launch_section(2, cfg.ks[i + 1], &d_bnew);
+        {
+            const WordCopy lc = last_triple_copy();
+            open_launch_nodes<H>(wprev, q, *next, &lc);
+        }
         open_finish<H>(*pending);  // the previous opening, while this section runs
         pending = std::move(next);
         wprev = std::move(wnext);

@@ -1203,17 +1203,21 @@ void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32
     }
 }
 
-// One thread stores seq at flag (host memory, system scope) once the
-// kernels before it on the stream are done.
-__global__ void k_signal(uint32_t* flag, uint32_t seq) {
-    if (threadIdx.x == 0) {
+// One wave: the first extra.n lanes copy extra words, then lane 0 stores
+// seq at flag (host memory, system scope) once the kernels before it on the
+// stream and those copies are done.
+__global__ void k_signal(uint32_t* flag, uint32_t seq, WordCopy extra) {
+    if ((int)threadIdx.x < extra.n) extra.dst[threadIdx.x] = extra.src[threadIdx.x];
+    if (threadIdx.x == 0 && flag) {
         __threadfence_system();
         __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
-void launch_signal(uint32_t* flag, uint32_t seq, hipStream_t s) {
-    hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, s, flag, seq);
+void launch_signal(uint32_t* flag, uint32_t seq, hipStream_t s, const WordCopy* extra) {
+    const WordCopy e = extra ? *extra : WordCopy{};
+    if (e.n > 64) throw std::invalid_argument("launch_signal: at most 64 extra words");
+    hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, s, flag, seq, e);
 }
 
 __global__ void k_copy_words(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, size_t n) {
@@ -1226,15 +1230,20 @@ void launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t
 }
 
 __global__ void k_gather_nodes(const uint32_t* __restrict__ tree, const uint64_t* __restrict__ idx, size_t count,
-                               uint32_t* __restrict__ out) {
+                               uint32_t* __restrict__ out, WordCopy extra) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < extra.n) extra.dst[threadIdx.x] = extra.src[threadIdx.x];
     size_t total = count * 8;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
         out[i] = tree[idx[i / 8] * 8 + (i % 8)];
 }
 
-void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t count, uint32_t* out, hipStream_t s) {
-    if (!count) return;
-    hipLaunchKernelGGL(k_gather_nodes, dim3(grid_for(count * 8, 256)), dim3(256), 0, s, tree, node_idx, count, out);
+void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t count, uint32_t* out, hipStream_t s,
+                         const WordCopy* extra) {
+    const WordCopy e = extra ? *extra : WordCopy{};
+    if (e.n > 256) throw std::invalid_argument("launch_gather_nodes: at most 256 extra words");
+    if (!count && !e.n) return;
+    hipLaunchKernelGGL(k_gather_nodes, dim3(count ? grid_for(count * 8, 256) : 1u), dim3(256), 0, s, tree, node_idx, count,
+                       out, e);
 }
 
 // ------------------------------------------------------------------ fold ---
@@ -1260,10 +1269,12 @@ struct FoldWeights32 {
     uint32_t w[1 << FOLD32_MAXK];
 };
 __global__ __launch_bounds__(256) void k_fold32(const uint32_t* __restrict__ poly, size_t m, int kk, FoldWeights32 gr,
+                                                 uint64_t* __restrict__ zero, int n_zero,
                                                  uint32_t* __restrict__ out) {
     constexpr int NC = 1 << FOLD32_MAXK, BATCH = 16;
     __shared__ __attribute__((aligned(256))) uint32_t tabs[NC * 128];
     __shared__ uint32_t wsh[NC];
+    if (blockIdx.x == 0 && (int)threadIdx.x < n_zero) zero[threadIdx.x] = 0ull;
     const int nc = 1 << kk;
     const uint32_t col_bytes = (uint32_t)(m * 4);
     const __amdgpu_buffer_rsrc_t rsrc = buf_rsrc(poly, col_bytes * (uint32_t)nc);
@@ -1296,11 +1307,13 @@ __global__ __launch_bounds__(256) void k_fold32(const uint32_t* __restrict__ pol
     if (i < m) out[i] = acc;
 }
 
-bool launch_fold32_weights(const uint32_t* poly, size_t m, int kk, const uint32_t* gr_host, uint32_t* out, hipStream_t s) {
+bool launch_fold32_weights(const uint32_t* poly, size_t m, int kk, const uint32_t* gr_host, uint32_t* out, hipStream_t s,
+                           uint64_t* zero, int n_zero) {
     if (kk < 0 || kk > FOLD32_MAXK || (m << kk) * 4 >= ((size_t)1 << 31) || (m + 255) / 256 > 0x7fffffffu) return false;
+    if (n_zero < 0 || n_zero > 256 || (n_zero && !zero)) throw std::invalid_argument("launch_fold32_weights: zero range");
     FoldWeights32 gr{};
     for (int j = 0; j < (1 << kk); j++) gr.w[j] = gr_host[j];
-    hipLaunchKernelGGL(k_fold32, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, poly, m, kk, gr, out);
+    hipLaunchKernelGGL(k_fold32, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, poly, m, kk, gr, zero, n_zero, out);
     return true;
 }
 
@@ -1999,25 +2012,6 @@ __global__ __launch_bounds__(64) void k_fs_step(DevTranscript* __restrict__ st, 
     }
 }
 
-// Thread 0: the reduced triple at res into tlog (if res); then, if flag,
-// seq at flag (mapped host memory, system scope) after everything before.
-template <class F>
-__global__ void k_log_signal(const typename F::Acc* __restrict__ res, typename F::T* __restrict__ tlog, uint32_t* flag,
-                             uint32_t seq) {
-    if (threadIdx.x != 0) return;
-    if (res)
-        for (int k = 0; k < 3; k++) tlog[k] = Ops<F>::acc_reduce(res[k]);
-    if (flag) {
-        __threadfence_system();
-        __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-}
-
-template <class F>
-void launch_log_signal(const typename F::Acc* res, typename F::T* tlog, uint32_t* flag, uint32_t seq, hipStream_t s) {
-    hipLaunchKernelGGL(k_log_signal<F>, dim3(1), dim3(64), 0, s, res, tlog, flag, seq);
-}
-
 template <class F>
 void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename F::Acc* res, const typename F::Acc* enf,
                     int n_enf, typename F::T* tlog, typename F::T* rout, typename F::T* rout2, int n_squeeze,
@@ -2051,7 +2045,6 @@ void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename
     template void launch_fs_step<F>(DevTranscript*, const DevTranscript*, const F::Acc*, const F::Acc*, int, F::T*, F::T*,  \
                                     F::T*, int, uint32_t*, hipStream_t);                                                    \
     template void launch_glue<F>(F::T*, const F::T*, size_t, const F::T*, hipStream_t);                                           \
-    template void launch_log_signal<F>(const F::Acc*, F::T*, uint32_t*, uint32_t, hipStream_t);                             \
     template void launch_induce_prep<F>(const uint64_t*, const F::T*, const F::T*, int, int, F::T*, F::T*, hipStream_t,    \
                                         const RowGather<F>*);                                                               \
     template void launch_enforced<F>(const F::T*, int, int, const F::T*, const F::T*, F::Acc*, hipStream_t);

@@ -93,14 +93,20 @@ void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t*
 // seq there after every node and the root (for a host polling mapped memory).
 void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out = nullptr,
                           uint32_t* flag = nullptr, uint32_t seq = 0);
-// Stores seq at flag (mapped host memory) after the stream's earlier kernels.
-void launch_signal(uint32_t* flag, uint32_t seq, hipStream_t s);
-// One thread: the reduced triple at res (unreduced accumulators) into tlog
-// if res is given, then seq at flag if flag is given.
-template <class F>
-void launch_log_signal(const typename F::Acc* res, typename F::T* tlog, uint32_t* flag, uint32_t seq, hipStream_t s);
+// A few words copied by the first threads of a launch (for instance a
+// section's last triple, as raw accumulators, to mapped host memory).
+struct WordCopy {
+    const uint32_t* src = nullptr;
+    uint32_t* dst = nullptr;
+    int n = 0;
+};
+// Stores seq at flag (mapped host memory, if flag is given) after the
+// stream's earlier kernels and the extra copy (at most 64 words).
+void launch_signal(uint32_t* flag, uint32_t seq, hipStream_t s, const WordCopy* extra = nullptr);
 // out[i] = node at byte offset offs[i] of the tree (32 bytes each).
-void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t count, uint32_t* out, hipStream_t s);
+// With extra, the first threads also copy those words (at most 256).
+void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t count, uint32_t* out, hipStream_t s,
+                         const WordCopy* extra = nullptr);
 
 // dst[i] = src[i] for i < n words (dst may be mapped pinned host memory).
 void launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s);
@@ -114,7 +120,9 @@ void launch_fold(const typename F::T* poly, size_t m, int kk, const typename F::
 // a kernel argument); the kernel builds its tables.  Returns false (nothing
 // launched) unless kk <= FOLD32_MAXK and the 2^kk columns span < 2 GiB.
 constexpr int FOLD32_MAXK = 6;
-bool launch_fold32_weights(const uint32_t* poly, size_t m, int kk, const uint32_t* gr_host, uint32_t* out, hipStream_t s);
+// It also zeroes zero[0..n_zero) (at most 256), if given.
+bool launch_fold32_weights(const uint32_t* poly, size_t m, int kk, const uint32_t* gr_host, uint32_t* out, hipStream_t s,
+                           uint64_t* zero = nullptr, int n_zero = 0);
 // out (and out2, if given) receive the count rows listed in rows.
 template <class F>
 void launch_gather_rows(const typename F::T* mat, size_t n, const uint64_t* rows, size_t count, typename F::T* out,
