@@ -1925,15 +1925,6 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
 }
 
 template <class F>
-void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t len, const typename F::T* r,
-                          typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
-                          hipStream_t s) {
-    size_t h2 = len / 4;
-    hipLaunchKernelGGL((k_sumcheck_fold<F, false>), dim3(grid_for(h2 ? h2 : 1, 256, 1024)), dim3(256), 0, s, f, b, len, r,
-                       FoldStep<F>{}, f2, b2, res, res_clear);
-}
-
-template <class F>
 void launch_sumcheck_fold_step(const typename F::T* f, const typename F::T* b, size_t len, const FoldStep<F>& step,
                                typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
                                hipStream_t s) {
@@ -1944,82 +1935,6 @@ void launch_sumcheck_fold_step(const typename F::T* f, const typename F::T* b, s
     size_t h2 = len / 4;
     hipLaunchKernelGGL((k_sumcheck_fold<F, true>), dim3(grid_for(h2 ? h2 : 1, 256, 1024)), dim3(256), 0, s, f, b, len,
                        (const typename F::T*)nullptr, step, f2, b2, res, res_clear);
-}
-
-template <class F>
-__global__ void k_glue(typename F::T* __restrict__ bsum, const typename F::T* __restrict__ bnew, size_t len,
-                       const typename F::T* __restrict__ betap) {
-    using T = typename F::T;
-    const T beta = *betap;
-    __shared__ T tab[F::TAB];
-    for (int e = threadIdx.x; e < F::TAB; e += blockDim.x) tab[e] = Ops<F>::table_entry(beta, e);
-    __syncthreads();
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x)
-        bsum[i] ^= Ops<F>::mul_tab(tab, bnew[i]);
-}
-
-template <class F>
-void launch_glue(typename F::T* bsum, const typename F::T* bnew, size_t len, const typename F::T* beta, hipStream_t s) {
-    hipLaunchKernelGGL(k_glue<F>, dim3(grid_for(len, 256, 2048)), dim3(256), 0, s, bsum, bnew, len, beta);
-}
-
-// One wave runs the transcript step; every lane computes the same values
-// and lane 0 writes them.
-template <class F>
-__global__ __launch_bounds__(64) void k_fs_step(DevTranscript* __restrict__ st, DevTranscript init, int use_init,
-                                                const typename F::Acc* __restrict__ res,
-                                                const typename F::Acc* __restrict__ enf, int n_enf,
-                                                typename F::T* __restrict__ tlog, typename F::T* __restrict__ rout,
-                                                typename F::T* __restrict__ rout2, int n_squeeze,
-                                                uint32_t* __restrict__ err_out) {
-    using T = typename F::T;
-    using A = typename F::Acc;
-    dfs::State s;
-    if (use_init)
-        s.load_value(init);
-    else
-        s.load(st);
-    uint32_t err = 0;
-    if (res) {
-        T t[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) t[k] = Ops<F>::acc_reduce(dfs::uniform_acc(res[k]));
-        if (enf) {
-            // parts loaded across the lanes, then XOR-reduced over the wave
-            A e = Ops<F>::acc_zero();
-            for (int i = threadIdx.x; i < n_enf; i += 64) e ^= enf[i];
-            e = wave_xor(e);
-            if (!Ops<F>::eq(t[0] ^ t[1], Ops<F>::acc_reduce(dfs::uniform_acc(e)))) err = 1;
-        }
-#pragma unroll
-        for (int k = 0; k < 3; k++)
-#pragma unroll
-            for (int i = 0; i < Ops<F>::WORDS; i++) s.word_le(Ops<F>::word(t[k], i));
-        if (threadIdx.x == 0)
-            for (int k = 0; k < 3; k++) tlog[k] = t[k];
-    }
-    for (int i = 0; i < n_squeeze; i++) {
-        uint32_t o[4];
-        s.squeeze(o);
-        if (threadIdx.x == 0) {
-            rout[i] = Ops<F>::from_squeeze(o);
-            if (rout2) rout2[i] = Ops<F>::from_squeeze(o);
-        }
-    }
-    if (threadIdx.x == 0) {
-        s.store(st);
-        if (err) *err_out = 1;
-    }
-}
-
-template <class F>
-void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename F::Acc* res, const typename F::Acc* enf,
-                    int n_enf, typename F::T* tlog, typename F::T* rout, typename F::T* rout2, int n_squeeze,
-                    uint32_t* err_out, hipStream_t s) {
-    DevTranscript iv{};
-    if (init) iv = *init;
-    hipLaunchKernelGGL(k_fs_step<F>, dim3(1), dim3(64), 0, s, st, iv, init ? 1 : 0, res, enf, n_enf, tlog, rout, rout2,
-                       n_squeeze, err_out);
 }
 
 // ------------------------------------------------------- instantiations ---
@@ -2038,13 +1953,8 @@ void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename
                                   const EnforcedIn<F>*, const Eval013In<F>*);                                               \
     template void launch_eval013<F>(const F::T*, const F::T*, size_t, F::Acc*, F::Acc*, hipStream_t);                       \
     template void launch_zero_acc<F>(F::Acc*, int, hipStream_t);                                                                 \
-    template void launch_sumcheck_fold<F>(const F::T*, const F::T*, size_t, const F::T*, F::T*, F::T*, F::Acc*, F::Acc*,  \
-                                          hipStream_t);                                                                     \
     template void launch_sumcheck_fold_step<F>(const F::T*, const F::T*, size_t, const FoldStep<F>&, F::T*, F::T*, F::Acc*, \
                                                F::Acc*, hipStream_t);                                                       \
-    template void launch_fs_step<F>(DevTranscript*, const DevTranscript*, const F::Acc*, const F::Acc*, int, F::T*, F::T*,  \
-                                    F::T*, int, uint32_t*, hipStream_t);                                                    \
-    template void launch_glue<F>(F::T*, const F::T*, size_t, const F::T*, hipStream_t);                                           \
     template void launch_induce_prep<F>(const uint64_t*, const F::T*, const F::T*, int, int, F::T*, F::T*, hipStream_t,    \
                                         const RowGather<F>*);                                                               \
     template void launch_enforced<F>(const F::T*, int, int, const F::T*, const F::T*, F::Acc*, hipStream_t);

@@ -189,14 +189,10 @@ template <class F>
 void launch_eval013(const typename F::T* f, const typename F::T* g, size_t len, typename F::Acc* res,
                     typename F::Acc* res_clear, hipStream_t s);
 template <class F> void launch_zero_acc(typename F::Acc* res, int count, hipStream_t s);
-// Folds f and b by r (length len -> len/2) into f2, b2 and writes the
-// triple of the folded pair (ligerito.rs:119-127).
-// r is read from device memory (written by launch_fs_step).
-template <class F>
-void launch_sumcheck_fold(const typename F::T* f, const typename F::T* b, size_t len, const typename F::T* r,
-                          typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
-                          hipStream_t s);
-// The transcript step that precedes a fold, run by the fold kernel itself:
+// Folds f and b by the round's challenge (length len -> len/2, ligerito.rs:
+// 119-127) into f2, b2 and adds the triple of the folded pair into res
+// (res_clear zeroed for the next round).  The transcript step that precedes
+// the fold runs in the fold kernel itself:
 // every workgroup takes the state at st_in (or, if st_in is null, the state
 // init, passed by value), absorbs the triple at res_in (reduced) and
 // squeezes n_squeeze challenges; the fold's r is the last one.  Workgroup 0
@@ -224,20 +220,4 @@ template <class F>
 void launch_sumcheck_fold_step(const typename F::T* f, const typename F::T* b, size_t len, const FoldStep<F>& step,
                                typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
                                hipStream_t s);
-// One transcript step on the device, on the state at st (or, if init is
-// given, on *init, passed by value): if res is given, reduce the triple
-// (three accumulators) and absorb it (QuadraticEvals::to_bytes,
-// sumcheck.rs:29-35), logging it to tlog and, if enf is given, setting
-// *err_out to 1 when t0 + t1 differs from the enforced sum, the XOR of the
-// n_enf parts at enf (ligerito.rs:35,52); then squeeze n_squeeze field
-// elements into rout (and rout2, if given) (get_field).  The new state is
-// stored at st.
-template <class F>
-void launch_fs_step(DevTranscript* st, const DevTranscript* init, const typename F::Acc* res, const typename F::Acc* enf,
-                    int n_enf, typename F::T* tlog, typename F::T* rout, typename F::T* rout2, int n_squeeze,
-                    uint32_t* err_out, hipStream_t s);
-// bsum[i] ^= beta * bnew[i]   (glue, ligerito.rs:59-67)
-template <class F>
-void launch_glue(typename F::T* bsum, const typename F::T* bnew, size_t len, const typename F::T* beta, hipStream_t s);
-
 }  // namespace lig
