@@ -1419,47 +1419,93 @@ constexpr int ACC_QMAX = 160;
 __global__ __launch_bounds__(1024) void k_induce_accum32(const uint32_t* __restrict__ P, const uint32_t* __restrict__ Kt, int nq,
                                                          int ell, int clo, uint32_t* __restrict__ basis,
                                                          const uint32_t* __restrict__ f, uint64_t* res, uint64_t* res_clear) {
-    __shared__ __attribute__((aligned(256))) uint32_t tabs[ACC_QMAX * 128];
+    __shared__ __attribute__((aligned(256))) uint32_t tabs[(ACC_QMAX + 1) * 128];
     __shared__ uint32_t part[1024];
     const size_t np = (size_t)1 << clo, nk = (size_t)1 << (ell - clo);
     const size_t X = blockIdx.x;
     const int qs = (int)(blockDim.x >> clo);  // query slices (blockDim.x = qs * 2^clo)
-    const int sl = (int)(threadIdx.x >> clo);
+    // the slice is uniform over a wave (2^clo >= 64); saying so keeps the
+    // loop's counts, table offsets and load offsets in scalar registers
+    const int sl = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> clo));
     const size_t x = threadIdx.x & (np - 1);
-    uint32_t acc = 0;
+    const size_t j = X * np + x, h = (nk * np) / 2;
+    // the triple's f entries do not depend on the basis: loaded first
+    uint32_t fj = 0, fo = 0;
+    if (f && sl == 0) {
+        fj = f[j];
+        fo = X < nk / 2 ? f[j + h] : f[j - h];
+    }
+    // Lane queries i = sl + k*qs, k < cnt.  P entries are loaded AP loads
+    // ahead of their multiplies (the first AP before the tables are built),
+    // so the loop waits on a memory latency once, not once per few queries;
+    // multiplies go four at a time with all their LDS reads in flight.  Past
+    // cnt a load repeats the last valid entry (in range) and its multiply
+    // uses the all-zero table after the query tables, so the loads need no
+    // branches.
+    constexpr int AP = 16;
+    constexpr uint32_t ZTAB = ACC_QMAX * 512;  // byte offset of the zero table
+    const __amdgpu_buffer_rsrc_t prs = buf_rsrc(P, (uint32_t)((size_t)nq * np * 4));
+    if (threadIdx.x < 128) tabs[ACC_QMAX * 128 + threadIdx.x] = 0u;
+    uint32_t acc4[4] = {0u, 0u, 0u, 0u};
     for (int q0 = 0; q0 < nq; q0 += ACC_QMAX) {
         const int qn = nq - q0 < ACC_QMAX ? nq - q0 : ACC_QMAX;
+        const int cnt = sl < qn ? (qn - sl + qs - 1) / qs : 0;
+        // P entry (q0 + sl + k*qs, x): wave-uniform byte offset in a scalar
+        // register, the lane's x*4 in a vector one
+        const uint32_t pofs = (uint32_t)(((size_t)q0 + (size_t)sl) * np * 4), pst = (uint32_t)(qs * np * 4);
+        const uint32_t xo = (uint32_t)(x * 4);
+        auto ld = [&](int k) { return buf_ld(prs, xo, pofs + (uint32_t)(k < cnt ? k : cnt - 1) * pst); };
+        auto step = [&](const uint32_t(&cur)[AP], uint32_t(&nxt)[AP], int k0) {
+#pragma unroll
+            for (int e = 0; e < AP; e++) nxt[e] = ld(k0 + AP + e);
+#pragma unroll
+            for (int g = 0; g < AP / 4; g++) {
+                if (k0 + 4 * g < cnt) {
+                    uint32_t tb[4], ww[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int k = k0 + 4 * g + e;
+                        tb[e] = k < cnt ? (uint32_t)((sl + k * qs) * 512) : ZTAB;
+                        ww[e] = cur[4 * g + e];
+                    }
+                    mul_tab_lds_n<4>(tabs, tb, ww, acc4);
+                }
+            }
+        };
+        uint32_t w[AP], wn[AP];
+        if (cnt > 0) {
+#pragma unroll
+            for (int e = 0; e < AP; e++) w[e] = ld(e);
+        }
         if (q0 > 0) LIG_LDS_BARRIER();  // the previous chunk's lookups are done
         for (int idx = threadIdx.x; idx < qn * 8; idx += blockDim.x)
             gf32_tab_build(tabs + (idx >> 3) * 128, Kt[(size_t)(q0 + (idx >> 3)) * nk + X], idx & 7);
         LIG_LDS_BARRIER();
-        const uint32_t* Pq = P + (size_t)q0 * np + x;
-#pragma unroll 8
-        for (int i = sl; i < qn; i += qs) acc ^= mul_tab_lds(tabs, (uint32_t)(i * 512), Pq[(size_t)i * np]);
+        for (int k0 = 0; k0 < cnt; k0 += 2 * AP) {
+            step(w, wn, k0);
+            if (k0 + AP < cnt) step(wn, w, k0 + AP);
+        }
     }
-    uint32_t b = acc;
+    uint32_t b = (acc4[0] ^ acc4[1]) ^ (acc4[2] ^ acc4[3]);
     if (qs > 1) {
-        part[threadIdx.x] = acc;
+        part[threadIdx.x] = b;
         LIG_LDS_BARRIER();
         if (sl == 0)
             for (int k = 1; k < qs; k++) b ^= part[k * np + x];
     }
-    const size_t j = X * np + x;
     if (sl == 0) basis[j] = b;
     if (!f) return;
     // this entry's share of the triple (see above); X < nk / 2 is uniform
     uint64_t s0 = 0, s1 = 0, s2 = 0;
     if (blockIdx.x == 0 && threadIdx.x < 3) res_clear[threadIdx.x] = 0;
     if (sl == 0) {
-        const size_t h = (nk * np) / 2;
-        const uint32_t fj = f[j];
         if (X < nk / 2) {
-            const uint32_t f1 = f[j + h];
+            const uint32_t f1 = fo;
             const uint32_t a2 = Ops<F32>::mul2(f1) ^ f1 ^ Ops<F32>::mul2(fj);
             Ops<F32>::acc_mul(s0, fj, b);
             Ops<F32>::acc_mul(s2, a2, Ops<F32>::mul2(b));
         } else {
-            const uint32_t f0 = f[j - h];
+            const uint32_t f0 = fo;
             const uint32_t a2 = Ops<F32>::mul2(fj) ^ fj ^ Ops<F32>::mul2(f0);
             Ops<F32>::acc_mul(s1, fj, b);
             Ops<F32>::acc_mul(s2, a2, Ops<F32>::mul2(b) ^ b);
@@ -1468,11 +1514,13 @@ __global__ __launch_bounds__(1024) void k_induce_accum32(const uint32_t* __restr
     block_xor3<F32>(s0, s1, s2, res);
 }
 
+// Low bits of a basis index: P holds 2^clo entries per query, K the other
+// 2^(ell - clo).  Aims at 256 K rows (one accumulate workgroup each, so
+// every compute unit gets one) with 2^6..2^10 entries per row.
 int induce_low_bits(int ell) {
-    int clo = ell - 8;
-    if (clo < 8) clo = ell < 8 ? ell : 8;
-    if (clo > 10) clo = 10;
-    return clo;
+    if (ell < 6) return ell;
+    const int clo = ell - 8;
+    return clo < 6 ? 6 : (clo > 10 ? 10 : clo);
 }
 
 // GF(2^32) variant: P and K by doubling, P[x + 2^t] = P[x] * c_t, with
@@ -1545,7 +1593,8 @@ int launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, i
     }
     if constexpr (F::BITS == 32) {
         // 1024 lanes per X row: 2^clo entries times 1024 / 2^clo query slices
-        if (clo >= 6 && clo <= 10 && (!ev || nk >= 2)) {  // the triple pairs rows X and X + nk/2
+        // the triple pairs rows X and X + nk/2; P is addressed with 32-bit offsets
+        if (clo >= 6 && clo <= 10 && (!ev || nk >= 2) && ((size_t)nq << (clo + 2)) < ((size_t)1 << 31)) {
             hipLaunchKernelGGL(k_induce_accum32, dim3((unsigned)nk), dim3(1024), 0, s, P, K, nq, ell, clo, basis,
                                ev ? ev->f : nullptr, ev ? ev->res : nullptr, ev ? ev->res_clear : nullptr);
             return n_parts;
