@@ -139,9 +139,7 @@ __device__ __forceinline__ void stage_words(uint32_t* __restrict__ dst, const ui
 
 __device__ __forceinline__ uint32_t mul_tab_lds(const uint32_t* tabs, uint32_t tb, uint32_t w);
 
-// GF(2^32) variant: the 64 low-part pis tables stay in LDS for the whole
-// workgroup, which walks several 64x64 tiles; each tile adds one table.
-// Tiles of 64 rows x 2^LCT columns (2^LCT = min(n, 64); m >= 64), with
+// GF(2^32) variant: tiles of 64 rows x 2^LCT columns (2^LCT = min(n, 64); m >= 64), with
 // workgroups persistent over tiles.  A tile's rows i0..i0+63 are scaled by
 // pis[i0 + r]: the workgroup builds one multiply table per row in LDS, so
 // each element costs one table multiply.  The next tile's loads are issued
