@@ -8,6 +8,7 @@
 
 #include <atomic>
 #include <stdexcept>
+#include <type_traits>
 
 #include "blake3.hpp"
 #include "fields.hpp"
@@ -633,6 +634,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
     // tiles of one workgroup share B (consecutive lo), hence their tables
     const uint32_t first_job = blockIdx.x * (uint32_t)tiles_per_wg;
     const uint32_t B = (first_job >> lgnc) >> sh;
+    uint32_t zt = 0;  // bit j: phase-1 stage j's first table is zero
 
     for (int tj = 0; tj < tiles_per_wg; tj++) {
         const uint32_t job = first_job + tj;
@@ -662,10 +664,45 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
             LIG_LDS_BARRIER();  // previous tile's exchange reads are done
         }
         LIG_LDS_BARRIER();  // tables complete (this tile's loads may still be in flight)
-
+        if (tj == 0) {
+            // Stage j's first table (block B*2^j) is all zero when its twiddle
+            // is: entry 1 of nibble 0 is the twiddle itself.  The heap's
+            // first twiddle of every layer is zero, so quarter 0's leading
+            // tiles skip those multiplies (a quarter of the first stage's
+            // work and less after it).  Same for every tile of the workgroup.
 #pragma unroll
-        for (int j = 0; j < K1; j++) {
-            const int h = L1 >> (j + 1);
+            for (int j = 0; j < K1; j++) zt |= (tabs[((1 << j) - 1) * 128 + 1] == 0u ? 1u : 0u) << j;
+            zt = __builtin_amdgcn_readfirstlane(zt);
+        }
+
+        auto stage1 = [&](auto J) {
+            constexpr int j = decltype(J)::value, h = L1 >> (j + 1);
+            // butterflies a < h sit in the stage's first block (table (j, 0))
+            constexpr int NS = L1 / 2 - h;  // butterflies outside the first block
+            if ((zt >> j) & 1u) {
+#pragma unroll
+                for (int a = 0; a < h; a++) v[a + h] ^= v[a];  // u unchanged: twiddle 0
+                if constexpr (NS > 0) {
+                    uint32_t tbs[NS], ws[NS], us[NS];
+                    int b = 0;
+#pragma unroll
+                    for (int a = 2 * h; a < L1; a++) {
+                        if (a & h) continue;
+                        tbs[b] = (uint32_t)(((1 << j) - 1 + (a >> (K1 - j))) * 512);
+                        us[b] = v[a];
+                        ws[b++] = v[a + h];
+                    }
+                    mul_tab_lds_n<NS>(tabs, tbs, ws, us);
+                    b = 0;
+#pragma unroll
+                    for (int a = 2 * h; a < L1; a++) {
+                        if (a & h) continue;
+                        v[a] = us[b];
+                        v[a + h] ^= us[b++];
+                    }
+                }
+                return;
+            }
             uint32_t tbs[L1 / 2], ws[L1 / 2], us[L1 / 2];
             int b = 0;
 #pragma unroll
@@ -683,7 +720,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
                 v[a] = us[b];
                 v[a + h] ^= us[b++];
             }
-        }
+        };
+        stage1(std::integral_constant<int, 0>{});
+        if constexpr (K1 > 1) stage1(std::integral_constant<int, 1>{});
+        if constexpr (K1 > 2) stage1(std::integral_constant<int, 2>{});
+        static_assert(K1 <= 3, "phase 1 holds at most three stages");
 
         if constexpr (K2 == 0) {
 #pragma unroll
