@@ -6,7 +6,6 @@
 // arrays.  All arithmetic is exact binary-field arithmetic.
 #include <hip/hip_runtime.h>
 
-#include <atomic>
 #include <stdexcept>
 #include <type_traits>
 
@@ -140,12 +139,12 @@ __device__ __forceinline__ void stage_words(uint32_t* __restrict__ dst, const ui
 
 __device__ __forceinline__ uint32_t mul_tab_lds(const uint32_t* tabs, uint32_t tb, uint32_t w);
 
-// GF(2^32) variant: tiles of 64 rows x 2^LCT columns (2^LCT = min(n, 64); m >= 64), with
-// workgroups persistent over tiles.  A tile's rows i0..i0+63 are scaled by
-// pis[i0 + r]: the workgroup builds one multiply table per row in LDS, so
-// each element costs one table multiply.  The next tile's loads are issued
-// before this tile is scaled and stored; the barriers wait for LDS traffic
-// only.
+// GF(2^32) variant: tiles of 64 rows x 2^LCT columns (2^LCT = min(n, 64);
+// m >= 64).  A tile's rows i0..i0+63 are scaled by pis[i0 + r]: the
+// workgroup builds one multiply table per row in LDS, so each element costs
+// one table multiply.  The launcher gives every tile its own workgroup; a
+// smaller grid walks the tiles, issuing the next tile's loads before this
+// tile is scaled and stored.  The barriers wait for LDS traffic only.
 template <int LCT>
 __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __restrict__ poly, size_t m, size_t n,
                                                           const uint32_t* __restrict__ pis, uint32_t* __restrict__ msg) {
@@ -191,19 +190,6 @@ __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __res
     }
 }
 
-// Compute units of the current device (cached per device).
-static int device_cu_count() {
-    static std::atomic<int> cached[64];
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    int cu = cached[dev].load(std::memory_order_relaxed);
-    if (cu <= 0) {
-        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0) cu = 256;
-        cached[dev].store(cu, std::memory_order_relaxed);
-    }
-    return cu;
-}
-
 template <class F>
 void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const typename F::T* pis,
                             const typename F::T* lo_tabs, const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s) {
@@ -212,9 +198,12 @@ void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const
         if (m >= 64 && n >= 4 && (m & (m - 1)) == 0 && (n & (n - 1)) == 0) {
             int lct = 0;
             while ((1 << lct) < (n < 64 ? (int)n : 64)) lct++;
-            // three workgroups fit a CU (49 KB of LDS at 64 columns)
-            const size_t cap = 3 * (size_t)device_cu_count();
-            const unsigned g = (unsigned)(tiles < cap ? tiles : cap);
+            // one workgroup per tile: three fit a CU (49 KB of LDS at 64
+            // columns), and independent workgroups overlap one another's
+            // table builds and loads (measured faster than a persistent grid
+            // of three per CU walking the tiles with a prefetch)
+            if (tiles > 0x7fffffffu) throw std::invalid_argument("transpose: too many tiles");
+            const unsigned g = (unsigned)tiles;
             switch (lct) {
                 case 2: hipLaunchKernelGGL(k_transpose_scale32<2>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
                 case 3: hipLaunchKernelGGL(k_transpose_scale32<3>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
