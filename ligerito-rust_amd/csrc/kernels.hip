@@ -1443,8 +1443,11 @@ __global__ __launch_bounds__(256) void k_induce_accum(const typename F::T* __res
 // both, s2 gets a2*(3 b1 + 2 b0) as in eval_013_product.
 template <class F> __device__ inline void block_xor3(typename F::Acc a0, typename F::Acc a1, typename F::Acc a2,
                                                      typename F::Acc* res);
-constexpr int ACC_QMAX = 160;
-__global__ __launch_bounds__(1024) void k_induce_accum32(const uint32_t* __restrict__ P, const uint32_t* __restrict__ Kt, int nq,
+// 148 query tables (79 KB of LDS with the zero table and the partial
+// sums) let two 1024-thread workgroups share a CU: eight waves per SIMD
+// overlap one another's LDS reads and table arithmetic.
+constexpr int ACC_QMAX = 148;
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_induce_accum32(const uint32_t* __restrict__ P, const uint32_t* __restrict__ Kt, int nq,
                                                          int ell, int clo, uint32_t* __restrict__ basis,
                                                          const uint32_t* __restrict__ f, uint64_t* res, uint64_t* res_clear) {
     __shared__ __attribute__((aligned(256))) uint32_t tabs[(ACC_QMAX + 1) * 128];
@@ -1470,7 +1473,7 @@ __global__ __launch_bounds__(1024) void k_induce_accum32(const uint32_t* __restr
     // cnt a load repeats the last valid entry (in range) and its multiply
     // uses the all-zero table after the query tables, so the loads need no
     // branches.
-    constexpr int AP = 16;
+    constexpr int AP = 8;
     constexpr uint32_t ZTAB = ACC_QMAX * 512;  // byte offset of the zero table
     const __amdgpu_buffer_rsrc_t prs = buf_rsrc(P, (uint32_t)((size_t)nq * np * 4));
     if (threadIdx.x < 128) tabs[ACC_QMAX * 128 + threadIdx.x] = 0u;
@@ -1543,11 +1546,11 @@ __global__ __launch_bounds__(1024) void k_induce_accum32(const uint32_t* __restr
 }
 
 // Low bits of a basis index: P holds 2^clo entries per query, K the other
-// 2^(ell - clo).  Aims at 256 K rows (one accumulate workgroup each, so
-// every compute unit gets one) with 2^6..2^10 entries per row.
+// 2^(ell - clo).  Aims at 512 K rows (one accumulate workgroup each, two
+// per compute unit) with 2^6..2^10 entries per row.
 int induce_low_bits(int ell) {
     if (ell < 6) return ell;
-    const int clo = ell - 8;
+    const int clo = ell - 9;
     return clo < 6 ? 6 : (clo > 10 ? 10 : clo);
 }
 
