@@ -1444,12 +1444,13 @@ __global__ __launch_bounds__(256) void k_induce_accum(const typename F::T* __res
 template <class F> __device__ inline void block_xor3(typename F::Acc a0, typename F::Acc a1, typename F::Acc a2,
                                                      typename F::Acc* res);
 // 148 query tables (79 KB of LDS with the zero table and the partial
-// sums) let two 1024-thread workgroups share a CU: eight waves per SIMD
-// overlap one another's LDS reads and table arithmetic.
+// sums) let two 1024-thread workgroups share a CU (eight waves per SIMD,
+// hence at most 64 registers per lane).  The loop is bound by LDS
+// throughput: eight table reads per product.
 constexpr int ACC_QMAX = 148;
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_induce_accum32(const uint32_t* __restrict__ P, const uint32_t* __restrict__ Kt, int nq,
-                                                         int ell, int clo, uint32_t* __restrict__ basis,
-                                                         const uint32_t* __restrict__ f, uint64_t* res, uint64_t* res_clear) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_induce_accum32(
+    const uint32_t* __restrict__ P, const uint32_t* __restrict__ Kt, int nq, int ell, int clo, uint32_t* __restrict__ basis,
+    const uint32_t* __restrict__ f, uint64_t* res, uint64_t* res_clear) {
     __shared__ __attribute__((aligned(256))) uint32_t tabs[(ACC_QMAX + 1) * 128];
     __shared__ uint32_t part[1024];
     const size_t np = (size_t)1 << clo, nk = (size_t)1 << (ell - clo);
