@@ -22,6 +22,10 @@ ERRORS = {-1: "LIG_E_INVALID", -2: "LIG_E_ASSERT", -3: "LIG_E_HIP", -4: "LIG_E_O
           -6: "LIG_E_MALFORMED", -7: "LIG_E_CAPACITY", -8: "LIG_E_MERKLE", -9: "LIG_E_LIGERO"}
 
 _lib = None
+# The loaded library, also held by _Handle._release's default argument so
+# that handles can be released while the interpreter shuts down (module
+# globals are cleared before the last objects are collected).
+_holder = [None]
 
 
 class LigeritoError(RuntimeError):
@@ -104,6 +108,7 @@ def lib():
             fn.restype = res
             fn.argtypes = args
         _lib = L
+        _holder[0] = L
     return _lib
 
 
@@ -132,7 +137,18 @@ def nelems(a, bits):
 
 
 # ------------------------------------------------------------------ context --
-class Context:
+class _Handle:
+    """Base of the wrappers that own a library handle (self._h)."""
+
+    _h = None
+
+    def _release(self, fn_name, _holder=_holder):
+        h, self._h = self._h, None
+        if h and _holder[0] is not None:
+            getattr(_holder[0], fn_name)(h)
+
+
+class Context(_Handle):
     """Device, stream and caches (lig_ctx)."""
 
     def __init__(self, device: int = 0):
@@ -162,16 +178,15 @@ class Context:
         return ms.value, n.value
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().lig_ctx_destroy(self._h)
-            self._h = None
+        if self._h:
+            self._release("lig_ctx_destroy")
 
     def __del__(self):
         self.close()
 
 
 # --------------------------------------------------------------- Reed–Solomon --
-class ReedSolomonEncoding:
+class ReedSolomonEncoding(_Handle):
     """src/reed_solomon.rs:7-114"""
 
     def __init__(self, ctx: Context, message_length: int, block_length: int, bits: int = 32):
@@ -180,9 +195,8 @@ class ReedSolomonEncoding:
         self._h, self.bits, self.ctx = h, bits, ctx
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().lig_rs_free(self._h)
-            self._h = None
+        if self._h:
+            self._release("lig_rs_free")
 
     def message_length(self):
         return lib().lig_rs_message_length(self._h)
@@ -251,7 +265,7 @@ def verify(root: bytes, proof, depth: int, leaves, indices) -> bool:
 
 
 # --------------------------------------------------------------- transcript --
-class FS:
+class FS(_Handle):
     """src/emulated_fs.rs:6-75"""
 
     def __init__(self, seed: int):
@@ -260,9 +274,8 @@ class FS:
         self._h = h
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().lig_fs_free(self._h)
-            self._h = None
+        if self._h:
+            self._release("lig_fs_free")
 
     def absorb(self, data: bytes):
         b = bytes(data)
@@ -289,16 +302,15 @@ class FS:
 
 
 # ------------------------------------------------------------ ligero_commit --
-class Witness:
+class Witness(_Handle):
     """RecursiveLigeroWitness (data_structures.rs:32-40), device-resident."""
 
     def __init__(self, h, bits):
         self._h, self.bits = h, bits
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().lig_witness_free(self._h)
-            self._h = None
+        if self._h:
+            self._release("lig_witness_free")
 
     @property
     def num_rows(self):
@@ -351,7 +363,7 @@ def ligero_commit_dev(d_poly_ptr: int, m: int, n: int, rs: ReedSolomonEncoding) 
 
 
 # ------------------------------------------------------------ prove / verify --
-class ProverConfig:
+class ProverConfig(_Handle):
     """ProverConfig (data_structures.rs:9-21); hardcoded_config_20/24 at log_n 20/24."""
 
     def __init__(self, ctx: Context, log_n: int, bits: int = 32):
@@ -360,9 +372,8 @@ class ProverConfig:
         self._h, self.bits, self.log_n, self.ctx = h, bits, log_n, ctx
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().lig_prover_config_free(self._h)
-            self._h = None
+        if self._h:
+            self._release("lig_prover_config_free")
 
 
 def hardcoded_config_20(ctx, bits=32):
@@ -373,16 +384,15 @@ def hardcoded_config_24(ctx, bits=32):
     return ProverConfig(ctx, 24, bits)
 
 
-class Proof:
+class Proof(_Handle):
     """FinalizedLigeritoProof (data_structures.rs:91-103)."""
 
     def __init__(self, h, bits):
         self._h, self.bits = h, bits
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().lig_proof_free(self._h)
-            self._h = None
+        if self._h:
+            self._release("lig_proof_free")
 
     def to_bytes(self) -> bytes:
         n = C.c_size_t(0)
@@ -428,7 +438,7 @@ def verifier_dev(ctx: Context, log_n: int, proof: Proof, fs_seed: int = 1234) ->
 
 
 # ---------------------------------------------------------------- batched --
-class Batch:
+class Batch(_Handle):
     """Many independent proves on one device (lig_batch; no reference counterpart)."""
 
     def __init__(self, device: int = 0, bits: int = 32, n_streams: int = 4):
@@ -455,15 +465,14 @@ class Batch:
         return [Proof(C.c_void_p(out[i]), self.bits) for i in range(n)]
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().lig_batch_destroy(self._h)
-            self._h = None
+        if self._h:
+            self._release("lig_batch_destroy")
 
     def __del__(self):
         self.close()
 
 
-class Server:
+class Server(_Handle):
     """Prove server with continuous batching (lig_server; no reference counterpart)."""
 
     def __init__(self, device: int = 0, bits: int = 32, n_slots: int = 4, hbm_budget: int = 32 << 30):
@@ -501,9 +510,8 @@ class Server:
         return self._take(lib().lig_serve_wait, ticket)
 
     def stop(self):
-        if getattr(self, "_h", None):
-            lib().lig_serve_stop(self._h)
-            self._h = None
+        if self._h:
+            self._release("lig_serve_stop")
             self._keep.clear()
 
     def __del__(self):
