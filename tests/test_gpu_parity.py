@@ -142,11 +142,11 @@ def test_commit_encode_kernel_shape_gf32(O, ctx):
     assert w.root() == root
 
 
-# inductions run over 2^(L-6) and 2^(L-10) entries: L = 16..23 here and 24
-# below give every size from 2^6 to 2^18, hence accumulate rows of 2^6 to 2^9
-# entries (2^10 needs L >= 25)
+# inductions run over 2^(L-6) and 2^(L-10) entries: L = 16..23 and 25 here
+# and 24 below give every size from 2^6 to 2^19, hence every accumulate row
+# length (2^6 to 2^10 entries)
 @pytest.mark.parametrize("bits,L", [(32, 16), (128, 16), (32, 17), (32, 18), (32, 19), (32, 20), (32, 21), (32, 22),
-                                    (32, 23)])
+                                    (32, 23), (32, 25)])
 def test_prover_bytes_match_oracle(O, ctx, bits, L):
     say("prove", bits, L)
     cfg = lh.ProverConfig(ctx, L, bits)
