@@ -145,6 +145,32 @@ void lig_witness_free(lig_witness *w);
  * other log_n in [16, 27] follow the same shape rules. */
 typedef struct lig_prover_config lig_prover_config;
 int lig_prover_config_new(lig_ctx *ctx, int field_bits, int log_n, lig_prover_config **out);
+/* ProverConfig built from its fields (data_structures.rs:9-21), as
+ * hardcoded_config_20/24 fill them (config.rs:19-46): initial_dims =
+ * (rows, cols) of the first matrix, dims[2i], dims[2i+1] = (rows, cols) of
+ * recursive step i, ks[i] its sumcheck rounds.  The shape must satisfy what
+ * prover() and verifier() rely on, else LIG_E_INVALID names the broken rule:
+ * recursive_steps >= 1 (prover.rs:198); initial_k = log2 initial_dims[1] and
+ * ks[i] = log2 dims[i].cols, both >= 1 (prover.rs:50-53,97-102);
+ * initial_dims[0] = dims[0].rows * dims[0].cols and dims[i].rows =
+ * dims[i+1].rows * dims[i+1].cols; 4 * rows >= 148 for every matrix
+ * (emulated_fs.rs:65 never ends otherwise); powers of two
+ * (reed_solomon.rs:49-64).  Limits of this library: at most 8 steps and
+ * 2^27 coefficients, rows of at most 4 KiB, a final message of at most
+ * 2 MiB.  The input of lig_prover is initial_dims[0] * initial_dims[1]
+ * elements. */
+int lig_prover_config_from_dims(lig_ctx *ctx, int field_bits, size_t recursive_steps, const size_t initial_dims[2],
+                                size_t initial_k, const size_t *dims, const size_t *ks, lig_prover_config **out);
+/* The generated configuration for 2^log_n coefficients (the shape
+ * lig_prover_config_new builds; config.rs:6-47,69-112 at log_n = 20, 24),
+ * in the layout of lig_prover_config_from_dims; no device needed.  dims and
+ * ks may be NULL (only the first three outputs are written); otherwise they
+ * hold cap_steps steps (LIG_E_CAPACITY if more are needed). */
+int lig_prover_config_shape(int log_n, size_t *recursive_steps, size_t initial_dims[2], size_t *initial_k,
+                            size_t *dims, size_t *ks, size_t cap_steps);
+/* The shape of a configuration, as lig_prover_config_shape writes it. */
+int lig_prover_config_dims(const lig_prover_config *cfg, size_t *recursive_steps, size_t initial_dims[2],
+                           size_t *initial_k, size_t *dims, size_t *ks, size_t cap_steps);
 void lig_prover_config_free(lig_prover_config *cfg);
 
 /* ---------------------------------------------------------------- prove -- */
@@ -176,6 +202,17 @@ int lig_verifier(int field_bits, int log_n, const lig_proof *p, uint32_t fs_seed
  * verifier.rs:54-61,148-155, the bulk of its arithmetic) on the context's
  * GPU; same results and status codes as lig_verifier. */
 int lig_verifier_dev(lig_ctx *ctx, int field_bits, int log_n, const lig_proof *p, uint32_t fs_seed);
+/* verifier (verifier.rs:10-164) on an explicit VerifierConfig
+ * (data_structures.rs:23-30): initial_dim = log2 of the first matrix's rows,
+ * log_dims[i] = log2 of step i's rows, initial_k and ks[i] the fold sizes
+ * (config.rs:49-67,114-134 for the two hardcoded ones).  A shape that
+ * breaks the rules of lig_prover_config_from_dims gives LIG_E_INVALID. */
+int lig_verifier_with_config(int field_bits, size_t recursive_steps, size_t initial_dim, const size_t *log_dims,
+                             size_t initial_k, const size_t *ks, const lig_proof *p, uint32_t fs_seed);
+/* The same with the basis inductions on the context's GPU. */
+int lig_verifier_with_config_dev(lig_ctx *ctx, int field_bits, size_t recursive_steps, size_t initial_dim,
+                                 const size_t *log_dims, size_t initial_k, const size_t *ks, const lig_proof *p,
+                                 uint32_t fs_seed);
 
 /* ---------------------------------------------------------------- batch -- */
 /* No counterpart in the reference (its prover runs one proof at a time):

@@ -60,9 +60,27 @@ template <class H> void Worker<H>::prove_one(int log_n, const void* poly, bool o
     prove<H>(cfg, d, seed, out);
 }
 
+template <class H> size_t Worker<H>::retained() const {
+    size_t b = ctx.pool.held();
+    for (const auto& kv : cfgs) b += kv.second->device_bytes();
+    return b;
+}
+
+template <class H> void Worker<H>::trim(int keep_log_n) {
+    (void)hipSetDevice(ctx.device);
+    for (auto it = cfgs.begin(); it != cfgs.end();) {
+        if (it->first == keep_log_n) ++it;
+        else it = cfgs.erase(it);
+    }
+    ctx.pool.drain();
+}
+
 size_t prove_footprint(int field_bits, int log_n) {
-    // codeword matrix 4N, message and input copies 2N, folds and sumcheck
-    // buffers below N, plus the first Merkle tree (2 * 4N/64 nodes of 32 B)
+    // input copy N, message N and codeword 4N elements of the first commit,
+    // folds, the recursive commits and sumcheck buffers below N elements,
+    // the first Merkle tree (2 * N/16 nodes of 32 B, 4N bytes) and the
+    // recursive trees; 64 MiB cover the configuration tables and the small
+    // buffers of the induction and the openings
     const size_t n = (size_t)1 << log_n, fb = (size_t)field_bits / 8;
     return 7 * n * fb + (n / 8) * 32 + ((size_t)64 << 20);
 }
@@ -122,7 +140,9 @@ void Batch<H>::run(size_t n_jobs, const int* log_n, const void* const* polys, bo
 template <class H> Server<H>::Server(int dev, int n_slots, uint64_t budget) : device(dev), hbm_budget(budget) {
     LIG_REQUIRE(n_slots >= 1 && n_slots <= 64, LIG_E_INVALID, "n_slots must be in [1, 64]");
     for (int i = 0; i < n_slots; i++) workers.emplace_back(new Worker<H>(dev));
-    for (int i = 0; i < n_slots; i++) threads.emplace_back([this, i] { loop(workers[i].get()); });
+    busy.assign(n_slots, 0);
+    charge.assign(n_slots, 0);
+    for (int i = 0; i < n_slots; i++) threads.emplace_back([this, i] { loop((size_t)i); });
 }
 
 template <class H> Server<H>::~Server() {
@@ -139,6 +159,7 @@ template <class H> Server<H>::~Server() {
         queue.clear();
     }
     cv.notify_all();
+    done_cv.notify_all();  // waiters on the failed requests
     for (auto& t : threads) t.join();
 }
 
@@ -159,29 +180,66 @@ template <class H> uint64_t Server<H>::submit(int log_n, const void* poly, uint3
     return t;
 }
 
-template <class H> bool Server<H>::admissible(const Req& r) const {
-    // a request larger than the whole budget runs alone
-    return in_use + r.footprint <= hbm_budget || in_use == 0;
+template <class H> size_t Server<H>::charged() const {
+    size_t b = 0;
+    for (size_t j = 0; j < workers.size(); j++) b += busy[j] ? charge[j] : workers[j]->retained();
+    return b;
 }
 
-template <class H> void Server<H>::loop(Worker<H>* w) {
+template <class H> bool Server<H>::admit(size_t wi, const Req& r) {
+    // Device bytes if r starts on worker wi now: the running workers'
+    // charges, the idle workers' retained memory and wi's retained memory
+    // plus the request's footprint bound.  Idle memory can be given back:
+    // wi keeps only the configuration r will use, the others everything.
+    Worker<H>& w = *workers[wi];
+    size_t running = 0, idle = 0;
+    bool others_busy = false;
+    for (size_t j = 0; j < workers.size(); j++) {
+        if (j == wi) continue;
+        if (busy[j]) {
+            running += charge[j];
+            others_busy = true;
+        } else {
+            idle += workers[j]->retained();
+        }
+    }
+    auto fits = [&](size_t mine) { return running + idle + mine <= hbm_budget; };
+    if (!fits(w.retained() + r.footprint)) {
+        // the least wi can hold is the configuration tables r reuses
+        auto kept = w.cfgs.find(r.log_n);
+        const size_t own_min = kept == w.cfgs.end() ? 0 : kept->second->device_bytes();
+        // a request that cannot fit even with every idle byte returned
+        // waits for running ones, unless nothing runs (then it runs alone)
+        if (others_busy && running + own_min + r.footprint > hbm_budget) return false;
+        w.trim(r.log_n);
+        for (size_t j = 0; j < workers.size() && !fits(w.retained() + r.footprint); j++) {
+            if (j == wi || busy[j]) continue;
+            const size_t before = workers[j]->retained();
+            workers[j]->trim(-1);
+            idle -= before - workers[j]->retained();
+        }
+    }
+    charge[wi] = w.retained() + r.footprint;
+    busy[wi] = 1;
+    return true;
+}
+
+template <class H> void Server<H>::loop(size_t wi) {
+    Worker<H>* w = workers[wi].get();
     for (;;) {
         uint64_t t;
-        size_t fp;
         int log_n;
         const void* poly;
         uint32_t seed;
         {
             std::unique_lock<std::mutex> lk(mu);
             // first come, first admitted: the head waits until it fits
-            cv.wait(lk, [&] { return stopping || (!queue.empty() && admissible(reqs[queue.front()])); });
+            cv.wait(lk, [&] { return stopping || (!queue.empty() && admit(wi, reqs[queue.front()])); });
             if (stopping && queue.empty()) return;
             if (queue.empty()) continue;
             t = queue.front();
             queue.pop_front();
             Req& r = reqs[t];
-            fp = r.footprint;
-            in_use += fp;
             r.t_start = std::chrono::steady_clock::now();
             log_n = r.log_n;
             poly = r.poly;
@@ -204,7 +262,8 @@ template <class H> void Server<H>::loop(Worker<H>* w) {
         }
         {
             std::lock_guard<std::mutex> g(mu);
-            in_use -= fp;
+            busy[wi] = 0;
+            charge[wi] = 0;
             Req& r = reqs[t];
             r.proof = std::move(proof);
             r.status = status;
@@ -220,11 +279,16 @@ template <class H> void Server<H>::loop(Worker<H>* w) {
 template <class H> int Server<H>::take(uint64_t ticket, bool wait, Proof<H>& out, double* latency_ms, std::string& msg) {
     std::unique_lock<std::mutex> lk(mu);
     auto it = reqs.find(ticket);
-    LIG_REQUIRE(it != reqs.end(), LIG_E_INVALID, "unknown ticket");
+    LIG_REQUIRE(it != reqs.end(), LIG_E_INVALID, "unknown or already taken ticket");
     if (!it->second.done) {
         if (!wait) return 0;
-        done_cv.wait(lk, [&] { return reqs[ticket].done; });
+        // another caller may take (and erase) the same ticket meanwhile
+        done_cv.wait(lk, [&] {
+            auto jt = reqs.find(ticket);
+            return jt == reqs.end() || jt->second.done;
+        });
         it = reqs.find(ticket);
+        LIG_REQUIRE(it != reqs.end(), LIG_E_INVALID, "ticket taken by another caller");
     }
     Req& r = it->second;
     const int status = r.status;

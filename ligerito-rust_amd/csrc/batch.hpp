@@ -23,10 +23,16 @@ template <class H> struct Worker {
     Config<H>& config(int log_n);
     // poly: 2^log_n elements in host memory, or device memory if on_device
     void prove_one(int log_n, const void* poly, bool on_device, uint32_t seed, Proof<H>& out);
+    // device bytes the worker holds between proves: its pool (cached
+    // blocks) and the tables of its prover configurations
+    size_t retained() const;
+    // returns the cached blocks to the device and drops every configuration
+    // but the one for keep_log_n (-1: all); only while the worker is idle
+    void trim(int keep_log_n);
 };
 
-// Device bytes one prove of 2^log_n elements holds at its peak (estimate
-// used for admission).
+// Device bytes one prove of 2^log_n elements allocates at most, from an
+// empty pool (upper bound used for admission).
 size_t prove_footprint(int field_bits, int log_n);
 
 template <class H> struct Batch {
@@ -56,9 +62,13 @@ template <class H> struct Server {
     std::deque<uint64_t> queue;
     std::unordered_map<uint64_t, Req> reqs;
     uint64_t last_ticket = 0;
-    size_t in_use = 0;
     bool stopping = false;
     std::vector<std::unique_ptr<Worker<H>>> workers;
+    // per worker: running a request, and the device bytes charged for it
+    // (what the worker held when the request started plus the request's
+    // footprint bound)
+    std::vector<char> busy;
+    std::vector<size_t> charge;
     std::vector<std::thread> threads;
 
     Server(int dev, int n_slots, uint64_t budget);
@@ -67,10 +77,15 @@ template <class H> struct Server {
     // 1 with the proof when done, 0 while pending (wait == false), or the
     // request's negative status; a finished ticket is released.
     int take(uint64_t ticket, bool wait, Proof<H>& out, double* latency_ms, std::string& msg);
+    // device bytes charged against the budget right now (every worker's
+    // retained memory, or its charge while it runs a request)
+    size_t charged() const;
 
 private:
-    bool admissible(const Req& r) const;
-    void loop(Worker<H>* w);
+    // Decides (under mu) whether request r may start on worker wi, trimming
+    // idle workers' caches when that makes it fit; sets charge[wi].
+    bool admit(size_t wi, const Req& r);
+    void loop(size_t wi);
 };
 
 }  // namespace lig

@@ -1,6 +1,7 @@
 // extern "C" boundary (include/ligerito.h).  Every entry point catches
 // library exceptions and returns a status; nothing unwinds across the ABI.
 #include <array>
+#include <atomic>
 #include <exception>
 
 #include "batch.hpp"
@@ -8,21 +9,64 @@
 
 using namespace lig;
 
+// A context is reference counted: the caller's handle holds one reference
+// and every handle made from the context (plans, witnesses, prover
+// configurations) holds another, since they keep pointers into its stream
+// and memory pool.  lig_ctx_destroy drops the caller's reference; the
+// context is torn down when the last reference goes.
 struct lig_ctx {
     Ctx c;
+    std::atomic<int> refs{1};
 };
 
+namespace {
+lig_ctx* ctx_retain(lig_ctx* ctx) {
+    ctx->refs.fetch_add(1, std::memory_order_relaxed);
+    return ctx;
+}
+
+void ctx_release(lig_ctx* ctx) {
+    if (!ctx || ctx->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    ctx->c.release_side();
+    try {
+        ctx->c.collect();
+    } catch (...) {
+    }
+    ctx->c.pool.drain();
+    (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+}
+
+// Holds one reference to a context for the lifetime of a child handle.
+struct CtxRef {
+    lig_ctx* ctx = nullptr;
+    CtxRef() = default;
+    explicit CtxRef(lig_ctx* c) : ctx(ctx_retain(c)) {}
+    CtxRef(const CtxRef&) = delete;
+    CtxRef& operator=(const CtxRef&) = delete;
+    ~CtxRef() { ctx_release(ctx); }
+};
+}  // namespace
+
+// Members are destroyed in reverse order: the device objects go before the
+// context reference that keeps their pool alive.
 struct lig_rs {
     int bits;
+    CtxRef ref;
     lig_ctx* ctx;
     std::unique_ptr<RsPlan<H32>> p32;
     std::unique_ptr<RsPlan<H128>> p128;
+    explicit lig_rs(lig_ctx* c) : ref(c), ctx(c) {}
 };
 
 struct lig_witness {
     int bits;
+    CtxRef ref;
     Witness<H32> w32;
     Witness<H128> w128;
+    explicit lig_witness(lig_ctx* c) : ref(c) {}
 };
 
 struct lig_fs {
@@ -32,8 +76,10 @@ struct lig_fs {
 
 struct lig_prover_config {
     int bits;
+    CtxRef ref;
     Config<H32> c32;
     Config<H128> c128;
+    explicit lig_prover_config(lig_ctx* c) : ref(c) {}
 };
 
 struct lig_proof {
@@ -114,19 +160,7 @@ int lig_ctx_create(int device, lig_ctx** out) {
     });
 }
 
-void lig_ctx_destroy(lig_ctx* ctx) {
-    if (!ctx) return;
-    (void)hipSetDevice(ctx->c.device);
-    (void)hipStreamSynchronize(ctx->c.stream);
-    ctx->c.release_side();
-    try {
-        ctx->c.collect();
-    } catch (...) {
-    }
-    ctx->c.pool.drain();
-    (void)hipStreamDestroy(ctx->c.stream);
-    delete ctx;
-}
+void lig_ctx_destroy(lig_ctx* ctx) { ctx_release(ctx); }
 
 int lig_ctx_profile(lig_ctx* ctx, int enable) {
     return guarded([&] {
@@ -171,9 +205,8 @@ int lig_rs_new(lig_ctx* ctx, int field_bits, size_t message_length, size_t block
         LIG_REQUIRE(ctx && out, LIG_E_INVALID, "NULL argument");
         check_bits(field_bits);
         LIG_HIP(hipSetDevice(ctx->c.device));
-        std::unique_ptr<lig_rs> r(new lig_rs());
+        std::unique_ptr<lig_rs> r(new lig_rs(ctx));
         r->bits = field_bits;
-        r->ctx = ctx;
         if (field_bits == 32) {
             r->p32.reset(new RsPlan<H32>());
             r->p32->init(&ctx->c, message_length, block_length);
@@ -345,7 +378,7 @@ static int commit_impl(lig_rs* rs, const void* poly, size_t m, size_t n, bool on
         LIG_REQUIRE(rs && poly && out, LIG_E_INVALID, "NULL argument");
         Ctx& c = rs->ctx->c;
         LIG_HIP(hipSetDevice(c.device));
-        std::unique_ptr<lig_witness> w(new lig_witness());
+        std::unique_ptr<lig_witness> w(new lig_witness(rs->ctx));
         w->bits = rs->bits;
         if (rs->bits == 32) {
             PBuf<uint32_t> tmp;
@@ -437,16 +470,75 @@ int lig_witness_extract_row(const lig_witness* w, size_t row, void* out) {
 void lig_witness_free(lig_witness* w) { delete w; }
 
 // ------------------------------------------------------------------ config --
+static int config_impl(lig_ctx* ctx, int field_bits, const Shape& sh, lig_prover_config** out) {
+    LIG_REQUIRE(ctx && out, LIG_E_INVALID, "NULL argument");
+    check_bits(field_bits);
+    sh.validate(field_bits / 8);
+    LIG_HIP(hipSetDevice(ctx->c.device));
+    std::unique_ptr<lig_prover_config> c(new lig_prover_config(ctx));
+    c->bits = field_bits;
+    if (field_bits == 32) c->c32.init(&ctx->c, sh);
+    else c->c128.init(&ctx->c, sh);
+    *out = c.release();
+    return LIG_OK;
+}
+
 int lig_prover_config_new(lig_ctx* ctx, int field_bits, int log_n, lig_prover_config** out) {
+    return guarded([&] { return config_impl(ctx, field_bits, Shape::generated(log_n), out); });
+}
+
+int lig_prover_config_from_dims(lig_ctx* ctx, int field_bits, size_t recursive_steps, const size_t initial_dims[2],
+                                size_t initial_k, const size_t* dims, const size_t* ks, lig_prover_config** out) {
     return guarded([&] {
-        LIG_REQUIRE(ctx && out, LIG_E_INVALID, "NULL argument");
-        check_bits(field_bits);
-        LIG_HIP(hipSetDevice(ctx->c.device));
-        std::unique_ptr<lig_prover_config> c(new lig_prover_config());
-        c->bits = field_bits;
-        if (field_bits == 32) c->c32.init(&ctx->c, log_n);
-        else c->c128.init(&ctx->c, log_n);
-        *out = c.release();
+        LIG_REQUIRE(initial_dims && dims && ks, LIG_E_INVALID, "NULL argument");
+        LIG_REQUIRE(recursive_steps >= 1 && recursive_steps <= 8, LIG_E_INVALID,
+                    "recursive_steps must be in [1, 8] (prover.rs:198 panics at 0)");
+        LIG_REQUIRE(initial_k < 40, LIG_E_INVALID, "configuration dimension out of range");
+        Shape sh;
+        sh.recursive_steps = (int)recursive_steps;
+        sh.initial_m = initial_dims[0];
+        sh.initial_n = initial_dims[1];
+        sh.initial_k = (int)initial_k;
+        for (size_t i = 0; i < recursive_steps; i++) {
+            LIG_REQUIRE(ks[i] < 40, LIG_E_INVALID, "configuration dimension out of range");
+            sh.dims_m.push_back(dims[2 * i]);
+            sh.dims_n.push_back(dims[2 * i + 1]);
+            sh.ks.push_back((int)ks[i]);
+        }
+        return config_impl(ctx, field_bits, sh, out);
+    });
+}
+
+static void write_shape(const Shape& sh, size_t* recursive_steps, size_t initial_dims[2], size_t* initial_k, size_t* dims,
+                        size_t* ks, size_t cap_steps) {
+    LIG_REQUIRE(recursive_steps && initial_dims && initial_k, LIG_E_INVALID, "NULL argument");
+    *recursive_steps = (size_t)sh.recursive_steps;
+    initial_dims[0] = sh.initial_m;
+    initial_dims[1] = sh.initial_n;
+    *initial_k = (size_t)sh.initial_k;
+    if (!dims || !ks) return;
+    LIG_REQUIRE((size_t)sh.recursive_steps <= cap_steps, LIG_E_CAPACITY, "more recursive steps than cap_steps");
+    for (int i = 0; i < sh.recursive_steps; i++) {
+        dims[2 * i] = sh.dims_m[i];
+        dims[2 * i + 1] = sh.dims_n[i];
+        ks[i] = (size_t)sh.ks[i];
+    }
+}
+
+int lig_prover_config_shape(int log_n, size_t* recursive_steps, size_t initial_dims[2], size_t* initial_k, size_t* dims,
+                            size_t* ks, size_t cap_steps) {
+    return guarded([&] {
+        write_shape(Shape::generated(log_n), recursive_steps, initial_dims, initial_k, dims, ks, cap_steps);
+        return LIG_OK;
+    });
+}
+
+int lig_prover_config_dims(const lig_prover_config* cfg, size_t* recursive_steps, size_t initial_dims[2],
+                           size_t* initial_k, size_t* dims, size_t* ks, size_t cap_steps) {
+    return guarded([&] {
+        LIG_REQUIRE(cfg, LIG_E_INVALID, "NULL argument");
+        write_shape(cfg->bits == 32 ? cfg->c32.shape() : cfg->c128.shape(), recursive_steps, initial_dims, initial_k,
+                    dims, ks, cap_steps);
         return LIG_OK;
     });
 }
@@ -519,21 +611,58 @@ int lig_proof_deserialize(int field_bits, const uint8_t* buf, size_t len, lig_pr
 size_t lig_proof_size(const lig_proof* p) { return !p ? 0 : p->bits == 32 ? proof_size<H32>(p->p32) : proof_size<H128>(p->p128); }
 void lig_proof_free(lig_proof* p) { delete p; }
 
+static int verify_impl(lig_ctx* ctx, int field_bits, const Shape& sh, const lig_proof* p, uint32_t fs_seed) {
+    LIG_REQUIRE(p, LIG_E_INVALID, "NULL proof");
+    LIG_REQUIRE(p->bits == field_bits, LIG_E_INVALID, "proof field does not match field_bits");
+    Ctx* dev = nullptr;
+    if (ctx) {
+        LIG_HIP(hipSetDevice(ctx->c.device));
+        dev = &ctx->c;
+    }
+    return field_bits == 32 ? verify<H32>(sh, p->p32, fs_seed, dev) : verify<H128>(sh, p->p128, fs_seed, dev);
+}
+
+static Shape verifier_shape(size_t recursive_steps, size_t initial_dim, const size_t* log_dims, size_t initial_k,
+                            const size_t* ks) {
+    LIG_REQUIRE(log_dims && ks, LIG_E_INVALID, "NULL argument");
+    LIG_REQUIRE(recursive_steps >= 1 && recursive_steps <= 8, LIG_E_INVALID,
+                "recursive_steps must be in [1, 8] (prover.rs:198 panics at 0)");
+    int ld[8], kk[8];
+    for (size_t i = 0; i < recursive_steps; i++) {
+        LIG_REQUIRE(log_dims[i] < 40 && ks[i] < 40, LIG_E_INVALID, "configuration dimension out of range");
+        ld[i] = (int)log_dims[i];
+        kk[i] = (int)ks[i];
+    }
+    LIG_REQUIRE(initial_dim < 40 && initial_k < 40, LIG_E_INVALID, "configuration dimension out of range");
+    return Shape::from_logs((int)recursive_steps, (int)initial_dim, ld, (int)initial_k, kk);
+}
+
 int lig_verifier(int field_bits, int log_n, const lig_proof* p, uint32_t fs_seed) {
-    return guarded([&] {
-        LIG_REQUIRE(p, LIG_E_INVALID, "NULL proof");
-        LIG_REQUIRE(p->bits == field_bits, LIG_E_INVALID, "proof field does not match field_bits");
-        return field_bits == 32 ? verify<H32>(log_n, p->p32, fs_seed) : verify<H128>(log_n, p->p128, fs_seed);
-    });
+    return guarded([&] { return verify_impl(nullptr, field_bits, Shape::generated(log_n), p, fs_seed); });
 }
 
 int lig_verifier_dev(lig_ctx* ctx, int field_bits, int log_n, const lig_proof* p, uint32_t fs_seed) {
     return guarded([&] {
-        LIG_REQUIRE(ctx && p, LIG_E_INVALID, "NULL argument");
-        LIG_REQUIRE(p->bits == field_bits, LIG_E_INVALID, "proof field does not match field_bits");
-        LIG_HIP(hipSetDevice(ctx->c.device));
-        return field_bits == 32 ? verify<H32>(log_n, p->p32, fs_seed, &ctx->c)
-                                : verify<H128>(log_n, p->p128, fs_seed, &ctx->c);
+        LIG_REQUIRE(ctx, LIG_E_INVALID, "NULL argument");
+        return verify_impl(ctx, field_bits, Shape::generated(log_n), p, fs_seed);
+    });
+}
+
+int lig_verifier_with_config(int field_bits, size_t recursive_steps, size_t initial_dim, const size_t* log_dims,
+                             size_t initial_k, const size_t* ks, const lig_proof* p, uint32_t fs_seed) {
+    return guarded([&] {
+        return verify_impl(nullptr, field_bits, verifier_shape(recursive_steps, initial_dim, log_dims, initial_k, ks), p,
+                           fs_seed);
+    });
+}
+
+int lig_verifier_with_config_dev(lig_ctx* ctx, int field_bits, size_t recursive_steps, size_t initial_dim,
+                                 const size_t* log_dims, size_t initial_k, const size_t* ks, const lig_proof* p,
+                                 uint32_t fs_seed) {
+    return guarded([&] {
+        LIG_REQUIRE(ctx, LIG_E_INVALID, "NULL argument");
+        return verify_impl(ctx, field_bits, verifier_shape(recursive_steps, initial_dim, log_dims, initial_k, ks), p,
+                           fs_seed);
     });
 }
 

@@ -13,6 +13,8 @@ void* Pool::get(size_t bytes) {
         if (it != free_.end()) {
             void* p = it->second;
             free_.erase(it);
+            cached -= bytes;
+            live += bytes;
             return p;
         }
     }
@@ -23,18 +25,22 @@ void* Pool::get(size_t bytes) {
         drain();  // retry once with the cache returned to the device
         LIG_HIP(hipMalloc(&p, bytes));
     }
+    live += bytes;
     return p;
 }
 
 void Pool::put(void* p, size_t bytes) {
     std::lock_guard<std::mutex> g(mu);
     free_.emplace(bytes, p);
+    live -= bytes;
+    cached += bytes;
 }
 
 void Pool::drain() {
     std::lock_guard<std::mutex> g(mu);
     for (auto& kv : free_) (void)hipFree(kv.second);
     free_.clear();
+    cached = 0;
 }
 
 const char* phase_name(int p) {
@@ -217,22 +223,83 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
 }
 
 // ----------------------------------------------------------------- config --
-template <class H> void Config<H>::init(Ctx* c, int L) {
+Shape Shape::generated(int L) {
     LIG_REQUIRE(L >= 16 && L <= 27, LIG_E_INVALID, "prover config: log_n must be in [16, 27]");
-    ctx = c;
-    log_n = L;
-    recursive_steps = config_steps(L);
-    initial_k = 6;
-    initial_m = (size_t)1 << (L - 6);
-    initial_n = 64;
-    dims_m.clear();
-    dims_n.clear();
-    ks.clear();
-    for (int i = 0; i < recursive_steps; i++) {
-        dims_m.push_back((size_t)1 << (L - 6 - 4 * (i + 1)));
-        dims_n.push_back(16);
-        ks.push_back(4);
+    Shape sh;
+    // config.rs:19-28 / 79-90: one recursive step up to 2^23 coefficients
+    // (cfg20), two at 2^24 (cfg24), one more per further 2^4
+    sh.recursive_steps = L <= 23 ? 1 : 2 + (L - 24) / 4;
+    sh.initial_k = 6;
+    sh.initial_m = (size_t)1 << (L - 6);
+    sh.initial_n = 64;
+    for (int i = 0; i < sh.recursive_steps; i++) {
+        sh.dims_m.push_back((size_t)1 << (L - 6 - 4 * (i + 1)));
+        sh.dims_n.push_back(16);
+        sh.ks.push_back(4);
     }
+    return sh;
+}
+
+Shape Shape::from_logs(int recursive_steps, int initial_dim, const int* log_dims, int initial_k, const int* ks) {
+    LIG_REQUIRE(recursive_steps >= 1 && recursive_steps <= 8, LIG_E_INVALID,
+                "recursive_steps must be in [1, 8] (prover.rs:198 panics at 0)");
+    LIG_REQUIRE(initial_dim >= 0 && initial_dim < 40 && initial_k >= 0 && initial_k < 40, LIG_E_INVALID,
+                "configuration dimension out of range");
+    Shape sh;
+    sh.recursive_steps = recursive_steps;
+    sh.initial_m = (size_t)1 << initial_dim;
+    sh.initial_k = initial_k;
+    sh.initial_n = (size_t)1 << initial_k;
+    for (int i = 0; i < recursive_steps; i++) {
+        LIG_REQUIRE(log_dims[i] >= 0 && log_dims[i] < 40 && ks[i] >= 0 && ks[i] < 40, LIG_E_INVALID,
+                    "configuration dimension out of range");
+        sh.dims_m.push_back((size_t)1 << log_dims[i]);
+        sh.dims_n.push_back((size_t)1 << ks[i]);
+        sh.ks.push_back(ks[i]);
+    }
+    return sh;
+}
+
+void Shape::validate(int field_bytes) const {
+    auto fail = [](const char* what) { throw Error(LIG_E_INVALID, std::string("prover config: ") + what); };
+    if (recursive_steps < 1 || recursive_steps > 8) fail("recursive_steps must be in [1, 8] (prover.rs:198 panics at 0)");
+    if ((int)dims_m.size() != recursive_steps || (int)dims_n.size() != recursive_steps || (int)ks.size() != recursive_steps)
+        fail("dims and ks need one entry per recursive step");
+    if (!is_pow2(initial_m) || !is_pow2(initial_n)) fail("dimensions must be powers of two (ReedSolomonEncoding::new)");
+    // partial_eval must reduce the polynomial to initial_dims.0 rows
+    // (prover.rs:50-53); evaluate_lagrange_basis needs a challenge (utils.rs:9)
+    if (initial_k < 1 || initial_n != ((size_t)1 << initial_k)) fail("initial_k must be log2(initial_dims.1) >= 1");
+    if (initial_m != dims_m[0] * dims_n[0]) fail("initial_dims.0 must equal dims[0].0 * dims[0].1");
+    for (int i = 0; i < recursive_steps; i++) {
+        if (!is_pow2(dims_m[i]) || !is_pow2(dims_n[i])) fail("dimensions must be powers of two (ReedSolomonEncoding::new)");
+        // each step's rounds fold the column variables away (prover.rs:97-102)
+        if (ks[i] < 1 || dims_n[i] != ((size_t)1 << ks[i])) fail("ks[i] must be log2(dims[i].1) >= 1");
+        if (i + 1 < recursive_steps && dims_m[i] != dims_m[i + 1] * dims_n[i + 1])
+            fail("dims[i].0 must equal dims[i+1].0 * dims[i+1].1");
+        // get_distinct_queries(4m, 148) never ends below 148 rows (emulated_fs.rs:65)
+        if (4 * dims_m[i] < 148) fail("every encoded matrix needs at least 148 rows (4 * dims[i].0 >= 148)");
+        if (dims_n[i] * (size_t)field_bytes > 4096) fail("rows above 4 KiB are not supported");
+    }
+    if (4 * initial_m < 148) fail("every encoded matrix needs at least 148 rows (4 * initial_dims.0 >= 148)");
+    if (initial_n * (size_t)field_bytes > 4096) fail("rows above 4 KiB are not supported");
+    // device limits of this library: 2^27 coefficients, FFT quarters below
+    // 2 GiB, the final message (yr) within its 2 MiB staging region
+    if (num_coeffs() > ((size_t)1 << 27)) fail("at most 2^27 coefficients are supported");
+    if (initial_m * initial_n * (size_t)field_bytes >= ((size_t)2 << 30)) fail("FFT quarters must stay below 2 GiB");
+    if (dims_m.back() * (size_t)field_bytes > ((size_t)2 << 20)) fail("the final message must stay within 2 MiB");
+}
+
+template <class H> void Config<H>::init(Ctx* c, const Shape& sh) {
+    sh.validate(H::BYTES);
+    ctx = c;
+    log_n = ilog2(sh.num_coeffs());
+    recursive_steps = sh.recursive_steps;
+    initial_k = sh.initial_k;
+    initial_m = sh.initial_m;
+    initial_n = sh.initial_n;
+    dims_m = sh.dims_m;
+    dims_n = sh.dims_n;
+    ks = sh.ks;
     initial_rs.reset(new RsPlan<H>());
     initial_rs->init(c, initial_m, 4 * initial_m);
     rs.clear();
@@ -240,10 +307,12 @@ template <class H> void Config<H>::init(Ctx* c, int L) {
         rs.emplace_back(new RsPlan<H>());
         rs.back()->init(c, dims_m[i], 4 * dims_m[i]);
     }
-    // induce over 2^ell for ell = L - 6 (initial fold) down by 4 per step
-    sks_by_ell.assign(L + 1, Vec<H>());
-    for (int ell = L - 6; ell >= 0 && ell >= L - 6 - 4 * recursive_steps; ell -= 4)
-        sks_by_ell[ell] = eval_sk_at_vks<H>((size_t)1 << ell);
+    // the inductions run over the first fold's 2^ell entries (ell =
+    // log2 initial_m) and over each non-final step's folded message
+    // (log2 dims_m[i], prover.rs:145-195)
+    sks_by_ell.assign(ilog2(initial_m) + 1, Vec<H>());
+    sks_by_ell[ilog2(initial_m)] = eval_sk_at_vks<H>(initial_m);
+    for (int i = 0; i + 1 < recursive_steps; i++) sks_by_ell[ilog2(dims_m[i])] = eval_sk_at_vks<H>(dims_m[i]);
 }
 
 // ----------------------------------------------------------------- prover --
@@ -496,6 +565,11 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     auto stage_final = [&](const Witness<H>& w) {
         LIG_REQUIRE(w.M == final_M && w.M * w.n * sizeof(T) == final_mat_bytes, LIG_E_ASSERT, "last witness shape");
         hipStream_t ss = ctx->side_stream();
+        // the copies start after the commit's kernels have completed (the
+        // host saw the root's flag, but stream order is what guarantees that
+        // every wave's stores are visible to the copy)
+        LIG_HIP(hipEventRecord(ctx->side_event(), s));
+        LIG_HIP(hipStreamWaitEvent(ss, ctx->side_event(), 0));
         LIG_HIP(hipMemcpyAsync(ctx->pin + PIN_FINAL, w.mat.p, final_mat_bytes, hipMemcpyDeviceToHost, ss));
         LIG_HIP(hipMemcpyAsync(ctx->pin + PIN_FINAL + final_mat_bytes, w.tree.p, (2 * w.M - 1) * 32, hipMemcpyDeviceToHost, ss));
     };

@@ -4,6 +4,7 @@
 #pragma once
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -17,12 +18,17 @@ namespace lig {
 
 // Device-memory cache: freed blocks are kept per exact size and reused, so
 // repeated proofs of one configuration allocate nothing after the first.
+// The byte counters let a server charge a pool's whole device footprint
+// (blocks handed out plus blocks cached) against its memory budget.
 struct Pool {
     std::multimap<size_t, void*> free_;
     std::mutex mu;
+    std::atomic<size_t> live{0}, cached{0};
     void* get(size_t bytes);
     void put(void* p, size_t bytes);
+    // returns every cached block to the device (blocks in use stay)
     void drain();
+    size_t held() const { return live.load() + cached.load(); }
     ~Pool() { drain(); }
 };
 
@@ -120,11 +126,22 @@ struct Ctx {
         if (!side) LIG_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         return side;
     }
+    // Orders the side stream's copies after work on the context stream.
+    hipEvent_t side_ev = nullptr;
+    hipEvent_t side_event() {
+        if (!side_ev) LIG_HIP(hipEventCreateWithFlags(&side_ev, hipEventDisableTiming));
+        return side_ev;
+    }
     void release_side() {
-        if (!side) return;
-        (void)hipStreamSynchronize(side);
-        (void)hipStreamDestroy(side);
-        side = nullptr;
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+            side = nullptr;
+        }
+        if (side_ev) {
+            (void)hipEventDestroy(side_ev);
+            side_ev = nullptr;
+        }
     }
     // Resolves recorded event pairs into per-phase totals.
     void collect();
@@ -202,6 +219,10 @@ template <class H> struct RsPlan {
     void encode_non_systematic_cols(T* d_cols, size_t n);
     // encode (systematic) on one host message.
     void encode_systematic(const T* msg, T* out);
+    // device memory held by the plan's tables
+    size_t device_bytes() const {
+        return (d_tw.n + d_tw_tabs.n + d_pis.n + d_pis_lo.n + d_pis_hi.n) * sizeof(T);
+    }
 };
 
 // RecursiveLigeroWitness<F> (data_structures.rs:32-40), device-resident:
@@ -219,10 +240,31 @@ template <class H> struct Witness {
 // ligero_commit (ligero.rs:70-95) on a device poly (column-major m x n).
 template <class H> void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witness<H>& w, int encode_phase = PH_ENCODE);
 
+// The dimensions of a ProverConfig (data_structures.rs:9-21); its
+// VerifierConfig (:23-30) is the same shape in log form (initial_dim =
+// log2 initial_m, log_dims[i] = log2 dims_m[i]).
+struct Shape {
+    int recursive_steps = 0;
+    size_t initial_m = 0, initial_n = 0;
+    int initial_k = 0;
+    std::vector<size_t> dims_m, dims_n;
+    std::vector<int> ks;
+    // The generated configuration for 2^L coefficients, L in [16, 27]:
+    // hardcoded_config_20 / _24 (config.rs:6-47, 69-112) at L = 20 / 24,
+    // the same rules elsewhere.
+    static Shape generated(int L);
+    // From a VerifierConfig's fields (config.rs:49-67, 114-134).
+    static Shape from_logs(int recursive_steps, int initial_dim, const int* log_dims, int initial_k, const int* ks);
+    // The invariants prover() and verifier() rely on (LIG_E_INVALID with
+    // the broken one otherwise), and this library's size limits.
+    void validate(int field_bytes) const;
+    size_t num_coeffs() const { return initial_m * initial_n; }
+};
+
 // ProverConfig<F> (data_structures.rs:9-21).
 template <class H> struct Config {
     Ctx* ctx = nullptr;
-    int log_n = 0;
+    int log_n = 0;  // log2 of the coefficient count
     int recursive_steps = 0;
     size_t initial_m = 0, initial_n = 0;
     int initial_k = 0;
@@ -232,7 +274,24 @@ template <class H> struct Config {
     std::vector<std::unique_ptr<RsPlan<H>>> rs;
     // eval_sk_at_vks(2^ell) for every ell a prove's sumcheck inductions use
     std::vector<Vec<H>> sks_by_ell;
-    void init(Ctx* c, int L);
+    void init(Ctx* c, int L) { init(c, Shape::generated(L)); }
+    void init(Ctx* c, const Shape& sh);
+    Shape shape() const {
+        Shape sh;
+        sh.recursive_steps = recursive_steps;
+        sh.initial_m = initial_m;
+        sh.initial_n = initial_n;
+        sh.initial_k = initial_k;
+        sh.dims_m = dims_m;
+        sh.dims_n = dims_n;
+        sh.ks = ks;
+        return sh;
+    }
+    size_t device_bytes() const {
+        size_t b = initial_rs ? initial_rs->device_bytes() : 0;
+        for (const auto& r : rs) b += r->device_bytes();
+        return b;
+    }
 };
 
 // RecursiveLigeroProof / FinalLigeroProof parts (data_structures.rs:42-66).
@@ -255,15 +314,13 @@ template <class H> struct Proof {
 
 // prover (prover.rs:15-199) on a device-resident poly.
 template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<H>& out);
-// verifier (verifier.rs:10-164) on the host; 1 accept, 0 reject, <0 panic sites.
-// verifier (verifier.rs:10-164) on the host; with dev, the basis
-// inductions (verifier.rs:54-61,148-155) run on that context's GPU.
-template <class H> int verify(int L, const Proof<H>& p, uint32_t fs_seed, Ctx* dev = nullptr);
+// verifier (verifier.rs:10-164) on the host for a configuration's shape:
+// 1 accept, 0 reject, <0 at the reference's panic sites; with dev, the
+// basis inductions (verifier.rs:54-61,148-155) run on that context's GPU.
+template <class H> int verify(const Shape& sh, const Proof<H>& p, uint32_t fs_seed, Ctx* dev = nullptr);
 template <class H> std::vector<uint8_t> serialize(const Proof<H>& p);
 template <class H> bool deserialize(const uint8_t* b, size_t len, Proof<H>& p);
 template <class H> size_t proof_size(const Proof<H>& p);
 
-// Generated configuration shape (config.rs:6-134 for L = 20, 24).
-inline int config_steps(int L) { return L <= 23 ? 1 : 2 + (L - 24) / 4; }
 
 }  // namespace lig

@@ -191,19 +191,20 @@ void induce_device(Ctx* ctx, int n, const Vec<H>& sks, const std::vector<typenam
 }
 }  // namespace
 
-template <class H> int verify(int L, const Proof<H>& p, uint32_t fs_seed, Ctx* dev) {
+template <class H> int verify(const Shape& sh, const Proof<H>& p, uint32_t fs_seed, Ctx* dev) {
     using T = typename H::T;
     auto induce = [&](int n, const Vec<H>& sks, const std::vector<T>& rows, size_t n_rows, size_t n_cols, const Vec<H>& v_ch,
                       const std::vector<size_t>& queries, T alpha, Vec<H>& basis_out, T& enforced) {
         if (dev) induce_device<H>(dev, n, sks, rows, n_rows, n_cols, v_ch, queries, alpha, basis_out, enforced);
         else induce_host<H>(n, sks, rows, n_rows, n_cols, v_ch, queries, alpha, basis_out, enforced);
     };
-    LIG_REQUIRE(L >= 16 && L <= 27, LIG_E_INVALID, "verifier config: log_n must be in [16, 27]");
-    const int R = config_steps(L), initial_dim = L - 6, initial_k = 6, log_inv_rate = 2;
+    sh.validate(H::BYTES);
+    // VerifierConfig (data_structures.rs:23-30) of the shape
+    const int R = sh.recursive_steps, initial_dim = ilog2(sh.initial_m), initial_k = sh.initial_k, log_inv_rate = 2;
     std::vector<int> log_dims, ks;
     for (int i = 0; i < R; i++) {
-        log_dims.push_back(L - 6 - 4 * (i + 1));
-        ks.push_back(4);
+        log_dims.push_back(ilog2(sh.dims_m[i]));
+        ks.push_back(sh.ks[i]);
     }
     const size_t S = 148;
     if ((int)p.rec_roots.size() < R || p.tr.size() < 3) return LIG_E_MALFORMED;
@@ -306,7 +307,7 @@ template bool deserialize<H32>(const uint8_t*, size_t, Proof<H32>&);
 template bool deserialize<H128>(const uint8_t*, size_t, Proof<H128>&);
 template size_t proof_size<H32>(const Proof<H32>&);
 template size_t proof_size<H128>(const Proof<H128>&);
-template int verify<H32>(int, const Proof<H32>&, uint32_t, Ctx*);
-template int verify<H128>(int, const Proof<H128>&, uint32_t, Ctx*);
+template int verify<H32>(const Shape&, const Proof<H32>&, uint32_t, Ctx*);
+template int verify<H128>(const Shape&, const Proof<H128>&, uint32_t, Ctx*);
 
 }  // namespace lig

@@ -86,6 +86,11 @@ def lib():
             "lig_witness_free": (None, [P]),
             "lig_prover_config_new": (I, [P, I, I, PP]),
             "lig_prover_config_free": (None, [P]),
+            "lig_prover_config_from_dims": (I, [P, I, SZ, P, SZ, P, P, PP]),
+            "lig_prover_config_shape": (I, [I, C.POINTER(SZ), P, C.POINTER(SZ), P, P, SZ]),
+            "lig_prover_config_dims": (I, [P, C.POINTER(SZ), P, C.POINTER(SZ), P, P, SZ]),
+            "lig_verifier_with_config": (I, [I, SZ, SZ, P, SZ, P, P, U32]),
+            "lig_verifier_with_config_dev": (I, [P, I, SZ, SZ, P, SZ, P, P, U32]),
             "lig_prover": (I, [P, P, U32, PP]),
             "lig_prover_dev": (I, [P, P, U32, PP]),
             "lig_proof_serialize": (I, [P, P, SZ, C.POINTER(SZ)]),
@@ -303,10 +308,11 @@ class FS(_Handle):
 
 # ------------------------------------------------------------ ligero_commit --
 class Witness(_Handle):
-    """RecursiveLigeroWitness (data_structures.rs:32-40), device-resident."""
+    """RecursiveLigeroWitness (data_structures.rs:32-40), device-resident.
+    Keeps its plan (and through it the context) referenced."""
 
-    def __init__(self, h, bits):
-        self._h, self.bits = h, bits
+    def __init__(self, h, bits, rs=None):
+        self._h, self.bits, self.rs = h, bits, rs
 
     def __del__(self):
         if self._h:
@@ -348,18 +354,27 @@ class Witness(_Handle):
         return out
 
 
+def _require_len(a, n, what):
+    """The C entry points copy exactly n elements from the pointer: a shorter
+    array would be read past its end (the reference panics on a wrong-length
+    input instead)."""
+    if a.shape[0] != n:
+        raise LigeritoError(-1, f"{what}: {a.shape[0]} elements given, {n} required")
+
+
 def ligero_commit(poly, m: int, n: int, rs: ReedSolomonEncoding) -> Witness:
     """src/ligero.rs:70-95"""
     p = as_elems(poly, rs.bits)
+    _require_len(p, m * n, "ligero_commit (ligero.rs:11-27 reads m*n coefficients)")
     h = C.c_void_p()
     _check(lib().lig_ligero_commit(rs._h, _p(p), m, n, C.byref(h)))
-    return Witness(h, rs.bits)
+    return Witness(h, rs.bits, rs)
 
 
 def ligero_commit_dev(d_poly_ptr: int, m: int, n: int, rs: ReedSolomonEncoding) -> Witness:
     h = C.c_void_p()
     _check(lib().lig_ligero_commit_dev(rs._h, C.c_void_p(d_poly_ptr), m, n, C.byref(h)))
-    return Witness(h, rs.bits)
+    return Witness(h, rs.bits, rs)
 
 
 # ------------------------------------------------------------ prove / verify --
@@ -370,10 +385,50 @@ class ProverConfig(_Handle):
         h = C.c_void_p()
         _check(lib().lig_prover_config_new(ctx.handle, bits, log_n, C.byref(h)))
         self._h, self.bits, self.log_n, self.ctx = h, bits, log_n, ctx
+        self.num_coeffs = 1 << log_n
 
     def __del__(self):
         if self._h:
             self._release("lig_prover_config_free")
+
+
+def _read_shape(fn, *lead):
+    r, ik = C.c_size_t(0), C.c_size_t(0)
+    init = np.zeros(2, dtype=np.uint64)
+    dims = np.zeros(16, dtype=np.uint64)
+    ks = np.zeros(8, dtype=np.uint64)
+    _check(fn(*lead, C.byref(r), _p(init), C.byref(ik), _p(dims), _p(ks), 8))
+    n = r.value
+    return {"recursive_steps": n, "initial_dims": (int(init[0]), int(init[1])), "initial_k": ik.value,
+            "dims": [(int(dims[2 * i]), int(dims[2 * i + 1])) for i in range(n)], "ks": [int(k) for k in ks[:n]]}
+
+
+def config_shape(log_n: int):
+    """The generated configuration for 2^log_n coefficients (no device):
+    a dict with ProverConfig's fields (data_structures.rs:9-21)."""
+    return _read_shape(lib().lig_prover_config_shape, log_n)
+
+
+class ProverConfigFromDims(ProverConfig):
+    """ProverConfig from explicit dimensions (data_structures.rs:9-21), a dict
+    with the keys config_shape returns."""
+
+    def __init__(self, ctx: Context, shape, bits: int = 32):
+        r = int(shape["recursive_steps"])
+        init = np.array(shape["initial_dims"], dtype=np.uint64)
+        dims = np.array([x for d in shape["dims"] for x in d] or [0], dtype=np.uint64)
+        ks = np.array(shape["ks"] or [0], dtype=np.uint64)
+        if len(shape["dims"]) != r or len(shape["ks"]) != r:
+            raise LigeritoError(-1, "dims and ks need one entry per recursive step")
+        h = C.c_void_p()
+        _check(lib().lig_prover_config_from_dims(ctx.handle, bits, r, _p(init), int(shape["initial_k"]), _p(dims), _p(ks),
+                                                 C.byref(h)))
+        self._h, self.bits, self.ctx = h, bits, ctx
+        self.num_coeffs = int(init[0]) * int(init[1])
+        self.log_n = self.num_coeffs.bit_length() - 1
+
+    def shape(self):
+        return _read_shape(lib().lig_prover_config_dims, self._h)
 
 
 def hardcoded_config_20(ctx, bits=32):
@@ -415,6 +470,7 @@ class Proof(_Handle):
 def prover(config: ProverConfig, poly, fs_seed: int = 1234) -> Proof:
     """src/prover.rs:15-199 (host-resident input)."""
     p = as_elems(poly, config.bits)
+    _require_len(p, config.num_coeffs, "prover (prover.rs:15 takes 2^log_n coefficients)")
     h = C.c_void_p()
     _check(lib().lig_prover(config._h, _p(p), fs_seed, C.byref(h)))
     return Proof(h, config.bits)
@@ -430,6 +486,21 @@ def prover_dev(config: ProverConfig, d_poly_ptr: int, fs_seed: int = 1234) -> Pr
 def verifier(log_n: int, proof: Proof, fs_seed: int = 1234) -> int:
     """src/verifier.rs:10-164: 1 accept, 0 reject, negative at the reference's panic sites."""
     return int(lib().lig_verifier(proof.bits, log_n, proof._h, fs_seed))
+
+
+def verifier_with_config(shape, proof: Proof, fs_seed: int = 1234, ctx: Context = None) -> int:
+    """verifier (verifier.rs:10-164) on an explicit VerifierConfig
+    (data_structures.rs:23-30) given as a config_shape-style dict; with ctx
+    the basis inductions run on its GPU."""
+    r = int(shape["recursive_steps"])
+    log_dims = np.array([int(d[0]).bit_length() - 1 for d in shape["dims"]] or [0], dtype=np.uint64)
+    ks = np.array(shape["ks"] or [0], dtype=np.uint64)
+    init_dim = int(shape["initial_dims"][0]).bit_length() - 1
+    if ctx is None:
+        return int(lib().lig_verifier_with_config(proof.bits, r, init_dim, _p(log_dims), int(shape["initial_k"]), _p(ks),
+                                                  proof._h, fs_seed))
+    return int(lib().lig_verifier_with_config_dev(ctx.handle, proof.bits, r, init_dim, _p(log_dims),
+                                                  int(shape["initial_k"]), _p(ks), proof._h, fs_seed))
 
 
 def verifier_dev(ctx: Context, log_n: int, proof: Proof, fs_seed: int = 1234) -> int:
@@ -449,6 +520,8 @@ class Batch(_Handle):
     def prove(self, log_ns, polys, fs_seeds, on_device: bool = False):
         """polys: host arrays, or device pointers (ints) when on_device."""
         n = len(log_ns)
+        if len(polys) != n or len(fs_seeds) != n:
+            raise LigeritoError(-1, "log_ns, polys and fs_seeds differ in length")
         ln = (C.c_int * n)(*[int(x) for x in log_ns])
         seeds = (C.c_uint32 * n)(*[int(x) for x in fs_seeds])
         keep = []
@@ -458,6 +531,7 @@ class Batch(_Handle):
                 ptrs[i] = int(pl)
             else:
                 a = as_elems(pl, self.bits)
+                _require_len(a, 1 << int(log_ns[i]), f"batch job {i}")
                 keep.append(a)
                 ptrs[i] = a.ctypes.data
         out = (C.c_void_p * n)()
@@ -488,6 +562,7 @@ class Server(_Handle):
             ptr, keep = poly, None
         else:
             keep = as_elems(poly, self.bits)
+            _require_len(keep, 1 << int(log_n), "serve request")
             ptr = keep.ctypes.data
         _check(lib().lig_serve_submit(self._h, log_n, C.c_void_p(ptr), fs_seed, C.byref(t)))
         self._keep[t.value] = keep

@@ -72,6 +72,9 @@ def lib():
             "oracle_set_threads": (None, [I]),
             "oracle_prove_timed": (I, [I, I, P, C.c_uint32, C.POINTER(C.c_double)]),
             "oracle_get_threads": (I, []),
+            "oracle_config_shape": (I, [I, C.POINTER(SZ), C.POINTER(SZ), C.POINTER(I), P, P]),
+            "oracle_prove_dims": (I, [I, I, SZ, SZ, I, P, P, P, C.c_uint32, P, SZ, C.POINTER(SZ)]),
+            "oracle_verify_dims": (I, [I, I, I, P, I, P, P, SZ, C.c_uint32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -417,3 +420,52 @@ def verify(proof: bytes, L: int, bits: int = 32, fs_seed: int = 1234) -> int:
     """src/verifier.rs:10-164.  1 accept, 0 reject, negative = reference panic site."""
     buf = np.frombuffer(bytes(proof), dtype=np.uint8).copy()
     return int(lib().oracle_verify(bits, L, _p(buf), buf.shape[0], fs_seed))
+
+
+def config_shape(L: int):
+    """The generated configuration for 2^L as a dict with the fields of
+    ProverConfig (data_structures.rs:9-21): recursive_steps, initial_dims,
+    initial_k, dims, ks."""
+    im, inn, ik = C.c_size_t(0), C.c_size_t(0), C.c_int(0)
+    dims = np.zeros(16, dtype=np.uint64)
+    ks = np.zeros(8, dtype=np.int32)
+    r = lib().oracle_config_shape(L, C.byref(im), C.byref(inn), C.byref(ik), _p(dims), _p(ks))
+    if r < 0:
+        raise RuntimeError(f"unsupported log size {L}")
+    return {"recursive_steps": r, "initial_dims": (im.value, inn.value), "initial_k": ik.value,
+            "dims": [(int(dims[2 * i]), int(dims[2 * i + 1])) for i in range(r)], "ks": [int(k) for k in ks[:r]]}
+
+
+def _dims_args(shape):
+    r = shape["recursive_steps"]
+    dims = np.array([x for d in shape["dims"] for x in d], dtype=np.uint64)
+    ks = np.array(shape["ks"], dtype=np.int32)
+    assert dims.shape[0] == 2 * r and ks.shape[0] == r
+    return r, dims, ks
+
+
+def prove_dims(poly, shape, bits: int = 32, fs_seed: int = 1234) -> bytes:
+    """prover.rs:15-199 on an explicit configuration (a dict as config_shape returns)."""
+    p = as_elems(poly, bits)
+    r, dims, ks = _dims_args(shape)
+    im, inn = shape["initial_dims"]
+    ln = C.c_size_t(0)
+    out = np.zeros(16 << 20, dtype=np.uint8)
+    args = (bits, r, im, inn, shape["initial_k"], _p(dims), _p(ks), _p(p), fs_seed)
+    rc = lib().oracle_prove_dims(*args, _p(out), out.size, C.byref(ln))
+    if rc == -11:
+        out = np.zeros(ln.value, dtype=np.uint8)
+        rc = lib().oracle_prove_dims(*args, _p(out), out.size, C.byref(ln))
+    if rc:
+        raise RuntimeError(f"oracle prover failed: {rc}")
+    return out[: ln.value].tobytes()
+
+
+def verify_dims(proof: bytes, shape, bits: int = 32, fs_seed: int = 1234) -> int:
+    """verifier.rs:10-164 on the VerifierConfig of an explicit configuration."""
+    buf = np.frombuffer(bytes(proof), dtype=np.uint8).copy()
+    r, dims, ks = _dims_args(shape)
+    log_dims = np.array([int(d[0]).bit_length() - 1 for d in shape["dims"]], dtype=np.int32)
+    init_dim = int(shape["initial_dims"][0]).bit_length() - 1
+    return int(lib().oracle_verify_dims(bits, r, init_dim, _p(log_dims), shape["initial_k"], _p(ks), _p(buf),
+                                        buf.shape[0], fs_seed))

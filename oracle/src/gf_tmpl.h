@@ -513,6 +513,27 @@ int FN(config_new)(FN(config) *c, int L) {
     return 0;
 }
 
+/* A configuration with explicit dimensions, the fields of ProverConfig
+ * (data_structures.rs:9-21) as hardcoded_config_20/24 fill them
+ * (config.rs:19-46).  No checks here: the caller passes a consistent shape. */
+int FN(config_from_dims)(FN(config) *c, int recursive_steps, size_t initial_m, size_t initial_n, int initial_k,
+                         const size_t *dims_m, const size_t *dims_n, const int *ks) {
+    if (recursive_steps < 1 || recursive_steps > 8) return -1;
+    memset(c, 0, sizeof(*c));
+    c->recursive_steps = recursive_steps;
+    c->initial_m = initial_m;
+    c->initial_n = initial_n;
+    c->initial_k = initial_k;
+    for (int i = 0; i < recursive_steps; i++) {
+        c->dims_m[i] = dims_m[i];
+        c->dims_n[i] = dims_n[i];
+        c->ks[i] = ks[i];
+    }
+    FN(rs_new)(&c->initial_rs, c->initial_m, c->initial_m * 4);
+    for (int i = 0; i < c->recursive_steps; i++) FN(rs_new)(&c->rs[i], c->dims_m[i], c->dims_m[i] * 4);
+    return 0;
+}
+
 void FN(config_free)(FN(config) *c) {
     FN(rs_free)(&c->initial_rs);
     for (int i = 0; i < c->recursive_steps; i++) FN(rs_free)(&c->rs[i]);

@@ -413,6 +413,80 @@ EXPORT int oracle_prove(int bits, int L, const uint8_t *poly, uint32_t fs_seed, 
     return rc;
 }
 
+/* The generated configuration's shape for 2^L (the rule of gf32_config_new):
+ * dims[2i], dims[2i+1] = (rows, cols) of recursive step i.  Returns the
+ * number of recursive steps, or -10 for an unsupported L. */
+EXPORT int oracle_config_shape(int L, size_t *initial_m, size_t *initial_n, int *initial_k, size_t *dims, int *ks) {
+    gf32_config cfg;
+    if (gf32_config_new(&cfg, L)) return -10;
+    *initial_m = cfg.initial_m;
+    *initial_n = cfg.initial_n;
+    *initial_k = cfg.initial_k;
+    for (int i = 0; i < cfg.recursive_steps; i++) {
+        dims[2 * i] = cfg.dims_m[i];
+        dims[2 * i + 1] = cfg.dims_n[i];
+        ks[i] = cfg.ks[i];
+    }
+    int r = cfg.recursive_steps;
+    gf32_config_free(&cfg);
+    return r;
+}
+
+/* prover.rs:15-199 on a configuration given by its dimensions (ProverConfig,
+ * data_structures.rs:9-21); dims as in oracle_config_shape.  The input has
+ * initial_m * initial_n elements. */
+EXPORT int oracle_prove_dims(int bits, int R, size_t initial_m, size_t initial_n, int initial_k, const size_t *dims,
+                             const int *ks, const uint8_t *poly, uint32_t fs_seed, uint8_t *out, size_t cap,
+                             size_t *len) {
+    int rc = 0;
+    if (R < 1 || R > 8) return -10;
+    size_t dm[8], dn[8];
+    for (int i = 0; i < R; i++) {
+        dm[i] = dims[2 * i];
+        dn[i] = dims[2 * i + 1];
+    }
+#define BODY(T, P)                                                                 \
+    {                                                                              \
+        P##_config cfg;                                                            \
+        if (P##_config_from_dims(&cfg, R, initial_m, initial_n, initial_k, dm, dn, ks)) return -10; \
+        LOADV(T, P, poly, initial_m * initial_n, pv);                              \
+        P##_proof pr;                                                              \
+        rc = P##_prover(&cfg, pv, fs_seed, &pr);                                   \
+        if (rc == 0) {                                                             \
+            size_t sz = P##_proof_serialized_size(&pr);                            \
+            *len = sz;                                                             \
+            if (out) {                                                             \
+                if (cap < sz) rc = -11;                                            \
+                else P##_proof_serialize(&pr, out);                                \
+            }                                                                      \
+            P##_proof_free(&pr);                                                   \
+        }                                                                          \
+        free(pv);                                                                  \
+        P##_config_free(&cfg);                                                     \
+    }
+    DISPATCH(bits, BODY(uint16_t, gf16), BODY(uint32_t, gf32), BODY(unsigned __int128, gf128));
+#undef BODY
+    return rc;
+}
+
+/* verifier.rs:10-164 on an explicit VerifierConfig (data_structures.rs:
+ * 23-30). */
+EXPORT int oracle_verify_dims(int bits, int R, int initial_dim, const int *log_dims, int initial_k, const int *ks,
+                              const uint8_t *proof, size_t len, uint32_t fs_seed) {
+    int rc = 0;
+    if (R < 1 || R > 8) return -10;
+#define BODY(T, P)                                                               \
+    {                                                                            \
+        P##_proof pr;                                                            \
+        if (P##_proof_deserialize(proof, len, &pr)) return -20;                  \
+        rc = P##_verifier(R, initial_dim, log_dims, initial_k, ks, &pr, fs_seed); \
+        P##_proof_free(&pr);                                                     \
+    }
+    DISPATCH(bits, BODY(uint16_t, gf16), BODY(uint32_t, gf32), BODY(unsigned __int128, gf128));
+#undef BODY
+    return rc;
+}
+
 /* The timing the reference's main prints (main.rs:29-49): the configuration
  * (twiddles, pis) is built before the clock starts and the proof bytes are
  * not serialized; *prove_s is the wall time of prover() alone. */

@@ -101,3 +101,35 @@ def test_malformed_proof_rejected():
     with pytest.raises(lh.LigeritoError) as e:
         lh.Proof.from_bytes(b"LGRT" + b"\0" * 10, 32)
     assert e.value.code == -6
+
+
+def _bare(cls, **attrs):
+    """A wrapper object with no library handle: calls that reach the C ABI
+    would fail, so a test passing through one shows the check ran first."""
+    o = cls.__new__(cls)
+    for k, v in attrs.items():
+        setattr(o, k, v)
+    return o
+
+
+def test_wrappers_reject_wrong_length_inputs():
+    # the C entry points copy 2^log_n or m*n elements from the pointer; the
+    # wrappers must refuse a short array before the call
+    cfg = _bare(lh.ProverConfig, _h=None, bits=32, log_n=16, num_coeffs=1 << 16)
+    with pytest.raises(lh.LigeritoError) as e:
+        lh.prover(cfg, np.zeros((1 << 16) - 1, dtype=np.uint32))
+    assert e.value.code == -1
+    rs = _bare(lh.ReedSolomonEncoding, _h=None, bits=128)
+    with pytest.raises(lh.LigeritoError) as e:
+        lh.ligero_commit(np.zeros((64 * 4 - 1, 2), dtype=np.uint64), 64, 4, rs)
+    assert e.value.code == -1
+    b = _bare(lh.Batch, _h=None, bits=32)
+    with pytest.raises(lh.LigeritoError) as e:
+        b.prove([16, 17], [np.zeros(1 << 16, np.uint32), np.zeros(1 << 16, np.uint32)], [1, 2])
+    assert e.value.code == -1
+    with pytest.raises(lh.LigeritoError):
+        b.prove([16], [np.zeros(1 << 16, np.uint32)], [1, 2])
+    s = _bare(lh.Server, _h=None, bits=32, _keep={})
+    with pytest.raises(lh.LigeritoError) as e:
+        s.submit(17, np.zeros(1 << 16, np.uint32))
+    assert e.value.code == -1

@@ -5,6 +5,8 @@ the plain single-proof path's proof of the same input and transcript seed
 (and, for a sample, to the oracle's)."""
 import ctypes as C
 import sys
+import threading
+import time
 
 import numpy as np
 import pytest
@@ -134,3 +136,99 @@ def test_served_tiny_budget_runs_one_at_a_time(O, ctx):
     cfgs = {}
     for t, L, poly, seed in zip(tickets, sizes, polys, seeds):
         assert done[t][0].to_bytes() == plain_bytes(ctx, cfgs, L, 32, poly, seed)
+
+
+def test_config_d_256_jobs_12_streams(O, ctx):
+    # the batched workload at its stated size: 256 polynomials, log2 sizes
+    # uniform in 16..22 (seed 3), each with its own transcript seed, on 12
+    # worker streams; every proof equals the plain path's, one per size the
+    # oracle's
+    rng = np.random.default_rng(3)
+    sizes = [int(x) for x in rng.integers(16, 23, size=256)]
+    say("batch config D: 256 jobs, sizes", sorted(set(sizes)))
+    seeds = [1234 + i for i in range(256)]
+    polys = [O.splitmix(3000 + i, 1 << L, 32) for i, L in enumerate(sizes)]
+    b = lh.Batch(0, 32, n_streams=12)
+    proofs = b.prove(sizes, polys, seeds)
+    b.close()
+    say("batch config D: comparing with the plain path")
+    cfgs = {}
+    first = {}
+    for i, (L, poly, seed, p) in enumerate(zip(sizes, polys, seeds, proofs)):
+        got = p.to_bytes()
+        assert got == plain_bytes(ctx, cfgs, L, 32, poly, seed), f"job {i} (2^{L})"
+        first.setdefault(L, (i, got))
+    for L, (i, got) in sorted(first.items()):
+        say("batch config D: oracle check of job", i, "size", L)
+        assert got == O.prove(polys[i], L, 32, fs_seed=seeds[i]), f"job {i} (2^{L})"
+
+
+def test_config_e_poisson_8_slots(O, ctx):
+    # the served workload: open-loop Poisson arrivals (seed 4), log2 sizes
+    # uniform in 16..24, 8 slots; every result equals the plain path's
+    rng = np.random.default_rng(4)
+    n_req, rate = 48, 400.0
+    sizes = [int(x) for x in rng.integers(16, 25, size=n_req)]
+    gaps = rng.exponential(1.0 / rate, size=n_req)
+    say("serve config E:", n_req, "requests, sizes", sorted(set(sizes)))
+    polys = [O.splitmix(4000 + i, 1 << L, 32) for i, L in enumerate(sizes)]
+    seeds = [4321 + i for i in range(n_req)]
+    srv = lh.Server(0, 32, n_slots=8, hbm_budget=64 << 30)
+    tickets = []
+    t0 = time.perf_counter()
+    due = 0.0
+    for L, p, s, g in zip(sizes, polys, seeds, gaps):
+        due += g
+        while time.perf_counter() - t0 < due:
+            pass
+        tickets.append(srv.submit(L, p, s))
+    results = [srv.wait(t) for t in tickets]
+    srv.stop()
+    say("serve config E: comparing with the plain path")
+    cfgs = {}
+    for i, (L, poly, seed, (p, lat)) in enumerate(zip(sizes, polys, seeds, results)):
+        assert lat > 0
+        assert p.to_bytes() == plain_bytes(ctx, cfgs, L, 32, poly, seed), f"request {i} (2^{L})"
+
+
+def _hip_used_bytes(hip):
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return total.value - free.value
+
+
+def test_served_memory_stays_within_budget(O, ctx):
+    # HBM-budget admission bounds the device memory the server really holds:
+    # what each worker keeps cached between requests is charged too and
+    # given back when a request would not fit otherwise
+    hip = C.CDLL("libamdhip64.so.7")
+    budget = 3 << 30
+    overhead = 3 << 29  # contexts, streams and the runtime's per-queue scratch
+    rng = np.random.default_rng(5)
+    sizes = [int(x) for x in rng.integers(20, 25, size=40)]
+    say("serve memory bound: budget", budget, "sizes", sizes)
+    polys = {L: O.splitmix(5000 + L, 1 << L, 32) for L in set(sizes)}
+    base = _hip_used_bytes(hip)
+    peak = [0]
+    stop = threading.Event()
+
+    def sample():
+        while not stop.is_set():
+            peak[0] = max(peak[0], _hip_used_bytes(hip) - base)
+            time.sleep(0.0005)
+
+    th = threading.Thread(target=sample)
+    th.start()
+    try:
+        srv = lh.Server(0, 32, n_slots=8, hbm_budget=budget)
+        tickets = [srv.submit(L, polys[L], 1000 + i) for i, L in enumerate(sizes)]
+        results = [srv.wait(t) for t in tickets]
+        srv.stop()
+    finally:
+        stop.set()
+        th.join()
+    say("serve memory bound: peak above baseline", peak[0])
+    assert peak[0] <= budget + overhead, f"device memory {peak[0]} above budget {budget} + {overhead}"
+    cfgs = {}
+    for i in (0, len(sizes) - 1):
+        assert results[i][0].to_bytes() == plain_bytes(ctx, cfgs, sizes[i], 32, polys[sizes[i]], 1000 + i)

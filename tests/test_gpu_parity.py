@@ -142,6 +142,22 @@ def test_commit_encode_kernel_shape_gf32(O, ctx):
     assert w.root() == root
 
 
+def test_commit_encode_kernel_shape_gf128(O, ctx):
+    # 1024 independent length-2^12 GF(2^128) rows encoded to 2^14: column
+    # chunks of the GF(2^128) passes beyond 64 columns, 16 KiB leaves
+    m, n = 1 << 12, 1024
+    say("commit gf128 1024x2^12")
+    rs = lh.ReedSolomonEncoding(ctx, m, 4 * m, 128)
+    poly = col(O, 7, m * n, 128)
+    w = lh.ligero_commit(poly, m, n, rs)
+    say("commit gf128 1024x2^12: oracle")
+    root, mat, layers = O.ligero_commit(poly, m, n, 128, want_matrix=True, want_layers=True)
+    assert w.root() == root
+    assert w.tree() == layers
+    for r in (0, 5, 4 * m - 1):
+        assert np.array_equal(w.extract_row(r), mat.reshape(n, 4 * m, 2)[:, r])
+
+
 # inductions run over 2^(L-6) and 2^(L-10) entries: L = 16..23 and 25 here
 # and 24 below give every size from 2^6 to 2^19, hence every accumulate row
 # length (2^6 to 2^10 entries)
@@ -184,6 +200,22 @@ def test_prover_2pow24_bit_exact(O, ctx):
     assert got == want
 
 
+def test_prover_gf128_2pow24_two_steps(O, ctx):
+    # GF(2^128) at 2^24: two recursive commits, the glue step and the
+    # generic fold path; the oracle takes minutes here, so its proof is
+    # pinned by the digest committed under tests/golden
+    say("prove gf128 2^24")
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "oracle_proofs.json")))["L24_seed1_gf128"]
+    cfg = lh.ProverConfig(ctx, 24, 128)
+    poly = col(O, 1, 1 << 24, 128)
+    p = lh.prover(cfg, poly)
+    got = p.to_bytes()
+    assert len(got) == g["bytes"]
+    assert hashlib.sha256(got).hexdigest() == g["sha256"]
+    assert lh.verifier(24, p) == 1
+    assert lh.verifier_dev(ctx, 24, p) == 1
+
+
 @pytest.mark.parametrize("bits,L", [(32, 16), (128, 16), (32, 20)])
 def test_gpu_verifier_matches_host_verifier(O, ctx, bits, L):
     # verifier-side induce on the GPU: same verdicts as the host verifier on
@@ -213,3 +245,63 @@ def test_gpu_verifier_2pow24(O, ctx):
     poly = col(O, 2, 1 << 24, 32)
     p = lh.prover(cfg, poly)
     assert lh.verifier_dev(ctx, 24, p) == 1
+
+
+# ------------------------------------------------ replay of GPU proofs --
+@pytest.mark.parametrize("bits,L", [(32, 16), (128, 16), (32, 20), (32, 24)])
+def test_replay_accepts_gpu_proofs(O, ctx, bits, L):
+    # the independent hashlib replay re-derives every challenge and query set
+    # from the GPU proof's bytes and checks it against the input
+    import replay as RP
+    from test_replay import oracle_commit
+
+    say("replay gpu proof", bits, L)
+    cfg = lh.ProverConfig(ctx, L, bits)
+    poly = col(O, 4, 1 << L, bits)
+    got = lh.prover(cfg, poly, fs_seed=99).to_bytes()
+    RP.replay(got, poly, lh.config_shape(L), bits, oracle_commit(O, bits), fs_seed=99)
+
+
+@pytest.mark.parametrize("which,bits", [("two_step_16", 32), ("two_step_16", 128), ("three_step_17", 32),
+                                        ("cfg24", 32)])
+def test_prover_from_dims_matches_oracle(O, ctx, which, bits):
+    # ProverConfig from explicit dimensions (data_structures.rs:9-21): the
+    # proof equals the oracle's for the same shape, both verifiers accept it
+    # with the matching VerifierConfig, and cfg24's literals give the same
+    # bytes as the generated 2^24 configuration
+    import replay as RP
+    from test_replay import CFG_LITERALS, THREE_STEP_17, TWO_STEP_16, oracle_commit
+
+    shape = {"two_step_16": TWO_STEP_16, "three_step_17": THREE_STEP_17, "cfg24": CFG_LITERALS[24]}[which]
+    say("prove from dims", which, bits)
+    cfg = lh.ProverConfigFromDims(ctx, shape, bits)
+    assert cfg.shape() == shape
+    poly = col(O, 8, cfg.num_coeffs, bits)
+    p = lh.prover(cfg, poly, fs_seed=77)
+    got = p.to_bytes()
+    assert lh.verifier_with_config(shape, p, fs_seed=77) == 1
+    assert lh.verifier_with_config(shape, p, fs_seed=77, ctx=ctx) == 1
+    if which == "cfg24":
+        assert got == lh.prover(lh.ProverConfig(ctx, 24, bits), poly, fs_seed=77).to_bytes()
+        return
+    assert got == O.prove_dims(poly, shape, bits, fs_seed=77)
+    RP.replay(got, poly, shape, bits, oracle_commit(O, bits), fs_seed=77)
+
+
+def test_prover_from_dims_rejects_broken_shapes(ctx):
+    from test_replay import TWO_STEP_16
+
+    say("prove from dims: invalid shapes")
+    bad = []
+    s = dict(TWO_STEP_16, initial_k=5)  # initial_k != log2 initial_dims.1
+    bad.append(s)
+    s = dict(TWO_STEP_16, dims=[(1 << 7, 1 << 2), (1 << 6, 1 << 2)])  # rows chain broken
+    bad.append(s)
+    s = dict(TWO_STEP_16, dims=[(1 << 8, 1 << 2), (1 << 5, 1 << 3)], ks=[2, 3])  # 4 * 32 < 148 rows
+    bad.append(s)
+    s = dict(TWO_STEP_16, recursive_steps=0, dims=[], ks=[])  # prover.rs:198
+    bad.append(s)
+    for sh in bad:
+        with pytest.raises(lh.LigeritoError) as e:
+            lh.ProverConfigFromDims(ctx, sh, 32)
+        assert e.value.code == -1
