@@ -300,9 +300,20 @@ def _batched(lh, torch, np, cfg24, ptr24, n_jobs, streams=12):
     total = sum(1 << L for L in sizes)
     b = lh.Batch(0, 32, n_streams=streams)
     b.prove(sizes, ptrs, seeds, on_device=True)  # builds each worker's configurations
-    t0 = time.perf_counter()
+    # each figure is the median of `reps` timed runs (one run of the batch
+    # is ~20 ms, so single runs vary with the host's scheduling)
+    reps = 5
+
+    def med(fn):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[reps // 2]
+
+    t_batch = med(lambda: b.prove(sizes, ptrs, seeds, on_device=True))
     proofs = b.prove(sizes, ptrs, seeds, on_device=True)
-    t_batch = time.perf_counter() - t0
     for i in (0, n_jobs // 2, n_jobs - 1):
         if lh.verifier(sizes[i], proofs[i], seeds[i]) != 1:
             raise SystemExit(f"verifier rejected batched proof {i}")
@@ -311,25 +322,27 @@ def _batched(lh, torch, np, cfg24, ptr24, n_jobs, streams=12):
     cfgs = {L: lh.ProverConfig(ctx, L, 32) for L in set(sizes)}
     lh.prover_dev(cfgs[sizes[0]], ptrs[0], seeds[0])
     ctx.synchronize()
-    t0 = time.perf_counter()
-    for L, p, s in zip(sizes, ptrs, seeds):
-        lh.prover_dev(cfgs[L], p, s)
-    ctx.synchronize()
-    t_plain = time.perf_counter() - t0
-    log(f"batched {n_jobs} jobs: {t_batch * 1e3:.1f} ms batched, {t_plain * 1e3:.1f} ms plain")
+
+    def plain_all():
+        for L, p, s in zip(sizes, ptrs, seeds):
+            lh.prover_dev(cfgs[L], p, s)
+        ctx.synchronize()
+
+    t_plain = med(plain_all)
+    log(f"batched {n_jobs} jobs: {t_batch * 1e3:.1f} ms batched, {t_plain * 1e3:.1f} ms plain (medians of {reps})")
     # eight 2^24 proves
     n24 = 8
     ptr_list = [ptr24] * n24
     seeds24 = [1234 + i for i in range(n24)]
     b.prove([24] * n24, ptr_list, seeds24, on_device=True)
-    t0 = time.perf_counter()
-    b.prove([24] * n24, ptr_list, seeds24, on_device=True)
-    t_b24 = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for s in seeds24:
-        lh.prover_dev(cfg24, ptr24, s)
-    t_p24 = time.perf_counter() - t0
-    log(f"eight 2^24 proves: {t_b24 * 1e3:.1f} ms batched, {t_p24 * 1e3:.1f} ms plain")
+    t_b24 = med(lambda: b.prove([24] * n24, ptr_list, seeds24, on_device=True))
+
+    def plain24():
+        for s in seeds24:
+            lh.prover_dev(cfg24, ptr24, s)
+
+    t_p24 = med(plain24)
+    log(f"eight 2^24 proves: {t_b24 * 1e3:.1f} ms batched, {t_p24 * 1e3:.1f} ms plain (medians of {reps})")
     b.close()
     return {
         "workload": f"{n_jobs} GF(2^32) polynomials, log2 sizes uniform in 16..22 (seed 3), own transcript seeds, "

@@ -561,17 +561,36 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
 
     // The last witness is copied to pinned memory on the side stream while
     // the last sumcheck section runs; the final openings then need no GPU
-    // round trip.
+    // round trip.  The copies are issued only once the host has seen an
+    // event recorded after the witness's commit complete: then every store
+    // of the commit's kernels is visible to them by stream semantics, with
+    // no device-side wait between the streams (a wait of the side stream
+    // on the context stream's event held up the hardware queue the side
+    // stream shares with other contexts' streams).
+    const T* final_mat = nullptr;
+    const uint32_t* final_tree = nullptr;
     auto stage_final = [&](const Witness<H>& w) {
         LIG_REQUIRE(w.M == final_M && w.M * w.n * sizeof(T) == final_mat_bytes, LIG_E_ASSERT, "last witness shape");
-        hipStream_t ss = ctx->side_stream();
-        // the copies start after the commit's kernels have completed (the
-        // host saw the root's flag, but stream order is what guarantees that
-        // every wave's stores are visible to the copy)
         LIG_HIP(hipEventRecord(ctx->side_event(), s));
-        LIG_HIP(hipStreamWaitEvent(ss, ctx->side_event(), 0));
-        LIG_HIP(hipMemcpyAsync(ctx->pin + PIN_FINAL, w.mat.p, final_mat_bytes, hipMemcpyDeviceToHost, ss));
-        LIG_HIP(hipMemcpyAsync(ctx->pin + PIN_FINAL + final_mat_bytes, w.tree.p, (2 * w.M - 1) * 32, hipMemcpyDeviceToHost, ss));
+        final_mat = w.mat.p;
+        final_tree = w.tree.p;
+    };
+    // issues the copies if the commit is known complete (or, with block,
+    // once it is)
+    auto issue_final = [&](bool block) {
+        if (!final_mat) return;
+        if (!block) {
+            const hipError_t e = hipEventQuery(ctx->side_event());
+            if (e == hipErrorNotReady) return;
+            LIG_HIP(e);
+        } else {
+            LIG_HIP(hipEventSynchronize(ctx->side_event()));
+        }
+        hipStream_t ss = ctx->side_stream();
+        LIG_HIP(hipMemcpyAsync(ctx->pin + PIN_FINAL, final_mat, final_mat_bytes, hipMemcpyDeviceToHost, ss));
+        LIG_HIP(hipMemcpyAsync(ctx->pin + PIN_FINAL + final_mat_bytes, final_tree, (2 * final_M - 1) * 32,
+                               hipMemcpyDeviceToHost, ss));
+        final_mat = nullptr;
     };
 
     Witness<H> wprev;
@@ -710,11 +729,13 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
         const WordCopy lc = last_triple_copy();
         open_launch_nodes<H>(w0, q, *pending, &lc);
     }
+    issue_final(false);
     w0 = Witness<H>();  // the first witness is released once its kernels ran
 
     for (int i = 0; i < cfg.recursive_steps; i++) {
         if (i == cfg.recursive_steps - 1) {
             // final step (prover.rs:104-142): yr, queries, openings
+            issue_final(true);  // the last commit completed long ago
             T* hy = (T*)(ctx->pin + PIN_YR);
             LIG_REQUIRE(PIN_YR + flen * sizeof(T) <= ((size_t)4 << 20), LIG_E_INVALID, "yr exceeds the staging region");
             // written by a kernel straight into the mapped pinned buffer
@@ -767,6 +788,7 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
             const WordCopy lc = last_triple_copy();
             open_launch_nodes<H>(wprev, q, *next, &lc);
         }
+        issue_final(false);
         open_finish<H>(*pending);  // the previous opening, while this section runs
         pending = std::move(next);
         wprev = std::move(wnext);
