@@ -249,6 +249,7 @@ int lig_rs_encode_non_systematic_cols_dev(lig_rs* rs, void* d_columns, size_t n_
         LIG_HIP(hipSetDevice(rs->ctx->c.device));
         if (rs->bits == 32) rs->p32->encode_non_systematic_cols((uint32_t*)d_columns, n_cols);
         else rs->p128->encode_non_systematic_cols((u128v*)d_columns, n_cols);
+        rs->ctx->c.sync();
         return LIG_OK;
     });
 }

@@ -102,31 +102,33 @@ template <class H> void RsPlan<H>::init(Ctx* c, size_t msg_len, size_t block_len
 // reed_solomon.rs:104-113 over n columns: pis scaling, then the full FFT
 // (including the first two stages, which are replication only when the
 // tail beyond m is zero).
-template <class H> void RsPlan<H>::encode_non_systematic_cols(T* d_cols, size_t n) {
+template <class H> void RsPlan<H>::encode_non_systematic_cols(T* d_cols, size_t n, T* scratch) {
     using F = typename DevField<H>::F;
     LIG_REQUIRE(is_pow2(n), LIG_E_INVALID, "encode_non_systematic: the column count must be a power of two");
     hipStream_t s = ctx->stream;
     const size_t M_ = M(), m_ = m();
-    // column-major -> row-major [M][n]
-    PBuf<T> rm(&ctx->pool, M_ * n);
-    std::vector<T> host(M_ * n), rmh(M_ * n);
-    LIG_HIP(hipMemcpyAsync(host.data(), d_cols, M_ * n * sizeof(T), hipMemcpyDeviceToHost, s));
-    ctx->sync();
-    for (size_t j = 0; j < n; j++)
-        for (size_t i = 0; i < M_; i++) rmh[i * n + j] = host[j * M_ + i];
-    LIG_HIP(hipMemcpyAsync(rm.p, rmh.data(), M_ * n * sizeof(T), hipMemcpyHostToDevice, s));
-    launch_scale_rows<F>(rm.p, m_, n, d_pis.p, s);
+    // the FFT kernels work on the row-major [M][n] matrix; one column is
+    // that matrix already
+    PBuf<T> own;
+    T* rm = d_cols;
+    if (n > 1) {
+        if (!scratch) {
+            own = PBuf<T>(&ctx->pool, M_ * n);
+            scratch = own.p;
+        }
+        rm = scratch;
+        launch_transpose<F>(d_cols, n, M_, rm, s);  // column-major [n][M] -> [M][n]
+    }
+    launch_scale_rows<F>(rm, m_, n, d_pis.p, s);
     // two leading stages with the general butterfly, then the quarter passes
     int stages = log_block < 2 ? log_block : 2;
-    for (int st = 0; st < stages; st++) launch_fft_stage<F>(rm.p, M_, n, st, d_tw.p, 0, s);
-    if (log_block >= 2) launch_fft_quarters<F>(nullptr, rm.p, M_ / 4, n, d_tw.p, d_tw_tabs.p, s);
+    for (int st = 0; st < stages; st++) launch_fft_stage<F>(rm, M_, n, st, d_tw.p, 0, s);
+    if (log_block >= 2) launch_fft_quarters<F>(nullptr, rm, M_ / 4, n, d_tw.p, d_tw_tabs.p, s);
+    if (n > 1) launch_transpose<F>(rm, M_, n, d_cols, s);
     LIG_HIP(hipGetLastError());
-    LIG_HIP(hipMemcpyAsync(rmh.data(), rm.p, M_ * n * sizeof(T), hipMemcpyDeviceToHost, s));
-    ctx->sync();
-    for (size_t j = 0; j < n; j++)
-        for (size_t i = 0; i < M_; i++) host[j * M_ + i] = rmh[i * n + j];
-    LIG_HIP(hipMemcpyAsync(d_cols, host.data(), M_ * n * sizeof(T), hipMemcpyHostToDevice, s));
-    ctx->sync();
+    // the pooled buffer returns to the pool after the kernels that use it
+    // (stream order); the caller's scratch stays busy until the stream
+    // reaches this point
 }
 
 // reed_solomon.rs:19-41

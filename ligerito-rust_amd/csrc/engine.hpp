@@ -215,8 +215,10 @@ template <class H> struct RsPlan {
     size_t m() const { return (size_t)1 << log_msg; }
     size_t M() const { return (size_t)1 << log_block; }
     void init(Ctx* c, size_t msg_len, size_t block_len);
-    // encode_non_systematic on n columns, column-major, in place (device).
-    void encode_non_systematic_cols(T* d_cols, size_t n);
+    // encode_non_systematic on n columns, column-major, in place (device),
+    // asynchronous on the context stream.  For n > 1 the kernels work on a
+    // row-major copy in scratch (M*n elements; from the pool if null).
+    void encode_non_systematic_cols(T* d_cols, size_t n, T* scratch = nullptr);
     // encode (systematic) on one host message.
     void encode_systematic(const T* msg, T* out);
     // device memory held by the plan's tables

@@ -6,7 +6,9 @@
 // arrays.  All arithmetic is exact binary-field arithmetic.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <stdexcept>
+#include <string>
 #include <type_traits>
 
 #include "blake3.hpp"
@@ -226,6 +228,33 @@ __global__ void k_scale_rows(typename F::T* __restrict__ buf, size_t rows, size_
 template <class F> void launch_scale_rows(typename F::T* buf, size_t rows, size_t n, const typename F::T* pis, hipStream_t s) {
     if (!rows) return;
     hipLaunchKernelGGL(k_scale_rows<F>, dim3(grid_for(rows * n, 256)), dim3(256), 0, s, buf, rows, n, pis);
+}
+
+// dst[c*rows + r] = src[r*cols + c]: 32 x 32 tiles through LDS (one padding
+// column so a tile column is read across banks), 256 threads per tile, the
+// grid striding over tiles.
+template <class F>
+__global__ __launch_bounds__(256) void k_transpose(const typename F::T* __restrict__ src, size_t rows, size_t cols,
+                                                   typename F::T* __restrict__ dst) {
+    using T = typename F::T;
+    __shared__ T t[32][33];
+    const size_t ty = (rows + 31) / 32, tx = (cols + 31) / 32;
+    const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+    for (size_t tile = blockIdx.x; tile < tx * ty; tile += gridDim.x) {
+        const size_t r0 = (tile / tx) * 32, c0 = (tile % tx) * 32;
+        for (int y = ly; y < 32; y += 8)
+            if (r0 + y < rows && c0 + lx < cols) t[y][lx] = src[(r0 + y) * cols + c0 + lx];
+        __syncthreads();
+        for (int y = ly; y < 32; y += 8)
+            if (c0 + y < cols && r0 + lx < rows) dst[(c0 + y) * rows + r0 + lx] = t[lx][y];
+        __syncthreads();
+    }
+}
+
+template <class F> void launch_transpose(const typename F::T* src, size_t rows, size_t cols, typename F::T* dst, hipStream_t s) {
+    if (!rows || !cols) return;
+    const size_t tiles = ((rows + 31) / 32) * ((cols + 31) / 32);
+    hipLaunchKernelGGL(k_transpose<F>, dim3(grid_for(tiles, 1, 1u << 20)), dim3(256), 0, s, src, rows, cols, dst);
 }
 
 // binary_fft.rs:180-206, one stage over all blocks; twiddle index of block b
@@ -2024,6 +2053,7 @@ void launch_sumcheck_fold_step(const typename F::T* f, const typename F::T* b, s
     template void launch_transpose_scale<F>(const F::T*, size_t, size_t, const F::T*, const F::T*, const F::T*, F::T*,      \
                                             hipStream_t);                                                                   \
     template void launch_scale_rows<F>(F::T*, size_t, size_t, const F::T*, hipStream_t);                                    \
+    template void launch_transpose<F>(const F::T*, size_t, size_t, F::T*, hipStream_t);                                     \
     template void launch_fft_stage<F>(F::T*, size_t, size_t, int, const F::T*, int, hipStream_t);                           \
     template void launch_fft_quarters<F>(const F::T*, F::T*, size_t, size_t, const F::T*, const F::T*, hipStream_t,         \
                                          PassObserver*);                     \

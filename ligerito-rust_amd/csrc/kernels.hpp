@@ -54,6 +54,9 @@ void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const
                             const typename F::T* lo_tabs, const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s);
 // buf[i*n + c] *= pis[i] for i < rows (general multiply).
 template <class F> void launch_scale_rows(typename F::T* buf, size_t rows, size_t n, const typename F::T* pis, hipStream_t s);
+// dst[c*rows + r] = src[r*cols + c] (src row-major rows x cols; dst != src).
+template <class F> void launch_transpose(const typename F::T* src, size_t rows, size_t cols, typename F::T* dst, hipStream_t s);
+
 // One additive-FFT stage on a row-major [N][n] buffer with general multiply
 // (binary_fft.rs:180-206 butterflies); inverse selects the IFFT butterfly.
 template <class F>

@@ -222,6 +222,12 @@ class ReedSolomonEncoding(_Handle):
         _check(lib().lig_rs_encode(self._h, _p(m), _p(out)))
         return out
 
+    def encode_non_systematic_cols_dev(self, d_columns: int, n_cols: int):
+        """encode_non_systematic in place on n_cols device-resident columns
+        (column-major, block_length elements each), as encode_cols
+        (ligero.rs:29-38) applies it."""
+        _check(lib().lig_rs_encode_non_systematic_cols_dev(self._h, C.c_void_p(d_columns), n_cols))
+
     def encode_non_systematic(self, message):
         """Returns the encoded block (the reference encodes in place)."""
         m = as_elems(message, self.bits).copy()

@@ -305,3 +305,25 @@ def test_prover_from_dims_rejects_broken_shapes(ctx):
         with pytest.raises(lh.LigeritoError) as e:
             lh.ProverConfigFromDims(ctx, sh, 32)
         assert e.value.code == -1
+
+
+@pytest.mark.parametrize("bits,logm,n", [(32, 0, 4), (32, 6, 2), (32, 8, 16), (32, 10, 64), (32, 7, 128),
+                                         (128, 6, 4), (128, 8, 64), (128, 5, 128)])
+def test_rs_encode_cols_dev_matches_oracle(O, ctx, bits, logm, n):
+    # encode_cols (ligero.rs:29-38) on device-resident columns: column-major
+    # full blocks (nonzero tails too), encoded in place on the device
+    from hipmem import DeviceBuffer
+
+    m = 1 << logm
+    M = 4 * m
+    say("rs encode cols dev", bits, m, n)
+    rs = lh.ReedSolomonEncoding(ctx, m, M, bits)
+    cols = col(O, 40 + logm + n, M * n, bits)
+    d = DeviceBuffer.of(cols)
+    rs.encode_non_systematic_cols_dev(d.value, n)
+    got = d.download(cols)
+    d.free()
+    shape = (n, M) if bits == 32 else (n, M, 2)
+    got, cols = got.reshape(shape), cols.reshape(shape)
+    for j in range(n):
+        assert np.array_equal(got[j], O.rs_encode_non_systematic(cols[j].copy(), m, bits)), f"column {j}"
