@@ -88,6 +88,16 @@ int lig_rs_encode_non_systematic(lig_rs *rs, void *message);
  * (ligero.rs:29-38) applies it.  n_cols must be a power of two
  * (LIG_E_INVALID otherwise).  Synchronous. */
 int lig_rs_encode_non_systematic_cols_dev(lig_rs *rs, void *d_columns, size_t n_cols);
+/* No counterpart in the reference: the same encode for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree or
+ * allocating routine: its row-major working copy lives in a static 64 MiB
+ * device array, one per GPU, held from the library's load to the end of the
+ * process.  block_length * n_cols * field_bits/8 must not exceed 64 MiB
+ * (LIG_E_INVALID otherwise).  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (from several threads or contexts) race.
+ * Synchronous; results equal lig_rs_encode_non_systematic_cols_dev's. */
+int lig_rs_encode_non_systematic_cols_dev_noalloc(lig_rs *rs, void *d_columns, size_t n_cols);
 /* compute_twiddles output of this plan (binary_fft.rs:30-58): block_length-1
  * elements. */
 int lig_rs_twiddles(const lig_rs *rs, void *out);

@@ -155,6 +155,13 @@ int lig_ctx_create(int device, lig_ctx** out) {
         std::unique_ptr<lig_ctx> c(new lig_ctx());
         c->c.device = device;
         LIG_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        // set-up of the no-allocation entry point's scratch on this device
+        // (its address looked up once, with the device current); a failure
+        // here is reported by that entry point, which looks again
+        try {
+            (void)noalloc_scratch_ptr(device);
+        } catch (const std::exception&) {
+        }
         *out = c.release();
         return LIG_OK;
     });
@@ -250,6 +257,27 @@ int lig_rs_encode_non_systematic_cols_dev(lig_rs* rs, void* d_columns, size_t n_
         if (rs->bits == 32) rs->p32->encode_non_systematic_cols((uint32_t*)d_columns, n_cols);
         else rs->p128->encode_non_systematic_cols((u128v*)d_columns, n_cols);
         rs->ctx->c.sync();
+        return LIG_OK;
+    });
+}
+
+int lig_rs_encode_non_systematic_cols_dev_noalloc(lig_rs* rs, void* d_columns, size_t n_cols) {
+    return guarded([&] {
+        LIG_REQUIRE(rs && d_columns && n_cols, LIG_E_INVALID, "bad argument");
+        const size_t row_bytes = lig_rs_block_length(rs) * elem_bytes(rs->bits);
+        LIG_REQUIRE(n_cols <= NOALLOC_SCRATCH_BYTES / row_bytes, LIG_E_INVALID,
+                    "encode_non_systematic (no allocation): block_length * n_cols elements exceed the 64 MiB scratch");
+        Ctx& c = rs->ctx->c;
+        LIG_HIP(hipSetDevice(c.device));
+        void* scratch = nullptr;
+        try {
+            scratch = noalloc_scratch_ptr(c.device);
+        } catch (const std::exception& e) {
+            throw Error(LIG_E_HIP, e.what());
+        }
+        if (rs->bits == 32) rs->p32->encode_non_systematic_cols((uint32_t*)d_columns, n_cols, (uint32_t*)scratch);
+        else rs->p128->encode_non_systematic_cols((u128v*)d_columns, n_cols, (u128v*)scratch);
+        c.sync();
         return LIG_OK;
     });
 }

@@ -257,6 +257,34 @@ template <class F> void launch_transpose(const typename F::T* src, size_t rows, 
     hipLaunchKernelGGL(k_transpose<F>, dim3(grid_for(tiles, 1, 1u << 20)), dim3(256), 0, s, src, rows, cols, dst);
 }
 
+// The scratch of the entry points that allocate nothing: one static array
+// per device (each GPU the library runs on holds its own copy until the
+// process ends).  Host code never passes `noalloc_scratch` itself to a
+// kernel: the name is a host-side stand-in, and the device address comes
+// from hipGetSymbolAddress with the device current.
+__device__ __align__(256) uint8_t noalloc_scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex noalloc_mu;
+void* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+void* noalloc_scratch_ptr(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) throw std::invalid_argument("device index beyond the scratch table");
+    std::lock_guard<std::mutex> g(noalloc_mu);
+    if (!noalloc_ptr[device]) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess || cur != device)
+            throw std::runtime_error("noalloc scratch: the device must be current");
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("hipGetSymbolAddress(noalloc_scratch) failed: ") + hipGetErrorString(e));
+        noalloc_ptr[device] = p;
+    }
+    return noalloc_ptr[device];
+}
+
 // binary_fft.rs:180-206, one stage over all blocks; twiddle index of block b
 // at stage `stage` is 2^stage - 1 + b (heap order, binary_fft.rs:127-139).
 template <class F>

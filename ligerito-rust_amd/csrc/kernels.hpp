@@ -56,6 +56,12 @@ void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const
 template <class F> void launch_scale_rows(typename F::T* buf, size_t rows, size_t n, const typename F::T* pis, hipStream_t s);
 // dst[c*rows + r] = src[r*cols + c] (src row-major rows x cols; dst != src).
 template <class F> void launch_transpose(const typename F::T* src, size_t rows, size_t cols, typename F::T* dst, hipStream_t s);
+// Static device scratch of the entry points that allocate nothing:
+// NOALLOC_SCRATCH_BYTES per device.  noalloc_scratch_ptr returns its device
+// address on `device`, which must be current (looked up once per device).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+void* noalloc_scratch_ptr(int device);
 
 // One additive-FFT stage on a row-major [N][n] buffer with general multiply
 // (binary_fft.rs:180-206 butterflies); inverse selects the IFFT butterfly.

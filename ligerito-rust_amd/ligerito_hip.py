@@ -64,6 +64,7 @@ def lib():
             "lig_rs_encode": (I, [P, P, P]),
             "lig_rs_encode_non_systematic": (I, [P, P]),
             "lig_rs_encode_non_systematic_cols_dev": (I, [P, P, SZ]),
+            "lig_rs_encode_non_systematic_cols_dev_noalloc": (I, [P, P, SZ]),
             "lig_rs_twiddles": (I, [P, P]),
             "lig_build_merkle_tree": (I, [P, P, SZ, SZ, P]),
             "lig_merkle_prove": (I, [P, SZ, P, SZ, P, SZ, C.POINTER(SZ)]),
@@ -227,6 +228,11 @@ class ReedSolomonEncoding(_Handle):
         (column-major, block_length elements each), as encode_cols
         (ligero.rs:29-38) applies it."""
         _check(lib().lig_rs_encode_non_systematic_cols_dev(self._h, C.c_void_p(d_columns), n_cols))
+
+    def encode_non_systematic_cols_dev_noalloc(self, d_columns: int, n_cols: int):
+        """The same with no device allocation (a static 64 MiB scratch per
+        GPU; calls that overlap on one GPU race)."""
+        _check(lib().lig_rs_encode_non_systematic_cols_dev_noalloc(self._h, C.c_void_p(d_columns), n_cols))
 
     def encode_non_systematic(self, message):
         """Returns the encoded block (the reference encodes in place)."""
