@@ -62,6 +62,17 @@ void lig_ctx_destroy(lig_ctx *ctx);
  * the totals. */
 int lig_ctx_profile(lig_ctx *ctx, int enable);
 int lig_ctx_profile_read(lig_ctx *ctx, const char *phase, double *total_ms, uint64_t *count);
+/* Microbenchmarks for the rooflines (no counterpart in the reference):
+ * which = 0: GF(2^32) products per second by wave-uniform constants with
+ * the FFT passes' nibble tables in LDS; 1: general GF(2^32) products per
+ * second (carry-less product from integer multiplies, as the induce and
+ * sumcheck kernels multiply); 2: 32-bit integer VALU instructions per
+ * second summed over lanes; 3: GF(2^32) products per second by constants
+ * with 5-bit-window tables (seven 4-byte LDS reads per product instead of
+ * eight); 4: LDS bytes read per second with the GF(2^128) table pattern
+ * (16-byte reads, 16 distinct entries per wave instruction).  Best of five
+ * launches on the context's GPU. */
+int lig_bench_rate(lig_ctx *ctx, int which, double *rate);
 /* The hipStream_t all work of this context is ordered on. */
 void *lig_ctx_stream(lig_ctx *ctx);
 int lig_ctx_synchronize(lig_ctx *ctx);

@@ -196,6 +196,21 @@ int lig_ctx_profile_read(lig_ctx* ctx, const char* phase, double* total_ms, uint
     });
 }
 
+int lig_bench_rate(lig_ctx* ctx, int which, double* rate) {
+    return guarded([&] {
+        LIG_REQUIRE(ctx && rate, LIG_E_INVALID, "NULL argument");
+        LIG_REQUIRE(which >= 0 && which <= 4, LIG_E_INVALID, "unknown microbenchmark");
+        LIG_HIP(hipSetDevice(ctx->c.device));
+        PBuf<uint32_t> word(&ctx->c.pool, 1);
+        try {
+            *rate = bench_rate(which, word.p, ctx->c.stream);
+        } catch (const std::exception& e) {
+            throw Error(LIG_E_HIP, e.what());
+        }
+        return LIG_OK;
+    });
+}
+
 void* lig_ctx_stream(lig_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
 
 int lig_ctx_synchronize(lig_ctx* ctx) {

@@ -192,6 +192,10 @@ __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __res
     }
 }
 
+constexpr int TS128_ROWS = 4;  // message rows per k_transpose_scale128 tile
+__global__ __launch_bounds__(256) void k_transpose_scale128(const u128v* __restrict__ poly, size_t m, size_t n,
+                                                            const u128v* __restrict__ pis, u128v* __restrict__ msg);
+
 template <class F>
 void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const typename F::T* pis,
                             const typename F::T* lo_tabs, const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s) {
@@ -213,6 +217,15 @@ void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const
                 case 5: hipLaunchKernelGGL(k_transpose_scale32<5>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
                 default: hipLaunchKernelGGL(k_transpose_scale32<6>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
             }
+        }
+    }
+    if constexpr (F::BITS == 128) {
+        if (n >= 64 && m >= TS128_ROWS && (m & (m - 1)) == 0 && (n & (n - 1)) == 0) {
+            const unsigned cw = n < 256 ? (unsigned)n : 256u;
+            const size_t g = (m / TS128_ROWS) * (n / cw);
+            if (g > 0x7fffffffu) throw std::invalid_argument("transpose: too many tiles");
+            hipLaunchKernelGGL(k_transpose_scale128, dim3((unsigned)g), dim3(cw), 0, s, poly, m, n, pis, msg);
+            return;
         }
     }
     hipLaunchKernelGGL(k_transpose_scale<F>, dim3((unsigned)tiles), dim3(256), 0, s, poly, m, n, lo_tabs, hi_tabs, msg);
@@ -418,6 +431,7 @@ void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m,
 }
 
 // ------------------------------------------------ GF(2^128) register pass --
+constexpr int FFT128_MAXJ = 16;  // jobs (64-thread column chunks) per workgroup
 // Multiply by the table at t (32 nibble positions x 16 entries, position k
 // of w at entries 16k..16k+15; the lo word holds positions 0..15).
 __device__ __forceinline__ u128v mul_tab128_lds(const uint4* __restrict__ t, u128v w) {
@@ -438,6 +452,67 @@ __device__ __forceinline__ uint4 u128_to4(u128v v) {
     return make_uint4((uint32_t)v.lo, (uint32_t)(v.lo >> 32), (uint32_t)v.hi, (uint32_t)(v.hi >> 32));
 }
 
+// Nibble position k (0..31) of the multiply-by-lam table at t (the layout
+// mul_tab128_lds reads): entry v = lam * v * x^(4k), from c_b = lam * x^(4k+b).
+__device__ __forceinline__ void gf128_tab_build(uint4* t, u128v lam, int k) {
+    const int s4 = 4 * k;
+    uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    if (s4 == 0) {
+        w0 = lam.lo;
+        w1 = lam.hi;
+    } else if (s4 < 64) {
+        w0 = lam.lo << s4;
+        w1 = (lam.hi << s4) | (lam.lo >> (64 - s4));
+        w2 = lam.hi >> (64 - s4);
+    } else if (s4 == 64) {
+        w1 = lam.lo;
+        w2 = lam.hi;
+    } else {
+        const int r = s4 - 64;
+        w1 = lam.lo << r;
+        w2 = (lam.hi << r) | (lam.lo >> (64 - r));
+        w3 = lam.hi >> (64 - r);
+    }
+    u128v c[4];
+    c[0] = gf128_reduce256(w0, w1, w2, w3);
+#pragma unroll
+    for (int b = 1; b < 4; b++) {
+        const uint64_t top = c[b - 1].hi >> 63;
+        c[b] = {(c[b - 1].lo << 1) ^ ((0ull - top) & GF128_POLY), (c[b - 1].hi << 1) | (c[b - 1].lo >> 63)};
+    }
+    uint4* tdst = t + (k < 16 ? 16 * k : 256 + 16 * (k - 16));
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        u128v e = {0ull, 0ull};
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            if ((v >> b) & 1) e = e ^ c[b];
+        tdst[v] = u128_to4(e);
+    }
+}
+
+// GF(2^128) transpose-and-scale: tiles of TR message rows x (64 * blockDim/64)
+// columns.  A thread reads TR consecutive elements of its column (one
+// stretch of the column-major input), the workgroup builds one multiply
+// table per row from pis in LDS, and each wave writes rows of 64 consecutive
+// columns, every lane multiplying by its row's table (wave-uniform, so the
+// reads of a nibble position hit 16 entries).
+__global__ __launch_bounds__(256) void k_transpose_scale128(const u128v* __restrict__ poly, size_t m, size_t n,
+                                                            const u128v* __restrict__ pis, u128v* __restrict__ msg) {
+    constexpr int TR = TS128_ROWS;
+    __shared__ __attribute__((aligned(16))) uint4 tabs[TR * 512];
+    const size_t cw = blockDim.x;
+    const size_t row_tiles = m / TR;
+    const size_t i0 = (blockIdx.x % row_tiles) * TR, c = (blockIdx.x / row_tiles) * cw + threadIdx.x;
+    u128v v[TR];
+#pragma unroll
+    for (int r = 0; r < TR; r++) v[r] = poly[c * m + i0 + r];
+    for (int idx = threadIdx.x; idx < TR * 32; idx += blockDim.x) gf128_tab_build(tabs + (idx >> 5) * 512, pis[i0 + (idx >> 5)], idx & 31);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < TR; r++) msg[(i0 + r) * n + c] = mul_tab128_lds(tabs + r * 512, v[r]);
+}
+
 // One launch = K stages (d0..d0+K-1) of every quarter's size-m FFT.  Each
 // thread owns one column of a 2^K-row tile (rows S apart) and runs the K
 // stages in registers; a workgroup of 64*J threads covers J consecutive jobs
@@ -445,9 +520,9 @@ __device__ __forceinline__ uint4 u128_to4(u128v v) {
 // builds in LDS from the plain twiddles (entry v of position k = lambda *
 // v * x^(4k), from c_b = lambda * x^(4k+b)).
 template <int K>
-__global__ __launch_bounds__(256) void k_fft_pass128(const u128v* __restrict__ src, u128v* __restrict__ dst, size_t m,
-                                                     size_t n, int ell, int d0, int q_arg, int src_shared,
-                                                     const u128v* __restrict__ tw) {
+__global__ __launch_bounds__(1024) void k_fft_pass128(const u128v* __restrict__ src, u128v* __restrict__ dst, size_t m,
+                                                      size_t n, int ell, int d0, int q_arg, int src_shared,
+                                                      const u128v* __restrict__ tw) {
     constexpr int R = 1 << K, NT = R - 1;
     __shared__ __attribute__((aligned(16))) uint4 tabs[NT * 512];
     const size_t q = q_arg >= 0 ? (size_t)q_arg : (size_t)blockIdx.y;
@@ -465,42 +540,7 @@ __global__ __launch_bounds__(256) void k_fft_pass128(const u128v* __restrict__ s
         const size_t tp = (size_t)(tt + 1) - ((size_t)1 << j);
         const int d = d0 + j;
         const size_t first = ((size_t)1 << (d + 2)) - 1;
-        const u128v lam = tw[first + (q << d) + (B << j) + tp];
-        // lam * x^(4k): shift into four words, reduce
-        const int s4 = 4 * k;
-        uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
-        if (s4 == 0) {
-            w0 = lam.lo;
-            w1 = lam.hi;
-        } else if (s4 < 64) {
-            w0 = lam.lo << s4;
-            w1 = (lam.hi << s4) | (lam.lo >> (64 - s4));
-            w2 = lam.hi >> (64 - s4);
-        } else if (s4 == 64) {
-            w1 = lam.lo;
-            w2 = lam.hi;
-        } else {
-            const int r = s4 - 64;
-            w1 = lam.lo << r;
-            w2 = (lam.hi << r) | (lam.lo >> (64 - r));
-            w3 = lam.hi >> (64 - r);
-        }
-        u128v c[4];
-        c[0] = gf128_reduce256(w0, w1, w2, w3);
-#pragma unroll
-        for (int b = 1; b < 4; b++) {
-            const uint64_t top = c[b - 1].hi >> 63;
-            c[b] = {(c[b - 1].lo << 1) ^ ((0ull - top) & GF128_POLY), (c[b - 1].hi << 1) | (c[b - 1].lo >> 63)};
-        }
-        uint4* tdst = tabs + tt * 512 + k * 16;
-#pragma unroll
-        for (int v = 0; v < 16; v++) {
-            u128v e = {0ull, 0ull};
-#pragma unroll
-            for (int b = 0; b < 4; b++)
-                if ((v >> b) & 1) e = e ^ c[b];
-            tdst[v] = u128_to4(e);
-        }
+        gf128_tab_build(tabs + tt * 512, tw[first + (q << d) + (B << j) + tp], k);
     }
     __syncthreads();
     const size_t job = job0 + sub;
@@ -543,8 +583,11 @@ static void fft_quarters128(const u128v* msg, u128v* mat, size_t m, size_t n, co
         const int K = ell / npass + (p < ell % npass ? 1 : 0);
         const size_t S = (size_t)1 << (ell - d0 - K);
         const size_t jobs = ((size_t)1 << (ell - K)) * nchunks;
+        // up to 16 jobs (1024 threads) share one workgroup's tables: the
+        // 56 KB of tables then serve 16 waves, and two such workgroups fill
+        // a CU with eight waves per SIMD to cover the table reads' latency
         int J = 1;
-        while (J < 4 && (size_t)(J * 2) <= S * nchunks && jobs % (size_t)(J * 2) == 0) J *= 2;
+        while (J < FFT128_MAXJ && (size_t)(J * 2) <= S * nchunks && jobs % (size_t)(J * 2) == 0) J *= 2;
         const u128v* src = (p == 0 && msg) ? msg : mat;
         const int shared = (p == 0 && msg) ? 1 : 0;
         dim3 grid((unsigned)(jobs / J), 4);
@@ -2073,6 +2116,198 @@ void launch_sumcheck_fold_step(const typename F::T* f, const typename F::T* b, s
     size_t h2 = len / 4;
     hipLaunchKernelGGL((k_sumcheck_fold<F, true>), dim3(grid_for(h2 ? h2 : 1, 256, 1024)), dim3(256), 0, s, f, b, len,
                        (const typename F::T*)nullptr, step, f2, b2, res, res_clear);
+}
+
+// -------------------------------------------------------- microbenchmarks ---
+// Peak rates the rooflines are priced against.  Every lane runs NB
+// independent chains; one step of a chain is one product (or one VALU
+// instruction), so a rate is lanes x chains x steps over the kernel time.
+// The chain values feed one conditional store, so no step can be dropped.
+constexpr int BENCH_CHAINS = 8, BENCH_BLOCKS = 4096, BENCH_THREADS = 256;
+
+__device__ __forceinline__ uint32_t bench_mix(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    return x ^ (x >> 16);
+}
+
+// GF(2^32) products by wave-uniform constants through the FFT passes'
+// nibble tables in LDS (mul_tab_lds_n: 8 table reads and the address and
+// XOR instructions of one product), each followed by the butterfly's XOR.
+__global__ __launch_bounds__(BENCH_THREADS) void k_bench_mul_tab(uint32_t seed, int steps, uint32_t* out) {
+    constexpr int NB = BENCH_CHAINS;
+    __shared__ __attribute__((aligned(256))) uint32_t tabs[NB * 128];
+    for (int idx = threadIdx.x; idx < NB * 8; idx += blockDim.x)
+        gf32_tab_build(tabs + (idx >> 3) * 128, bench_mix(seed + (uint32_t)(idx >> 3)), idx & 7);
+    __syncthreads();
+    uint32_t tb[NB], w[NB], u[NB];
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        tb[i] = (uint32_t)i * 512u;
+        w[i] = bench_mix(id * NB + i + seed);
+        u[i] = bench_mix(w[i]);
+    }
+    for (int s = 0; s < steps; s++) {
+        mul_tab_lds_n<NB>(tabs, tb, w, u);
+#pragma unroll
+        for (int i = 0; i < NB; i++) w[i] ^= u[i];
+    }
+    uint32_t x = 0;
+#pragma unroll
+    for (int i = 0; i < NB; i++) x ^= u[i] ^ w[i];
+    if (x == seed) out[0] = x;
+}
+
+// General GF(2^32) products (both operands per lane): the carry-less
+// product from integer multiplies and its reduction (Ops<F32>::mul).
+__global__ __launch_bounds__(BENCH_THREADS) void k_bench_mul_clmul(uint32_t seed, int steps, uint32_t* out) {
+    constexpr int NB = BENCH_CHAINS;
+    uint32_t a[NB], b[NB];
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        a[i] = bench_mix(id * NB + i + seed);
+        b[i] = bench_mix(a[i] + 1u);
+    }
+    for (int s = 0; s < steps; s++)
+#pragma unroll
+        for (int i = 0; i < NB; i++) a[i] = Ops<F32>::mul(a[i], b[i]) ^ b[(i + 1) % NB];
+    uint32_t x = 0;
+#pragma unroll
+    for (int i = 0; i < NB; i++) x ^= a[i];
+    if (x == seed) out[0] = x;
+}
+
+// 32-bit integer VALU: each step is one three-input bitwise instruction
+// (v_bitop3_b32) per chain, majority and select functions so that no chain
+// simplifies across steps.
+__global__ __launch_bounds__(BENCH_THREADS) void k_bench_valu(uint32_t seed, int steps, uint32_t* out) {
+    constexpr int NB = 16;
+    uint32_t x[NB];
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < NB; i++) x[i] = bench_mix(id * NB + i + seed);
+    for (int s = 0; s < steps; s++) {
+#pragma unroll
+        for (int i = 0; i < NB; i += 2) {
+            x[i] = __builtin_amdgcn_bitop3_b32(x[i], x[(i + 5) % NB], x[(i + 11) % NB], 0xCA);
+            x[i + 1] = __builtin_amdgcn_bitop3_b32(x[i + 1], x[(i + 6) % NB], x[(i + 12) % NB], 0xE8);
+        }
+    }
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 0; i < NB; i++) r ^= x[i];
+    if (r == seed) out[0] = r;
+}
+
+// A formulation with fewer LDS bytes per product: 5-bit windows of w (seven
+// positions, 32-entry tables of 128 bytes, 28 bytes read per product instead
+// of 32) at two instructions per address (window extract, scaled add).
+__global__ __launch_bounds__(BENCH_THREADS) void k_bench_mul_tab5(uint32_t seed, int steps, uint32_t* out) {
+    constexpr int NB = BENCH_CHAINS, NP = 7;
+    __shared__ uint32_t tabs[NB * NP * 32];
+    for (int idx = threadIdx.x; idx < NB * NP; idx += blockDim.x) {
+        const int t = idx / NP, k = idx % NP;
+        uint32_t c[5];
+        c[0] = gf32_reduce64((uint64_t)bench_mix(seed + (uint32_t)t) << (5 * k));
+        for (int b = 1; b < 5; b++) c[b] = gf32_dbl(c[b - 1]);
+        for (int v = 0; v < 32; v++) {
+            uint32_t e = 0;
+            for (int b = 0; b < 5; b++)
+                if ((v >> b) & 1) e ^= c[b];
+            tabs[(t * NP + k) * 32 + v] = e;
+        }
+    }
+    __syncthreads();
+    uint32_t w[NB], u[NB];
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        w[i] = bench_mix(id * NB + i + seed);
+        u[i] = bench_mix(w[i]);
+    }
+    for (int s = 0; s < steps; s++) {
+        uint32_t r[NB][NP];
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+#pragma unroll
+            for (int k = 0; k < NP; k++) r[i][k] = tabs[(i * NP + k) * 32 + ((w[i] >> (5 * k)) & 31u)];
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            u[i] = xor3(xor3(r[i][0], r[i][1], r[i][2]), xor3(r[i][3], r[i][4], r[i][5]), r[i][6] ^ u[i]);
+            w[i] ^= u[i];
+        }
+    }
+    uint32_t x = 0;
+#pragma unroll
+    for (int i = 0; i < NB; i++) x ^= u[i] ^ w[i];
+    if (x == seed) out[0] = x;
+}
+
+// LDS read bandwidth with the table pattern of the GF(2^128) passes: 64
+// lanes read 16-byte entries of one 256-byte group (ds_read_b128), 16
+// distinct entries per instruction; the rate is bytes read per second.
+__global__ __launch_bounds__(BENCH_THREADS) void k_bench_lds128(uint32_t seed, int steps, uint32_t* out) {
+    constexpr int NB = BENCH_CHAINS;
+    __shared__ uint4 t[NB * 16];
+    for (int idx = threadIdx.x; idx < NB * 16; idx += blockDim.x)
+        t[idx] = make_uint4(bench_mix(idx + seed), bench_mix(idx + 1000u), bench_mix(idx + 2000u), bench_mix(idx + 3000u));
+    __syncthreads();
+    uint32_t sel = bench_mix(blockIdx.x * blockDim.x + threadIdx.x + seed);
+    uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+    for (int s = 0; s < steps; s++) {
+        uint4 r[NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++) r[i] = t[i * 16 + ((sel >> (4 * i)) & 15u)];
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            acc.x ^= r[i].x;
+            acc.y ^= r[i].y;
+            acc.z ^= r[i].z;
+            acc.w ^= r[i].w;
+        }
+        sel = (sel << 1) | (acc.x & 1u);  // the next addresses depend on the reads
+    }
+    if ((acc.x ^ acc.y ^ acc.z ^ acc.w) == seed) out[0] = acc.x;
+}
+
+double bench_rate(int which, uint32_t* scratch_word, hipStream_t s) {
+    static const int steps_of[5] = {1024, 256, 4096, 1024, 1024};
+    static const double per_step_of[5] = {BENCH_CHAINS, BENCH_CHAINS, 16.0, BENCH_CHAINS, BENCH_CHAINS * 16.0};
+    if (which < 0 || which > 4) throw std::invalid_argument("unknown microbenchmark");
+    const int steps = steps_of[which];
+    const double per_step = per_step_of[which];
+    auto launch = [&](int st) {
+        const dim3 g(BENCH_BLOCKS), b(BENCH_THREADS);
+        switch (which) {
+            case 0: hipLaunchKernelGGL(k_bench_mul_tab, g, b, 0, s, 12345u, st, scratch_word); break;
+            case 1: hipLaunchKernelGGL(k_bench_mul_clmul, g, b, 0, s, 12345u, st, scratch_word); break;
+            case 2: hipLaunchKernelGGL(k_bench_valu, g, b, 0, s, 12345u, st, scratch_word); break;
+            case 3: hipLaunchKernelGGL(k_bench_mul_tab5, g, b, 0, s, 12345u, st, scratch_word); break;
+            default: hipLaunchKernelGGL(k_bench_lds128, g, b, 0, s, 12345u, st, scratch_word); break;
+        }
+    };
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) throw std::runtime_error("hipEventCreate failed");
+    launch(steps / 8);  // warm-up (code load, clocks)
+    float best = 0.f;
+    for (int rep = 0; rep < 5; rep++) {
+        (void)hipEventRecord(a, s);
+        launch(steps);
+        (void)hipEventRecord(b, s);
+        if (hipEventSynchronize(b) != hipSuccess) throw std::runtime_error("microbenchmark failed");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (rep == 0 || ms < best) best = ms;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (hipGetLastError() != hipSuccess) throw std::runtime_error("microbenchmark launch failed");
+    const double work = (double)BENCH_BLOCKS * BENCH_THREADS * per_step * steps;
+    return work / (best * 1e-3);
 }
 
 // ------------------------------------------------------- instantiations ---

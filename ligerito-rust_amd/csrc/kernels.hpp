@@ -120,6 +120,13 @@ void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t 
 // dst[i] = src[i] for i < n words (dst may be mapped pinned host memory).
 void launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s);
 
+// Microbenchmarks (best of five timed launches, synchronous): which = 0
+// GF(2^32) products per second by wave-uniform constants with nibble tables
+// in LDS (the FFT passes' multiply), 1 general GF(2^32) products per second
+// (carry-less product from integer multiplies), 2 32-bit VALU instructions
+// per second per lane (lane-ops/s).  scratch_word: one device word.
+double bench_rate(int which, uint32_t* scratch_word, hipStream_t s);
+
 // ------------------------------------------------------------ folds -------
 // out[i] = sum_j gr[j] * poly[j*m + i], j < 2^kk (partial_eval over the top
 // kk variables, multilinear_poly.rs:66-87).  gr_tabs: 2^kk tables.

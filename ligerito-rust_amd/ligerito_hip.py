@@ -57,6 +57,7 @@ def lib():
             "lig_ctx_profile": (I, [P, I]),
             "lig_ctx_profile_read": (I, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
             "lig_ctx_synchronize": (I, [P]),
+            "lig_bench_rate": (I, [P, I, C.POINTER(C.c_double)]),
             "lig_rs_new": (I, [P, I, SZ, SZ, PP]),
             "lig_rs_free": (None, [P]),
             "lig_rs_message_length": (SZ, [P]),
@@ -176,6 +177,16 @@ class Context(_Handle):
     def profile(self, enable: bool = True):
         """Enables per-phase HIP-event timing on the library stream (resets totals)."""
         _check(lib().lig_ctx_profile(self._h, 1 if enable else 0))
+
+    def bench_rate(self, which: str) -> float:
+        """Microbenchmarked peak: "mul_tab" (GF(2^32) products/s, LDS nibble
+        tables), "mul_clmul" (general GF(2^32) products/s), "valu" (32-bit
+        lane-ops/s), "mul_tab5" (GF(2^32) products/s, 5-bit-window tables)
+        or "lds128" (LDS bytes/s, 16-byte table reads)."""
+        r = C.c_double(0)
+        idx = {"mul_tab": 0, "mul_clmul": 1, "valu": 2, "mul_tab5": 3, "lds128": 4}[which]
+        _check(lib().lig_bench_rate(self._h, idx, C.byref(r)))
+        return r.value
 
     def profile_read(self, phase: str):
         """(total_ms, count) of a phase since profiling was enabled."""
