@@ -57,9 +57,8 @@ void lig_ctx_destroy(lig_ctx *ctx);
  * Phases: "encode0" (Reed–Solomon encode of the first commit of a prove:
  * transpose+scale and FFT pass kernels), "encode" (encodes of the other
  * commits), "merkle" (leaf hashing + tree levels), "fold", "induce",
- * "sumcheck", "open", and "pass64" (each radix-2^6 FFT pass launch of the
- * first commit on its own, over GF(2^32)).  Enabling (or disabling) resets
- * the totals. */
+ * "sumcheck", "open", and "pass" (each FFT pass launch of the first commit
+ * of a prove on its own).  Enabling (or disabling) resets the totals. */
 int lig_ctx_profile(lig_ctx *ctx, int enable);
 int lig_ctx_profile_read(lig_ctx *ctx, const char *phase, double *total_ms, uint64_t *count);
 /* Microbenchmarks for the rooflines (no counterpart in the reference):
@@ -109,6 +108,18 @@ int lig_rs_encode_non_systematic_cols_dev(lig_rs *rs, void *d_columns, size_t n_
  * calls that overlap on one GPU (from several threads or contexts) race.
  * Synchronous; results equal lig_rs_encode_non_systematic_cols_dev's. */
 int lig_rs_encode_non_systematic_cols_dev_noalloc(lig_rs *rs, void *d_columns, size_t n_cols);
+/* Autotuning of the encode (no counterpart in the reference; no effect on
+ * any result): the FFT passes of an encode of n_cols columns can be launched
+ * in several shapes — kmax, the most butterfly stages per pass (GF(2^32)
+ * 2..6, GF(2^128) 1..3), and group, the most tiles (GF(2^32), up to 8) or
+ * 64-column jobs (GF(2^128), up to 16) per workgroup.  The first commit of
+ * n_cols columns on a plan times every candidate on this GPU and keeps the
+ * fastest (not for small matrices, nor with LIG_AUTOTUNE=0 in the
+ * environment; 0 then stands for the built-in default).  lig_rs_autotune
+ * tunes now if that has not happened and returns the choice;
+ * lig_rs_set_tuning fixes it (0 = default). */
+int lig_rs_autotune(lig_rs *rs, size_t n_cols, int *kmax, int *group);
+int lig_rs_set_tuning(lig_rs *rs, size_t n_cols, int kmax, int group);
 /* compute_twiddles output of this plan (binary_fft.rs:30-58): block_length-1
  * elements. */
 int lig_rs_twiddles(const lig_rs *rs, void *out);

@@ -255,6 +255,34 @@ int lig_rs_twiddles(const lig_rs* rs, void* out) {
     });
 }
 
+int lig_rs_set_tuning(lig_rs* rs, size_t n_cols, int kmax, int group) {
+    return guarded([&] {
+        LIG_REQUIRE(rs && n_cols, LIG_E_INVALID, "bad argument");
+        const bool ok32 = kmax == 0 || (kmax >= 2 && kmax <= 6), ok128 = kmax >= 0 && kmax <= 3;
+        const int gmax = rs->bits == 32 ? 8 : 16;
+        LIG_REQUIRE((rs->bits == 32 ? ok32 : ok128) && group >= 0 && group <= gmax && (group & (group - 1)) == 0,
+                    LIG_E_INVALID, "tuning: kmax 2..6 (GF(2^32)) or 1..3 (GF(2^128)), group a power of two up to 8 / 16");
+        PassTune t;
+        t.kmax = kmax;
+        t.group = group;
+        if (rs->bits == 32) rs->p32->set_tune(n_cols, t);
+        else rs->p128->set_tune(n_cols, t);
+        return LIG_OK;
+    });
+}
+
+int lig_rs_autotune(lig_rs* rs, size_t n_cols, int* kmax, int* group) {
+    return guarded([&] {
+        LIG_REQUIRE(rs && n_cols && kmax && group, LIG_E_INVALID, "bad argument");
+        LIG_REQUIRE(is_pow2(n_cols), LIG_E_INVALID, "the column count must be a power of two");
+        LIG_HIP(hipSetDevice(rs->ctx->c.device));
+        const PassTune t = rs->bits == 32 ? rs->p32->tune_for(n_cols) : rs->p128->tune_for(n_cols);
+        *kmax = t.kmax;
+        *group = t.group;
+        return LIG_OK;
+    });
+}
+
 int lig_rs_encode(lig_rs* rs, const void* message, void* out) {
     return guarded([&] {
         LIG_REQUIRE(rs && message && out, LIG_E_INVALID, "NULL argument");

@@ -2,6 +2,8 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cstdlib>
+#include <tuple>
 
 namespace lig {
 
@@ -44,7 +46,7 @@ void Pool::drain() {
 }
 
 const char* phase_name(int p) {
-    static const char* names[PH_COUNT] = {"encode0", "encode", "merkle", "fold", "induce", "sumcheck", "open", "pass64"};
+    static const char* names[PH_COUNT] = {"encode0", "encode", "merkle", "fold", "induce", "sumcheck", "open", "pass"};
     return (p >= 0 && p < PH_COUNT) ? names[p] : "";
 }
 
@@ -123,12 +125,101 @@ template <class H> void RsPlan<H>::encode_non_systematic_cols(T* d_cols, size_t 
     // two leading stages with the general butterfly, then the quarter passes
     int stages = log_block < 2 ? log_block : 2;
     for (int st = 0; st < stages; st++) launch_fft_stage<F>(rm, M_, n, st, d_tw.p, 0, s);
-    if (log_block >= 2) launch_fft_quarters<F>(nullptr, rm, M_ / 4, n, d_tw.p, d_tw_tabs.p, s);
+    // a shape already tuned by a commit of n columns; never tunes here (the
+    // no-allocation entry point runs this too)
+    const auto kt = tunes.find(n);
+    const PassTune tune = kt == tunes.end() ? PassTune{} : kt->second;
+    if (log_block >= 2) launch_fft_quarters<F>(nullptr, rm, M_ / 4, n, d_tw.p, d_tw_tabs.p, s, nullptr, tune);
     if (n > 1) launch_transpose<F>(rm, M_, n, d_cols, s);
     LIG_HIP(hipGetLastError());
     // the pooled buffer returns to the pool after the kernels that use it
     // (stream order); the caller's scratch stays busy until the stream
     // reaches this point
+}
+
+// ---------------------------------------------------------------- autotune --
+static bool autotune_enabled() {
+    static const bool on = [] {
+        const char* e = getenv("LIG_AUTOTUNE");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+// Choices shared by every plan of the process, per (device, field, message
+// length, columns): a shape is timed once, by the first plan that encodes
+// it (batch and serve workers each own plans of the same shapes).
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, size_t, size_t>, PassTune> g_tunes;
+}  // namespace
+
+// Times every candidate launch shape of the FFT passes for n columns (one
+// warm-up and three timed encodes of scratch buffers each, best of three)
+// and keeps the fastest.  The candidates differ only in how the passes are
+// launched, never in the codeword.
+template <class H> PassTune RsPlan<H>::tune_for(size_t n) {
+    using F = typename DevField<H>::F;
+    auto it = tunes.find(n);
+    if (it != tunes.end()) return it->second;
+    const size_t m_ = m();
+    const auto key = std::make_tuple(ctx->device, (int)F::BITS, m_, n);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto gt = g_tunes.find(key);
+        if (gt != g_tunes.end()) return tunes[n] = gt->second;
+    }
+    PassTune best{};
+    // small encodes take the defaults: their passes are latency-bound and
+    // timing them costs more than any shape saves
+    if (!autotune_enabled() || m_ < 64 || m_ * n < ((size_t)1 << 18)) {
+        tunes[n] = best;
+        return best;
+    }
+    std::vector<PassTune> cands;
+    if constexpr (F::BITS == 32) {
+        for (int k : {6, 5, 4})
+            for (int g : {1, 2, 4, 8}) cands.push_back({k, g});
+    } else {
+        for (int k : {3, 2})
+            for (int g : {4, 8, 16}) cands.push_back({k, g});
+    }
+    hipStream_t s = ctx->stream;
+    PBuf<T> msg(&ctx->pool, m_ * n), mat(&ctx->pool, 4 * m_ * n);
+    hipEvent_t a, b;
+    LIG_HIP(hipEventCreate(&a));
+    LIG_HIP(hipEventCreate(&b));
+    float best_ms = 0.f;
+    try {
+        for (const PassTune& c : cands) {
+            launch_fft_quarters<F>(msg.p, mat.p, m_, n, d_tw.p, d_tw_tabs.p, s, nullptr, c);
+            float t_best = 0.f;
+            for (int rep = 0; rep < 3; rep++) {
+                LIG_HIP(hipEventRecord(a, s));
+                launch_fft_quarters<F>(msg.p, mat.p, m_, n, d_tw.p, d_tw_tabs.p, s, nullptr, c);
+                LIG_HIP(hipEventRecord(b, s));
+                LIG_HIP(hipEventSynchronize(b));
+                float ms = 0.f;
+                LIG_HIP(hipEventElapsedTime(&ms, a, b));
+                if (rep == 0 || ms < t_best) t_best = ms;
+            }
+            if (best_ms == 0.f || t_best < best_ms) {
+                best_ms = t_best;
+                best = c;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        throw;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    LIG_HIP(hipGetLastError());
+    tunes[n] = best;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tunes.emplace(key, best);
+    return best;
 }
 
 // reed_solomon.rs:19-41
@@ -176,6 +267,7 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
     Ctx* ctx = rs.ctx;
     hipStream_t s = ctx->stream;
     const size_t M = 4 * m;
+    const PassTune tune = rs.tune_for(n);  // times the candidates on the first commit of n columns
     w.ctx = ctx;
     w.m = m;
     w.n = n;
@@ -184,29 +276,28 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
     {
         PBuf<T> msg(&ctx->pool, m * n);
         w.mat = PBuf<T>(&ctx->pool, M * n);
-        // times the radix-2^6 pass launches of the first commit one by one
+        // times the FFT pass launches of the first commit one by one
         struct Pass64Timer final : PassObserver {
             Ctx* ctx;
             hipEvent_t a = nullptr;
             explicit Pass64Timer(Ctx* c) : ctx(c) {}
             void before(int radix_log, size_t) override {
-                if (radix_log != 6) return;
                 LIG_HIP(hipEventCreate(&a));
                 LIG_HIP(hipEventRecord(a, ctx->stream));
             }
             void after(int radix_log, size_t) override {
-                if (radix_log != 6 || !a) return;
+                if (!a) return;
                 hipEvent_t b = nullptr;
                 LIG_HIP(hipEventCreate(&b));
                 LIG_HIP(hipEventRecord(b, ctx->stream));
-                ctx->marks.push_back({PH_PASS64, a, b});
+                ctx->marks.push_back({PH_PASS, a, b});
                 a = nullptr;
             }
         } p64(ctx);
         const bool time_passes = ctx->profiling && encode_phase == PH_ENCODE0;
         PhaseTimer pt(ctx, encode_phase);
         launch_transpose_scale<F>(d_poly, m, n, rs.d_pis.p, rs.d_pis_lo.p, rs.d_pis_hi.p, msg.p, s);
-        launch_fft_quarters<F>(msg.p, w.mat.p, m, n, rs.d_tw.p, rs.d_tw_tabs.p, s, time_passes ? &p64 : nullptr);
+        launch_fft_quarters<F>(msg.p, w.mat.p, m, n, rs.d_tw.p, rs.d_tw_tabs.p, s, time_passes ? &p64 : nullptr, tune);
         // msg returns to the pool; stream order keeps the FFT's reads first
     }
     w.tree = PBuf<uint32_t>(&ctx->pool, (2 * M - 1) * 8);

@@ -33,8 +33,8 @@ struct Pool {
 };
 
 // Phases timed with HIP events when profiling is enabled.
-// PH_PASS64 times each radix-2^6 FFT pass launch of the first commit alone.
-enum Phase { PH_ENCODE0 = 0, PH_ENCODE, PH_MERKLE, PH_FOLD, PH_INDUCE, PH_SUMCHECK, PH_OPEN, PH_PASS64, PH_COUNT };
+// PH_PASS times each FFT pass launch of the first commit of a prove alone.
+enum Phase { PH_ENCODE0 = 0, PH_ENCODE, PH_MERKLE, PH_FOLD, PH_INDUCE, PH_SUMCHECK, PH_OPEN, PH_PASS, PH_COUNT };
 const char* phase_name(int p);
 
 struct Ctx {
@@ -221,6 +221,14 @@ template <class H> struct RsPlan {
     void encode_non_systematic_cols(T* d_cols, size_t n, T* scratch = nullptr);
     // encode (systematic) on one host message.
     void encode_systematic(const T* msg, T* out);
+    // Launch shape of the FFT passes per column count (no effect on the
+    // result): chosen at the first encode of that many columns by timing
+    // the candidate shapes on this GPU (autotune), unless set_tune fixed it
+    // or autotuning is off (LIG_AUTOTUNE=0 in the environment) or the
+    // matrix is small (the defaults then).
+    std::map<size_t, PassTune> tunes;
+    PassTune tune_for(size_t n);
+    void set_tune(size_t n, PassTune t) { tunes[n] = t; }
     // device memory held by the plan's tables
     size_t device_bytes() const {
         return (d_tw.n + d_tw_tabs.n + d_pis.n + d_pis_lo.n + d_pis_hi.n) * sizeof(T);

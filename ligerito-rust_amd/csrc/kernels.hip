@@ -395,24 +395,25 @@ __global__ __launch_bounds__(256) void k_fft_pass(const typename F::T* __restric
 }
 
 static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw, hipStream_t s,
-                           PassObserver* obs);
-static void fft_quarters128(const u128v* msg, u128v* mat, size_t m, size_t n, const u128v* tw, hipStream_t s);
+                           PassObserver* obs, PassTune tune);
+static void fft_quarters128(const u128v* msg, u128v* mat, size_t m, size_t n, const u128v* tw, hipStream_t s,
+                            PassObserver* obs, PassTune tune);
 
 template <class F>
 void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw,
-                         const typename F::T* tw_tabs, hipStream_t s, PassObserver* obs) {
+                         const typename F::T* tw_tabs, hipStream_t s, PassObserver* obs, PassTune tune) {
     constexpr int K = PassShape<F>::K, C = PassShape<F>::C;
     if (n == 0 || (n & (n - 1)) != 0) throw std::invalid_argument("column count must be a power of two");
     int ell = 0;
     while (((size_t)1 << ell) < m) ell++;
     if constexpr (F::BITS == 32) {
         if (ell > 0) {
-            fft_quarters32(msg, mat, m, n, tw, s, obs);
+            fft_quarters32(msg, mat, m, n, tw, s, obs, tune);
             return;
         }
     } else {
         if (ell > 0) {
-            fft_quarters128(msg, mat, m, n, tw, s);
+            fft_quarters128(msg, mat, m, n, tw, s, obs, tune);
             return;
         }
     }
@@ -569,10 +570,14 @@ __global__ __launch_bounds__(1024) void k_fft_pass128(const u128v* __restrict__ 
     for (int a = 0; a < R; a++) dst[(q * m + rbase + a * S) * n + c0 + lane] = v[a];
 }
 
-static void fft_quarters128(const u128v* msg, u128v* mat, size_t m, size_t n, const u128v* tw, hipStream_t s) {
+static void fft_quarters128(const u128v* msg, u128v* mat, size_t m, size_t n, const u128v* tw, hipStream_t s,
+                            PassObserver* obs, PassTune tune) {
     int ell = 0;
     while (((size_t)1 << ell) < m) ell++;
-    const int KMAX = 3;
+    if (tune.kmax < 0 || tune.kmax > 3 || tune.group < 0 || tune.group > FFT128_MAXJ || (tune.group & (tune.group - 1)))
+        throw std::invalid_argument("GF(2^128) passes: kmax 1..3, group a power of two up to 16");
+    const int KMAX = tune.kmax ? tune.kmax : 3;
+    const int JMAX = tune.group ? tune.group : FFT128_MAXJ;
     const int npass = (ell + KMAX - 1) / KMAX;
     const size_t nchunks = (n + 63) / 64;
     const int cc = n < 64 ? (int)n : 64;
@@ -587,15 +592,17 @@ static void fft_quarters128(const u128v* msg, u128v* mat, size_t m, size_t n, co
         // 56 KB of tables then serve 16 waves, and two such workgroups fill
         // a CU with eight waves per SIMD to cover the table reads' latency
         int J = 1;
-        while (J < FFT128_MAXJ && (size_t)(J * 2) <= S * nchunks && jobs % (size_t)(J * 2) == 0) J *= 2;
+        while (J < JMAX && (size_t)(J * 2) <= S * nchunks && jobs % (size_t)(J * 2) == 0) J *= 2;
         const u128v* src = (p == 0 && msg) ? msg : mat;
         const int shared = (p == 0 && msg) ? 1 : 0;
         dim3 grid((unsigned)(jobs / J), 4);
+        if (obs) obs->before(K, m * n * 4);
         switch (K) {
             case 1: hipLaunchKernelGGL(k_fft_pass128<1>, grid, dim3(64 * J), 0, s, src, mat, m, n, ell, d0, -1, shared, tw); break;
             case 2: hipLaunchKernelGGL(k_fft_pass128<2>, grid, dim3(64 * J), 0, s, src, mat, m, n, ell, d0, -1, shared, tw); break;
             default: hipLaunchKernelGGL(k_fft_pass128<3>, grid, dim3(64 * J), 0, s, src, mat, m, n, ell, d0, -1, shared, tw); break;
         }
+        if (obs) obs->after(K, m * n * 4);
         d0 += K;
     }
 }
@@ -890,12 +897,15 @@ static void launch_pass32(int K, dim3 grid, int block, hipStream_t s, const uint
 // GF(2^32) encode passes, quarter by quarter so that a quarter (m x n x 4
 // bytes) stays resident in the last-level cache between its passes.
 static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw, hipStream_t s,
-                           PassObserver* obs) {
+                           PassObserver* obs, PassTune tune) {
     if (m * n * sizeof(uint32_t) >= ((size_t)1 << 31)) throw std::invalid_argument("FFT quarter of 2 GiB or more");
     if (n == 0 || (n & (n - 1)) != 0) throw std::invalid_argument("column count must be a power of two");
+    if (tune.kmax < 0 || tune.kmax == 1 || tune.kmax > 6 || tune.group < 0 || tune.group > 8 || (tune.group & (tune.group - 1)))
+        throw std::invalid_argument("GF(2^32) passes: kmax 2..6, group a power of two up to 8");
     int ell = 0;
     while (((size_t)1 << ell) < m) ell++;
-    const int KMAX = 6;
+    const int KMAX = tune.kmax ? tune.kmax : 6;
+    const int TPW = tune.group ? tune.group : 4;
     const int npass = (ell + KMAX - 1) / KMAX;
     // Narrow matrices (n < 64): the 2^v = 64/n consecutive rows of an
     // aligned group are contiguous, so a layer that pairs rows 2^v or more
@@ -922,7 +932,7 @@ static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t 
         // tiles sharing B (hence tables) are grouped per workgroup
         size_t jobs = tiles * nchunks;
         int tpw = 1;
-        while (tpw < 4 && (size_t)(tpw * 2) <= S * nchunks && jobs / (tpw * 2) >= 512) tpw *= 2;
+        while (tpw < TPW && (size_t)(tpw * 2) <= S * nchunks && jobs / (tpw * 2) >= 512) tpw *= 2;
         const uint32_t* src = (p == 0 && msg) ? msg : mat;
         const int shared = (p == 0 && msg) ? 1 : 0;
         dim3 grid((unsigned)(jobs / tpw), 4);
@@ -2319,7 +2329,7 @@ double bench_rate(int which, uint32_t* scratch_word, hipStream_t s) {
     template void launch_transpose<F>(const F::T*, size_t, size_t, F::T*, hipStream_t);                                     \
     template void launch_fft_stage<F>(F::T*, size_t, size_t, int, const F::T*, int, hipStream_t);                           \
     template void launch_fft_quarters<F>(const F::T*, F::T*, size_t, size_t, const F::T*, const F::T*, hipStream_t,         \
-                                         PassObserver*);                     \
+                                         PassObserver*, PassTune);                                                           \
     template void launch_leaf_hash<F>(const F::T*, size_t, size_t, uint32_t*, uint32_t*, hipStream_t);                                 \
     template void launch_fold<F>(const F::T*, size_t, int, const F::T*, F::T*, hipStream_t);                                \
     template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, F::T*, hipStream_t);                  \

@@ -83,9 +83,17 @@ protected:
 // from mat itself.  tw holds the heap-ordered twiddles; tw_tabs one
 // multiply table per twiddle, read only where the GF(2^32) passes do not
 // apply (GF(2^128), or m == 1), and may be null otherwise.
+// Launch shape of the FFT passes (no effect on the result): kmax = most
+// butterfly stages per pass (GF(2^32): 2..6, GF(2^128): 1..3), group = most
+// tiles (GF(2^32)) or 64-column jobs (GF(2^128)) per workgroup, a power of
+// two (GF(2^32): 1..8, GF(2^128): 1..16).  0 takes the default.
+struct PassTune {
+    int kmax = 0;
+    int group = 0;
+};
 template <class F>
 void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw,
-                         const typename F::T* tw_tabs, hipStream_t s, PassObserver* obs = nullptr);
+                         const typename F::T* tw_tabs, hipStream_t s, PassObserver* obs = nullptr, PassTune tune = {});
 
 // -------------------------------------------------------------- Merkle ----
 // layer0[row*8 .. +8] = BLAKE3(row bytes) for every row of the [rows][n] matrix.

@@ -67,6 +67,8 @@ def lib():
             "lig_rs_encode_non_systematic_cols_dev": (I, [P, P, SZ]),
             "lig_rs_encode_non_systematic_cols_dev_noalloc": (I, [P, P, SZ]),
             "lig_rs_twiddles": (I, [P, P]),
+            "lig_rs_autotune": (I, [P, SZ, C.POINTER(I), C.POINTER(I)]),
+            "lig_rs_set_tuning": (I, [P, SZ, I, I]),
             "lig_build_merkle_tree": (I, [P, P, SZ, SZ, P]),
             "lig_merkle_prove": (I, [P, SZ, P, SZ, P, SZ, C.POINTER(SZ)]),
             "lig_merkle_verify": (I, [P, P, SZ, SZ, P, SZ, P, SZ]),
@@ -220,6 +222,17 @@ class ReedSolomonEncoding(_Handle):
 
     def block_length(self):
         return lib().lig_rs_block_length(self._h)
+
+    def autotune(self, n_cols: int):
+        """(kmax, group) of the encode's FFT passes for n_cols columns,
+        timing the candidates now if no commit of n_cols columns did."""
+        k, g = C.c_int(0), C.c_int(0)
+        _check(lib().lig_rs_autotune(self._h, n_cols, C.byref(k), C.byref(g)))
+        return k.value, g.value
+
+    def set_tuning(self, n_cols: int, kmax: int, group: int):
+        """Fixes the launch shape of the FFT passes for n_cols columns."""
+        _check(lib().lig_rs_set_tuning(self._h, n_cols, kmax, group))
 
     def twiddles(self):
         out = empty(self.block_length() - 1, self.bits)

@@ -327,3 +327,28 @@ def test_rs_encode_cols_dev_matches_oracle(O, ctx, bits, logm, n):
     got, cols = got.reshape(shape), cols.reshape(shape)
     for j in range(n):
         assert np.array_equal(got[j], O.rs_encode_non_systematic(cols[j].copy(), m, bits)), f"column {j}"
+
+
+@pytest.mark.parametrize("bits,logm,n", [(32, 12, 64), (32, 10, 16), (128, 9, 128), (128, 8, 64)])
+def test_every_tuned_pass_shape_gives_the_same_commit(O, ctx, bits, logm, n):
+    # the autotuner picks among launch shapes of the FFT passes; every
+    # candidate must give the oracle's codeword and root
+    m = 1 << logm
+    say("tuning shapes", bits, m, n)
+    rs = lh.ReedSolomonEncoding(ctx, m, 4 * m, bits)
+    poly = col(O, 60 + logm, m * n, bits)
+    root, mat, _ = O.ligero_commit(poly, m, n, bits, want_matrix=True)
+    shapes = [(k, g) for k in (6, 5, 4, 3, 2) for g in (1, 2, 4, 8)] if bits == 32 else \
+        [(k, g) for k in (3, 2, 1) for g in (1, 2, 4, 8, 16)]
+    for k, g in shapes:
+        rs.set_tuning(n, k, g)
+        w = lh.ligero_commit(poly, m, n, rs)
+        assert w.root() == root, (k, g)
+        assert np.array_equal(w.flat_mat(), mat), (k, g)
+    rs.set_tuning(n, 0, 0)
+    assert lh.ligero_commit(poly, m, n, rs).root() == root
+    # the autotuner's own choice is one of the candidates
+    rs2 = lh.ReedSolomonEncoding(ctx, m, 4 * m, bits)
+    k, g = rs2.autotune(n)
+    assert (k, g) == (0, 0) or (k, g) in shapes
+    assert lh.ligero_commit(poly, m, n, rs2).root() == root
