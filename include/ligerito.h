@@ -129,6 +129,17 @@ int lig_rs_twiddles(const lig_rs *rs, void *out);
  * leaves (host buffer, leaf i = leaves[i*leaf_len ..]).  layers_out receives
  * every layer, leaf hashes first, 32*(2*n_leaves-1) bytes. */
 int lig_build_merkle_tree(lig_ctx *ctx, const void *leaves, size_t n_leaves, size_t leaf_len, uint8_t *layers_out);
+/* build_merkle_tree over leaves of any lengths (its AsRef<[u8]> leaves,
+ * merkle_tree.rs:22-60): leaf i is bytes[offsets[i] .. offsets[i+1]), so
+ * offsets holds n_leaves + 1 non-decreasing byte offsets; leaves may be empty
+ * and at most 256 KiB each.  n_leaves must be a power of two (LIG_E_ASSERT,
+ * merkle_tree.rs:26).  layers_out as for lig_build_merkle_tree. */
+int lig_build_merkle_tree_var(lig_ctx *ctx, const void *bytes, const size_t *offsets, size_t n_leaves,
+                              uint8_t *layers_out);
+/* verify (merkle_tree.rs:164-188) with leaves of any lengths, laid out as
+ * for lig_build_merkle_tree_var (n_queries + 1 offsets): 1 (valid) or 0. */
+int lig_merkle_verify_var(const uint8_t root[32], const uint8_t *proof, size_t n_nodes, size_t depth, const void *bytes,
+                          const size_t *offsets, const size_t *indices, size_t n_queries);
 /* prove (merkle_tree.rs:98-111) from the layers written above; queries
  * sorted and distinct.  Writes up to cap_nodes 32-byte nodes. */
 int lig_merkle_prove(const uint8_t *layers, size_t n_leaves, const size_t *queries, size_t n_queries, uint8_t *proof_out,

@@ -363,6 +363,54 @@ int lig_build_merkle_tree(lig_ctx* ctx, const void* leaves, size_t n_leaves, siz
     });
 }
 
+static void check_offsets(const size_t* offsets, size_t n_leaves) {
+    for (size_t i = 0; i < n_leaves; i++) {
+        LIG_REQUIRE(offsets[i + 1] >= offsets[i], LIG_E_INVALID, "leaf offsets must not decrease");
+        LIG_REQUIRE(offsets[i + 1] - offsets[i] <= 256 * 1024, LIG_E_INVALID, "leaves above 256 KiB are not supported");
+    }
+}
+
+int lig_build_merkle_tree_var(lig_ctx* ctx, const void* bytes, const size_t* offsets, size_t n_leaves, uint8_t* layers_out) {
+    return guarded([&] {
+        LIG_REQUIRE(ctx && offsets && layers_out, LIG_E_INVALID, "NULL argument");
+        // merkle_tree.rs:26 asserts a power-of-two leaf count
+        LIG_REQUIRE(is_pow2(n_leaves), LIG_E_ASSERT, "build_merkle_tree: leaf count must be a power of two");
+        check_offsets(offsets, n_leaves);
+        const size_t total = offsets[n_leaves] - offsets[0];
+        LIG_REQUIRE(bytes || total == 0, LIG_E_INVALID, "NULL leaf bytes");
+        Ctx& c = ctx->c;
+        LIG_HIP(hipSetDevice(c.device));
+        std::vector<uint64_t> offs(n_leaves + 1);
+        for (size_t i = 0; i <= n_leaves; i++) offs[i] = offsets[i] - offsets[0];
+        PBuf<uint8_t> d_bytes(&c.pool, std::max<size_t>(total, 1));
+        if (total)
+            LIG_HIP(hipMemcpyAsync(d_bytes.p, (const uint8_t*)bytes + offsets[0], total, hipMemcpyHostToDevice, c.stream));
+        PBuf<uint64_t> d_offs = upload<uint64_t>(c, offs.data(), offs.size());
+        PBuf<uint32_t> tree(&c.pool, (2 * n_leaves - 1) * 8);
+        launch_leaf_hash_var(d_bytes.p, d_offs.p, n_leaves, tree.p, c.stream);
+        if (n_leaves > 1) launch_merkle_levels(tree.p, n_leaves, c.stream);
+        LIG_HIP(hipGetLastError());
+        LIG_HIP(hipMemcpyAsync(layers_out, tree.p, (2 * n_leaves - 1) * 32, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        return LIG_OK;
+    });
+}
+
+int lig_merkle_verify_var(const uint8_t root[32], const uint8_t* proof, size_t n_nodes, size_t depth, const void* bytes,
+                          const size_t* offsets, const size_t* indices, size_t n_queries) {
+    int r = guarded([&] {
+        LIG_REQUIRE(root && offsets && indices && n_queries, LIG_E_INVALID, "bad argument");
+        LIG_REQUIRE(bytes || offsets[n_queries] == offsets[0], LIG_E_INVALID, "NULL leaf bytes");
+        check_offsets(offsets, n_queries);
+        std::vector<std::array<uint8_t, 32>> lh(n_queries);
+        for (size_t i = 0; i < n_queries; i++)
+            blake3_bytes((const uint8_t*)bytes + offsets[i], offsets[i + 1] - offsets[i], lh[i].data());
+        std::vector<size_t> q(indices, indices + n_queries);
+        return merkle_verify_hashes(root, proof, n_nodes, depth, lh, q) ? 1 : 0;
+    });
+    return r < 0 ? 0 : r;
+}
+
 int lig_merkle_prove(const uint8_t* layers, size_t n_leaves, const size_t* queries, size_t n_queries, uint8_t* proof_out,
                      size_t cap_nodes, size_t* n_nodes) {
     return guarded([&] {

@@ -1002,6 +1002,49 @@ __global__ void k_leaf_hash(const typename F::T* __restrict__ mat, size_t rows, 
     }
 }
 
+// Leaves of any byte length (build_merkle_tree over AsRef<[u8]> leaves,
+// merkle_tree.rs:4-20): leaf i is bytes[offs[i] .. offs[i+1]), at most
+// 256 KiB.  One lane per leaf assembles little-endian words from bytes (a
+// leaf may start at any byte) and runs the BLAKE3 chunk chain; the last
+// block, zero-padded, may be partial or (for an empty leaf) empty.
+__global__ __launch_bounds__(256) void k_leaf_hash_var(const uint8_t* __restrict__ bytes, const uint64_t* __restrict__ offs,
+                                                       size_t n_leaves, uint32_t* __restrict__ layer0) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_leaves; i += (size_t)gridDim.x * blockDim.x) {
+        const uint8_t* p = bytes + offs[i];
+        const uint64_t len = offs[i + 1] - offs[i];
+        const uint64_t nblocks = len == 0 ? 1 : (len + 63) / 64;
+        b3::Hasher h;
+        h.init();
+        uint32_t m[16];
+        for (uint64_t b = 0; b < nblocks; b++) {
+            const uint64_t base = b * 64;
+#pragma unroll
+            for (int w = 0; w < 16; w++) {
+                uint32_t x = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint64_t at = base + 4 * w + k;
+                    if (at < len) x |= (uint32_t)p[at] << (8 * k);
+                }
+                m[w] = x;
+            }
+            if (b + 1 < nblocks) h.block(m);
+            else {
+                uint32_t out[8];
+                h.finish(m, (uint32_t)(len - base), out);
+                uint4* o = reinterpret_cast<uint4*>(layer0 + i * 8);
+                o[0] = make_uint4(out[0], out[1], out[2], out[3]);
+                o[1] = make_uint4(out[4], out[5], out[6], out[7]);
+            }
+        }
+    }
+}
+
+void launch_leaf_hash_var(const uint8_t* bytes, const uint64_t* offs, size_t n_leaves, uint32_t* layer0, hipStream_t s) {
+    if (!n_leaves) return;
+    hipLaunchKernelGGL(k_leaf_hash_var, dim3(grid_for(n_leaves, 256)), dim3(256), 0, s, bytes, offs, n_leaves, layer0);
+}
+
 // Rows of NB whole 64-byte blocks (one chunk): each lane loads its whole row
 // first (all loads in flight together, so each row's cache lines are read
 // from HBM once), then chains the NB compressions from registers.

@@ -103,6 +103,9 @@ void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m,
 size_t leaf_hash_scratch_words(size_t rows, size_t row_bytes);
 template <class F>
 void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, uint32_t* scratch, hipStream_t s);
+// layer0[i*8 .. +8] = BLAKE3(bytes[offs[i] .. offs[i+1])) for leaves of any
+// length up to 256 KiB (offs: n_leaves + 1 non-decreasing byte offsets).
+void launch_leaf_hash_var(const uint8_t* bytes, const uint64_t* offs, size_t n_leaves, uint32_t* layer0, hipStream_t s);
 // Builds layers 1.. in place after layer 0 (contiguous node layout,
 // merkle_tree.rs:22-60).  n_leaves must be a power of two, at least 2.
 // If root_out is given, the root's 8 words are also written there (for

@@ -72,6 +72,8 @@ def lib():
             "lig_build_merkle_tree": (I, [P, P, SZ, SZ, P]),
             "lig_merkle_prove": (I, [P, SZ, P, SZ, P, SZ, C.POINTER(SZ)]),
             "lig_merkle_verify": (I, [P, P, SZ, SZ, P, SZ, P, SZ]),
+            "lig_build_merkle_tree_var": (I, [P, P, P, SZ, P]),
+            "lig_merkle_verify_var": (I, [P, P, SZ, SZ, P, P, P, SZ]),
             "lig_fs_new": (I, [U32, PP]),
             "lig_fs_free": (None, [P]),
             "lig_fs_absorb": (I, [P, P, SZ]),
@@ -268,13 +270,29 @@ class ReedSolomonEncoding(_Handle):
 
 
 # ------------------------------------------------------------------- Merkle --
-def build_merkle_tree(ctx: Context, leaves: np.ndarray):
-    """src/merkle_tree.rs:22-60 for equal-length leaves (rows of a 2-D uint8
-    array).  Returns the list of layers as bytes, leaves first."""
-    leaves = np.ascontiguousarray(leaves, dtype=np.uint8)
-    n, ll = leaves.shape
-    out = np.zeros(32 * (2 * n - 1), dtype=np.uint8)
-    _check(lib().lig_build_merkle_tree(ctx.handle, _p(leaves), n, ll, _p(out)))
+def _var_leaves(leaves):
+    """A list of byte strings as one byte array plus n + 1 offsets."""
+    parts = [bytes(x) for x in leaves]
+    offs = np.zeros(len(parts) + 1, dtype=np.uint64)
+    np.cumsum([len(x) for x in parts], out=offs[1:])
+    buf = np.frombuffer(b"".join(parts) + b"\0", dtype=np.uint8)
+    return buf, offs
+
+
+def build_merkle_tree(ctx: Context, leaves):
+    """src/merkle_tree.rs:22-60.  leaves: rows of a 2-D uint8 array (equal
+    lengths, a multiple of 4 bytes) or a list of byte strings of any lengths.
+    Returns the list of layers as bytes, leaves first."""
+    if isinstance(leaves, np.ndarray) and leaves.ndim == 2 and leaves.shape[1] % 4 == 0:
+        leaves = np.ascontiguousarray(leaves, dtype=np.uint8)
+        n, ll = leaves.shape
+        out = np.zeros(32 * (2 * n - 1), dtype=np.uint8)
+        _check(lib().lig_build_merkle_tree(ctx.handle, _p(leaves), n, ll, _p(out)))
+    else:
+        buf, offs = _var_leaves(leaves)
+        n = offs.shape[0] - 1
+        out = np.zeros(32 * max(2 * n - 1, 1), dtype=np.uint8)
+        _check(lib().lig_build_merkle_tree_var(ctx.handle, _p(buf), _p(offs), n, _p(out)))
     layers, pos, d = [], 0, 0
     while (n >> d) >= 1:
         sz = 32 * (n >> d)
@@ -297,12 +315,17 @@ def prove(tree_layers, queries):
 
 
 def verify(root: bytes, proof, depth: int, leaves, indices) -> bool:
-    """src/merkle_tree.rs:164-188 for equal-length leaves."""
-    lv = np.ascontiguousarray(leaves, dtype=np.uint8)
+    """src/merkle_tree.rs:164-188.  leaves: rows of a 2-D uint8 array or a
+    list of byte strings of any lengths."""
     pr = np.frombuffer(b"".join(proof) + b"\0", dtype=np.uint8)
     rt = np.frombuffer(bytes(root), dtype=np.uint8)
     idx = np.ascontiguousarray(indices, dtype=np.uint64)
-    return bool(lib().lig_merkle_verify(_p(rt), _p(pr), len(proof), depth, _p(lv), lv.shape[1], _p(idx), idx.shape[0]))
+    if isinstance(leaves, np.ndarray) and leaves.ndim == 2:
+        lv = np.ascontiguousarray(leaves, dtype=np.uint8)
+        return bool(lib().lig_merkle_verify(_p(rt), _p(pr), len(proof), depth, _p(lv), lv.shape[1], _p(idx),
+                                            idx.shape[0]))
+    buf, offs = _var_leaves(leaves)
+    return bool(lib().lig_merkle_verify_var(_p(rt), _p(pr), len(proof), depth, _p(buf), _p(offs), _p(idx), idx.shape[0]))
 
 
 # --------------------------------------------------------------- transcript --

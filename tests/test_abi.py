@@ -133,3 +133,17 @@ def test_wrappers_reject_wrong_length_inputs():
     with pytest.raises(lh.LigeritoError) as e:
         s.submit(17, np.zeros(1 << 16, np.uint32))
     assert e.value.code == -1
+
+
+def test_merkle_verify_variable_length_leaves(O):
+    # the crate's own leaves (merkle_tree.rs:195-255 use "hello", "world",
+    # "rust", "test") and leaves of mixed lengths, as AsRef<[u8]> allows
+    for leaves in ([b"hello", b"world", b"rust", b"test"],
+                   [bytes(range(k % 251)) * (1 + k // 251) for k in (0, 1, 5, 64, 65, 1023, 1024, 1500)]):
+        layers = O.build_merkle_tree(leaves)
+        depth = len(layers) - 1
+        for q in ([0, 2], [0, 1, 2], list(range(len(leaves)))[1::2]):
+            proof = O.merkle_prove(leaves, q)
+            assert lh.verify(layers[-1], proof, depth, [leaves[i] for i in q], q)
+            bad = [bytearray(leaves[i]) + b"x" for i in q]
+            assert not lh.verify(layers[-1], proof, depth, [bytes(b) for b in bad], q)

@@ -352,3 +352,20 @@ def test_every_tuned_pass_shape_gives_the_same_commit(O, ctx, bits, logm, n):
     k, g = rs2.autotune(n)
     assert (k, g) == (0, 0) or (k, g) in shapes
     assert lh.ligero_commit(poly, m, n, rs2).root() == root
+
+
+def test_build_merkle_tree_variable_length_leaves(O, ctx):
+    # build_merkle_tree<T: AsRef<[u8]>> (merkle_tree.rs:22-60): the crate's
+    # own KAT leaves, then leaves of every length class (empty, partial and
+    # whole blocks, one and several 1 KiB chunks, unaligned starts)
+    say("merkle variable-length leaves")
+    kat = [b"hello", b"world", b"rust", b"test"]
+    assert lh.build_merkle_tree(ctx, kat) == O.build_merkle_tree(kat)
+    rng = np.random.default_rng(11)
+    lens = [0, 1, 3, 63, 64, 65, 127, 1023, 1024, 1025, 2048, 3001, 5, 70000, 16384, 9]
+    leaves = [bytes(rng.integers(0, 256, size=L, dtype=np.uint8)) for L in lens]
+    assert lh.build_merkle_tree(ctx, leaves) == O.build_merkle_tree(leaves)
+    assert lh.build_merkle_tree(ctx, [b"only leaf"]) == O.build_merkle_tree([b"only leaf"])
+    with pytest.raises(lh.LigeritoError) as e:
+        lh.build_merkle_tree(ctx, [b"a", b"b", b"c"])
+    assert e.value.code == -2  # merkle_tree.rs:26 asserts a power of two
