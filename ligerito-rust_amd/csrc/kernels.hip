@@ -58,12 +58,33 @@ __device__ __forceinline__ void gf32_tab_write(uint32_t* t, int k, uint32_t c0, 
     d4[2] = make_uint4(c3, c3 ^ c0, c3 ^ c1, c3 ^ c01);
     d4[3] = make_uint4(c23, c23 ^ c0, c23 ^ c1, c23 ^ c01);
 }
+// The same entries, the position's four 16-byte chunks (entries 4c..4c+3 =
+// X_c ^ {0, c0, c1, c0^c1}, X_c the c-combination of c2, c3) written in an
+// order rotated by lane.  Where consecutive lanes build consecutive
+// positions 64 bytes apart, lanes 4 apart otherwise write the same LDS
+// banks at every step; rotated, 16 consecutive lanes write 16 distinct bank
+// groups.  A few more registers than gf32_tab_write.
+__device__ __forceinline__ void gf32_tab_write_rot(uint32_t* t, int k, uint32_t c0, uint32_t c1, uint32_t c2,
+                                                   uint32_t c3) {
+    const uint32_t c01 = c0 ^ c1;
+    uint4* d4 = reinterpret_cast<uint4*>(t + k * 16);
+    const uint32_t rot = (threadIdx.x >> 2) & 3u;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const uint32_t c = ((uint32_t)s + rot) & 3u;
+        const uint32_t x = ((0u - (c & 1u)) & c2) ^ ((0u - (c >> 1)) & c3);
+        d4[c] = make_uint4(x, x ^ c0, x ^ c1, x ^ c01);
+    }
+}
 __device__ __forceinline__ uint32_t gf32_dbl(uint32_t c) { return (c << 1) ^ ((0u - (c >> 31)) & GF32_POLY); }
 // nibble position k of the multiply-by-lam table: c_b = lam * x^(4k+b)
+// (ROT: written with gf32_tab_write_rot)
+template <bool ROT = false>
 __device__ __forceinline__ void gf32_tab_build(uint32_t* t, uint32_t lam, int k) {
     const uint32_t c0 = gf32_reduce64((uint64_t)lam << (4 * k));
     const uint32_t c1 = gf32_dbl(c0), c2 = gf32_dbl(c1), c3 = gf32_dbl(c2);
-    gf32_tab_write(t, k, c0, c1, c2, c3);
+    if constexpr (ROT) gf32_tab_write_rot(t, k, c0, c1, c2, c3);
+    else gf32_tab_write(t, k, c0, c1, c2, c3);
 }
 // nibble position k of the squaring table: c_b = x^(8k+2b)
 __device__ __forceinline__ void gf32_tab_build_sqr(uint32_t* t, int k) {
@@ -178,7 +199,7 @@ __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __res
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int item = tid + 256 * h;
-            gf32_tab_build(rtab + (item >> 3) * 128, lam[h], item & 7);
+            gf32_tab_build<true>(rtab + (item >> 3) * 128, lam[h], item & 7);
         }
         LIG_LDS_BARRIER();
         if (tile_i + gridDim.x < ntiles) load(tile_i + gridDim.x);
@@ -754,7 +775,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
                 const int d = d0 + j;
                 const size_t first = ((size_t)1 << (d + 2)) - 1;
                 const uint32_t lam = tw[first + ((size_t)q << d) + ((size_t)B << j) + tp];
-                gf32_tab_build(tabs + tt * 128, lam, k);
+                gf32_tab_build<true>(tabs + tt * 128, lam, k);
             }
         } else {
             LIG_LDS_BARRIER();  // previous tile's exchange reads are done
@@ -1469,7 +1490,7 @@ __global__ __launch_bounds__(256) void k_fold32(const uint32_t* __restrict__ pol
         for (int j = 0; j < NC; j++) wsh[j] = gr.w[j];
     }
     LIG_LDS_BARRIER();
-    for (int idx = threadIdx.x; idx < nc * 8; idx += blockDim.x) gf32_tab_build(tabs + (idx >> 3) * 128, wsh[idx >> 3], idx & 7);
+    for (int idx = threadIdx.x; idx < nc * 8; idx += blockDim.x) gf32_tab_build<true>(tabs + (idx >> 3) * 128, wsh[idx >> 3], idx & 7);
     LIG_LDS_BARRIER();
     uint32_t acc = 0;
     for (int j0 = 0; j0 < nc; j0 += BATCH) {
