@@ -162,44 +162,45 @@ __device__ __forceinline__ void stage_words(uint32_t* __restrict__ dst, const ui
 
 __device__ __forceinline__ uint32_t mul_tab_lds(const uint32_t* tabs, uint32_t tb, uint32_t w);
 
-// GF(2^32) variant: tiles of 64 rows x 2^LCT columns (2^LCT = min(n, 64);
-// m >= 64).  A tile's rows i0..i0+63 are scaled by pis[i0 + r]: the
+// GF(2^32) variant: tiles of 2^LTR rows x 2^LCT columns (2^LCT = min(n, 64);
+// m >= 2^LTR).  A tile's rows i0..i0+2^LTR-1 are scaled by pis[i0 + r]: the
 // workgroup builds one multiply table per row in LDS, so each element costs
 // one table multiply.  The launcher gives every tile its own workgroup; a
 // smaller grid walks the tiles, issuing the next tile's loads before this
 // tile is scaled and stored.  The barriers wait for LDS traffic only.
-template <int LCT>
+template <int LCT, int LTR>
 __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __restrict__ poly, size_t m, size_t n,
                                                           const uint32_t* __restrict__ pis, uint32_t* __restrict__ msg) {
-    constexpr int CT = 1 << LCT, KPT = (64 * CT) / 256;  // elements per thread
-    static_assert(KPT >= 1, "at least four columns");
-    __shared__ __attribute__((aligned(256))) uint32_t rtab[64 * 128];
-    __shared__ uint32_t tile[CT][65];
-    const size_t row_tiles = m / 64, ntiles = row_tiles * (n / CT);
+    constexpr int CT = 1 << LCT, TR = 1 << LTR, KPT = (TR * CT) / 256;  // elements per thread
+    constexpr int NH = (TR * 8 + 255) / 256;                            // table positions per thread
+    static_assert(KPT >= 1, "at least 256 elements per tile");
+    __shared__ __attribute__((aligned(256))) uint32_t rtab[TR * 128];
+    __shared__ uint32_t tile[CT][TR + 1];
+    const size_t row_tiles = m / TR, ntiles = row_tiles * (n / CT);
     const int tid = threadIdx.x;
-    uint32_t tv[KPT], lam[2];
+    uint32_t tv[KPT], lam[NH];
     auto load = [&](size_t ti) {
-        const size_t i0 = (ti % row_tiles) * 64, c0 = (ti / row_tiles) * CT;
+        const size_t i0 = (ti % row_tiles) * TR, c0 = (ti / row_tiles) * CT;
 #pragma unroll
         for (int k = 0; k < KPT; k++) {
             const int idx = tid + 256 * k;
-            tv[k] = poly[(c0 + (idx >> 6)) * m + i0 + (idx & 63)];
+            tv[k] = poly[(c0 + (idx >> LTR)) * m + i0 + (idx & (TR - 1))];
         }
 #pragma unroll
-        for (int h = 0; h < 2; h++) lam[h] = pis[i0 + ((tid + 256 * h) >> 3)];
+        for (int h = 0; h < NH; h++) lam[h] = pis[i0 + ((tid + 256 * h) >> 3)];
     };
     if (blockIdx.x < ntiles) load(blockIdx.x);
     for (size_t tile_i = blockIdx.x; tile_i < ntiles; tile_i += gridDim.x) {
-        const size_t i0 = (tile_i % row_tiles) * 64, c0 = (tile_i / row_tiles) * CT;
+        const size_t i0 = (tile_i % row_tiles) * TR, c0 = (tile_i / row_tiles) * CT;
 #pragma unroll
         for (int k = 0; k < KPT; k++) {
             const int idx = tid + 256 * k;
-            tile[idx >> 6][idx & 63] = tv[k];
+            tile[idx >> LTR][idx & (TR - 1)] = tv[k];
         }
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < NH; h++) {
             const int item = tid + 256 * h;
-            gf32_tab_build<true>(rtab + (item >> 3) * 128, lam[h], item & 7);
+            if (item < TR * 8) gf32_tab_build<true>(rtab + (item >> 3) * 128, lam[h], item & 7);
         }
         LIG_LDS_BARRIER();
         if (tile_i + gridDim.x < ntiles) load(tile_i + gridDim.x);
@@ -214,6 +215,10 @@ __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __res
 }
 
 constexpr int TS128_ROWS = 4;  // message rows per k_transpose_scale128 tile
+// log2 rows per k_transpose_scale32 tile: 32-row tiles (24.8 KB of LDS) let
+// five workgroups share a CU against three of 64 rows (49.4 KB), which hides
+// more of the column reads' latency (2^24 encode: 429.7 -> 426.6 us)
+constexpr int TS32_LTR = 5;
 __global__ __launch_bounds__(256) void k_transpose_scale128(const u128v* __restrict__ poly, size_t m, size_t n,
                                                             const u128v* __restrict__ pis, u128v* __restrict__ msg);
 
@@ -225,18 +230,18 @@ void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const
         if (m >= 64 && n >= 4 && (m & (m - 1)) == 0 && (n & (n - 1)) == 0) {
             int lct = 0;
             while ((1 << lct) < (n < 64 ? (int)n : 64)) lct++;
-            // one workgroup per tile: three fit a CU (49 KB of LDS at 64
-            // columns), and independent workgroups overlap one another's
-            // table builds and loads (measured faster than a persistent grid
-            // of three per CU walking the tiles with a prefetch)
-            if (tiles > 0x7fffffffu) throw std::invalid_argument("transpose: too many tiles");
-            const unsigned g = (unsigned)tiles;
+            // one workgroup per tile of 2^LTR rows (64 rows for four columns:
+            // a 256-thread tile needs 256 elements)
+            constexpr int LTR = TS32_LTR;
+            const size_t t32 = (m >> (lct == 2 ? 6 : LTR)) * (n >> lct);
+            if (t32 > 0x7fffffffu) throw std::invalid_argument("transpose: too many tiles");
+            const unsigned g = (unsigned)t32;
             switch (lct) {
-                case 2: hipLaunchKernelGGL(k_transpose_scale32<2>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
-                case 3: hipLaunchKernelGGL(k_transpose_scale32<3>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
-                case 4: hipLaunchKernelGGL(k_transpose_scale32<4>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
-                case 5: hipLaunchKernelGGL(k_transpose_scale32<5>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
-                default: hipLaunchKernelGGL(k_transpose_scale32<6>, dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
+                case 2: hipLaunchKernelGGL((k_transpose_scale32<2, 6>), dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
+                case 3: hipLaunchKernelGGL((k_transpose_scale32<3, LTR>), dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
+                case 4: hipLaunchKernelGGL((k_transpose_scale32<4, LTR>), dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
+                case 5: hipLaunchKernelGGL((k_transpose_scale32<5, LTR>), dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
+                default: hipLaunchKernelGGL((k_transpose_scale32<6, LTR>), dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
             }
         }
     }
