@@ -46,9 +46,11 @@ template <class F> void launch_build_tables(const typename F::T* lams, size_t co
 
 // ------------------------------------------------------------ Reed–Solomon --
 // msg[i*n + c] = poly[c*m + i] * pis[i]  (poly is column-major as in
-// ligero.rs:11-27; msg is row-major).  GF(2^32) reads the plain pis (m
-// words); GF(2^128) uses pis[i] = pis[i & 63] * pis[i & ~63] with lo_tabs
-// (min(m,64) tables) and hi_tabs (max(m/64,1) tables).
+// ligero.rs:11-27; msg is row-major).  Both fields read the plain pis (m
+// elements) and build one table per message row of a tile in LDS; narrow
+// matrices (GF(2^32): fewer than 4 columns or 64 rows, GF(2^128): fewer
+// than 64 columns) use pis[i] = pis[i & 63] * pis[i & ~63] with lo_tabs
+// (min(m,64) tables) and hi_tabs (max(m/64,1) tables) instead.
 template <class F>
 void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const typename F::T* pis,
                             const typename F::T* lo_tabs, const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s);
@@ -135,7 +137,9 @@ void launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t
 // GF(2^32) products per second by wave-uniform constants with nibble tables
 // in LDS (the FFT passes' multiply), 1 general GF(2^32) products per second
 // (carry-less product from integer multiplies), 2 32-bit VALU instructions
-// per second per lane (lane-ops/s).  scratch_word: one device word.
+// per second per lane (lane-ops/s), 3 products per second with 5-bit-window
+// tables, 4 LDS bytes read per second with the GF(2^128) table pattern.
+// scratch_word: one device word.
 double bench_rate(int which, uint32_t* scratch_word, hipStream_t s);
 
 // ------------------------------------------------------------ folds -------
