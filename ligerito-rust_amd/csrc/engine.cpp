@@ -315,6 +315,64 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
     memcpy(w.root, ctx->pin, 32);
 }
 
+// The first commits of several polys of one shape, each kernel launched once
+// for all of them: messages [count][m][n] from a table of input pointers,
+// codewords [count][4m][n] (the FFT passes see 4*count quarters), trees of
+// the standard layout tree_stride words apart, roots to pinned memory.
+template <class H>
+void commit_group(RsPlan<H>& rs, const typename H::T* const* d_polys, int count, size_t m, size_t n,
+                  std::vector<Witness<H>>& out, std::shared_ptr<void>& keep) {
+    using F = typename DevField<H>::F;
+    using T = typename H::T;
+    LIG_REQUIRE(F::BITS == 32, LIG_E_INVALID, "grouped commits are GF(2^32) only");
+    LIG_REQUIRE(count >= 1 && count <= 4096, LIG_E_INVALID, "grouped commits: 1..4096 polys");
+    LIG_REQUIRE(rs.m() == m && rs.M() == 4 * m && m >= 64 && n >= 4 && is_pow2(n), LIG_E_INVALID,
+                "grouped commits: plan m -> 4m, m >= 64, n >= 4 a power of two");
+    Ctx* ctx = rs.ctx;
+    hipStream_t s = ctx->stream;
+    const size_t M = 4 * m, tstride = (2 * M - 1) * 8;
+    const PassTune tune = rs.tune_for(n);
+    struct Group {
+        PBuf<T> mat;
+        PBuf<uint32_t> tree;
+    };
+    std::shared_ptr<Group> g = std::make_shared<Group>();
+    g->mat = PBuf<T>(&ctx->pool, (size_t)count * M * n);
+    g->tree = PBuf<uint32_t>(&ctx->pool, (size_t)count * tstride);
+    // pinned: the pointer table at PIN_PTRS, the roots after it
+    constexpr size_t PIN_PTRS = 4096;
+    uint8_t* pin = ctx->pinned(PIN_PTRS + (size_t)count * (8 + 32));
+    memcpy(pin + PIN_PTRS, d_polys, (size_t)count * sizeof(const T*));
+    uint32_t* roots_dev = reinterpret_cast<uint32_t*>(ctx->pin_dev + PIN_PTRS + (size_t)count * 8);
+    PBuf<const T*> d_ptrs(&ctx->pool, (size_t)count);
+    LIG_HIP(hipMemcpyAsync(d_ptrs.p, pin + PIN_PTRS, (size_t)count * sizeof(const T*), hipMemcpyHostToDevice, s));
+    {
+        PBuf<T> msg(&ctx->pool, (size_t)count * m * n);
+        launch_transpose_scale<F>(nullptr, m, n, rs.d_pis.p, rs.d_pis_lo.p, rs.d_pis_hi.p, msg.p, s, d_ptrs.p, count);
+        launch_fft_quarters<F>(msg.p, g->mat.p, m, n, rs.d_tw.p, rs.d_tw_tabs.p, s, nullptr, tune, count);
+        PBuf<uint32_t> scratch(&ctx->pool, leaf_hash_scratch_words(M, n * sizeof(T)));
+        launch_leaf_hash<F>(g->mat.p, (size_t)count * M, n, g->tree.p, scratch.n ? scratch.p : nullptr, s, count, tstride);
+        launch_merkle_levels(g->tree.p, M, s, roots_dev, nullptr, 0, count, tstride);
+        const uint32_t seq = ctx->next_flag();
+        launch_signal(ctx->flag_dev(), seq, s);
+        LIG_HIP(hipGetLastError());
+        ctx->wait_flag(seq);  // the roots are in pinned memory; msg and scratch return to the pool
+    }
+    out.clear();
+    out.resize((size_t)count);
+    for (int i = 0; i < count; i++) {
+        Witness<H>& w = out[(size_t)i];
+        w.ctx = ctx;
+        w.m = m;
+        w.n = n;
+        w.M = M;
+        w.mat = PBuf<T>::view(g->mat.p + (size_t)i * M * n, M * n);
+        w.tree = PBuf<uint32_t>::view(g->tree.p + (size_t)i * tstride, tstride);
+        memcpy(w.root, ctx->pin + PIN_PTRS + (size_t)count * 8 + (size_t)i * 32, 32);
+    }
+    keep = g;
+}
+
 // ----------------------------------------------------------------- config --
 Shape Shape::generated(int L) {
     LIG_REQUIRE(L >= 16 && L <= 27, LIG_E_INVALID, "prover config: log_n must be in [16, 27]");
@@ -562,7 +620,8 @@ void induce_launch(Ctx* ctx, int ell, const Vec<H>& sks, const PendingOpen<H>& p
 
 }  // namespace
 
-template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<H>& proof) {
+template <class H>
+void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<H>& proof, Witness<H>* first) {
     using F = typename DevField<H>::F;
     using T = typename H::T;
     using A = typename F::Acc;
@@ -585,7 +644,17 @@ template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint3
     const size_t final_M = 4 * cfg.dims_m.back(), final_mat_bytes = final_M * cfg.dims_n.back() * sizeof(T);
     ctx->pinned(PIN_FINAL + final_mat_bytes + (2 * final_M - 1) * 32);
     Witness<H> w0;
-    commit<H>(*cfg.initial_rs, d_poly, cfg.initial_m, cfg.initial_n, w0, PH_ENCODE0);
+    if (first) {
+        LIG_REQUIRE(first->m == cfg.initial_m && first->n == cfg.initial_n && first->M == 4 * cfg.initial_m && first->mat.p &&
+                        first->tree.p,
+                    LIG_E_INVALID, "prove: the given first commit has another shape");
+        w0 = std::move(*first);
+        // its openings run on this prove's stream, pool and pinned memory,
+        // not on those of the context that made the commit
+        w0.ctx = ctx;
+    } else {
+        commit<H>(*cfg.initial_rs, d_poly, cfg.initial_m, cfg.initial_n, w0, PH_ENCODE0);
+    }
     memcpy(proof.initial_root.data(), w0.root, 32);
     fs.absorb(w0.root, 32);
 
@@ -895,7 +964,9 @@ template struct Config<H32>;
 template struct Config<H128>;
 template void commit<H32>(RsPlan<H32>&, const uint32_t*, size_t, size_t, Witness<H32>&, int);
 template void commit<H128>(RsPlan<H128>&, const u128v*, size_t, size_t, Witness<H128>&, int);
-template void prove<H32>(Config<H32>&, const uint32_t*, uint32_t, Proof<H32>&);
-template void prove<H128>(Config<H128>&, const u128v*, uint32_t, Proof<H128>&);
+template void prove<H32>(Config<H32>&, const uint32_t*, uint32_t, Proof<H32>&, Witness<H32>*);
+template void prove<H128>(Config<H128>&, const u128v*, uint32_t, Proof<H128>&, Witness<H128>*);
+template void commit_group<H32>(RsPlan<H32>&, const uint32_t* const*, int, size_t, size_t, std::vector<Witness<H32>>&,
+                                std::shared_ptr<void>&);
 
 }  // namespace lig

@@ -167,7 +167,8 @@ struct PhaseTimer {
     }
 };
 
-// Pool-backed device buffer.
+// Pool-backed device buffer (or, made by view(), a view of memory owned
+// elsewhere, which release() leaves alone).
 template <class T> struct PBuf {
     Pool* pool = nullptr;
     T* p = nullptr;
@@ -188,8 +189,14 @@ template <class T> struct PBuf {
         return *this;
     }
     ~PBuf() { release(); }
+    static PBuf view(T* ptr, size_t count) {
+        PBuf b;
+        b.p = ptr;
+        b.n = count;
+        return b;
+    }
     void release() {
-        if (p) pool->put(p, n * sizeof(T));
+        if (p && pool) pool->put(p, n * sizeof(T));
         p = nullptr;
         n = 0;
     }
@@ -249,6 +256,13 @@ template <class H> struct Witness {
 
 // ligero_commit (ligero.rs:70-95) on a device poly (column-major m x n).
 template <class H> void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witness<H>& w, int encode_phase = PH_ENCODE);
+// ligero_commit of count device polys of one plan at once (GF(2^32) only:
+// H = H32; m >= 64, n >= 4): every launch covers all of them.  out[i] views buffers that keep
+// owns (from the plan's context pool); keep must outlive the witnesses and
+// every kernel that reads them.  Returns after the roots are on the host.
+template <class H>
+void commit_group(RsPlan<H>& rs, const typename H::T* const* d_polys, int count, size_t m, size_t n,
+                  std::vector<Witness<H>>& out, std::shared_ptr<void>& keep);
 
 // The dimensions of a ProverConfig (data_structures.rs:9-21); its
 // VerifierConfig (:23-30) is the same shape in log form (initial_dim =
@@ -323,7 +337,10 @@ template <class H> struct Proof {
 };
 
 // prover (prover.rs:15-199) on a device-resident poly.
-template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<H>& out);
+// With first, the first commit of d_poly (from commit_group on the same
+// shape) is taken from it instead of being computed.
+template <class H>
+void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<H>& out, Witness<H>* first = nullptr);
 // verifier (verifier.rs:10-164) on the host for a configuration's shape:
 // 1 accept, 0 reject, <0 at the reference's panic sites; with dev, the
 // basis inductions (verifier.rs:54-61,148-155) run on that context's GPU.

@@ -168,9 +168,16 @@ __device__ __forceinline__ uint32_t mul_tab_lds(const uint32_t* tabs, uint32_t t
 // one table multiply.  The launcher gives every tile its own workgroup; a
 // smaller grid walks the tiles, issuing the next tile's loads before this
 // tile is scaled and stored.  The barriers wait for LDS traffic only.
+// With polys, blockIdx.y picks one of several inputs, polys[blockIdx.y],
+// and the blockIdx.y-th m x n block of msg.
 template <int LCT, int LTR>
 __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __restrict__ poly, size_t m, size_t n,
-                                                          const uint32_t* __restrict__ pis, uint32_t* __restrict__ msg) {
+                                                          const uint32_t* __restrict__ pis, uint32_t* __restrict__ msg,
+                                                          const uint32_t* const* __restrict__ polys) {
+    if (polys) {
+        poly = polys[blockIdx.y];
+        msg += (size_t)blockIdx.y * m * n;
+    }
     constexpr int CT = 1 << LCT, TR = 1 << LTR, KPT = (TR * CT) / 256;  // elements per thread
     constexpr int NH = (TR * 8 + 255) / 256;                            // table positions per thread
     static_assert(KPT >= 1, "at least 256 elements per tile");
@@ -224,10 +231,14 @@ __global__ __launch_bounds__(256) void k_transpose_scale128(const u128v* __restr
 
 template <class F>
 void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const typename F::T* pis,
-                            const typename F::T* lo_tabs, const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s) {
+                            const typename F::T* lo_tabs, const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s,
+                            const typename F::T* const* polys, int inputs) {
     size_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
+    const bool fast32 = F::BITS == 32 && m >= 64 && n >= 4 && (m & (m - 1)) == 0 && (n & (n - 1)) == 0;
+    if (inputs != 1 && (!fast32 || !polys || inputs < 1 || inputs > 65535))
+        throw std::invalid_argument("several inputs per launch: GF(2^32), m >= 64, n >= 4, a pointer table");
     if constexpr (F::BITS == 32) {
-        if (m >= 64 && n >= 4 && (m & (m - 1)) == 0 && (n & (n - 1)) == 0) {
+        if (fast32) {
             int lct = 0;
             while ((1 << lct) < (n < 64 ? (int)n : 64)) lct++;
             // one workgroup per tile of 2^LTR rows (64 rows for four columns:
@@ -237,11 +248,16 @@ void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const
             if (t32 > 0x7fffffffu) throw std::invalid_argument("transpose: too many tiles");
             const unsigned g = (unsigned)t32;
             switch (lct) {
-                case 2: hipLaunchKernelGGL((k_transpose_scale32<2, 6>), dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
-                case 3: hipLaunchKernelGGL((k_transpose_scale32<3, LTR>), dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
-                case 4: hipLaunchKernelGGL((k_transpose_scale32<4, LTR>), dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
-                case 5: hipLaunchKernelGGL((k_transpose_scale32<5, LTR>), dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
-                default: hipLaunchKernelGGL((k_transpose_scale32<6, LTR>), dim3(g), dim3(256), 0, s, poly, m, n, pis, msg); return;
+                case 2: hipLaunchKernelGGL((k_transpose_scale32<2, 6>), dim3(g, (unsigned)inputs), dim3(256), 0, s, poly, m, n, pis, msg,
+                                           reinterpret_cast<const uint32_t* const*>(polys)); return;
+                case 3: hipLaunchKernelGGL((k_transpose_scale32<3, LTR>), dim3(g, (unsigned)inputs), dim3(256), 0, s, poly, m, n, pis, msg,
+                                           reinterpret_cast<const uint32_t* const*>(polys)); return;
+                case 4: hipLaunchKernelGGL((k_transpose_scale32<4, LTR>), dim3(g, (unsigned)inputs), dim3(256), 0, s, poly, m, n, pis, msg,
+                                           reinterpret_cast<const uint32_t* const*>(polys)); return;
+                case 5: hipLaunchKernelGGL((k_transpose_scale32<5, LTR>), dim3(g, (unsigned)inputs), dim3(256), 0, s, poly, m, n, pis, msg,
+                                           reinterpret_cast<const uint32_t* const*>(polys)); return;
+                default: hipLaunchKernelGGL((k_transpose_scale32<6, LTR>), dim3(g, (unsigned)inputs), dim3(256), 0, s, poly, m, n, pis, msg,
+                                           reinterpret_cast<const uint32_t* const*>(polys)); return;
             }
         }
     }
@@ -421,20 +437,22 @@ __global__ __launch_bounds__(256) void k_fft_pass(const typename F::T* __restric
 }
 
 static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw, hipStream_t s,
-                           PassObserver* obs, PassTune tune);
+                           PassObserver* obs, PassTune tune, int codewords);
 static void fft_quarters128(const u128v* msg, u128v* mat, size_t m, size_t n, const u128v* tw, hipStream_t s,
                             PassObserver* obs, PassTune tune);
 
 template <class F>
 void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw,
-                         const typename F::T* tw_tabs, hipStream_t s, PassObserver* obs, PassTune tune) {
+                         const typename F::T* tw_tabs, hipStream_t s, PassObserver* obs, PassTune tune, int codewords) {
     constexpr int K = PassShape<F>::K, C = PassShape<F>::C;
     if (n == 0 || (n & (n - 1)) != 0) throw std::invalid_argument("column count must be a power of two");
     int ell = 0;
     while (((size_t)1 << ell) < m) ell++;
+    if (codewords != 1 && (F::BITS != 32 || ell == 0 || codewords < 1 || codewords > 16383))
+        throw std::invalid_argument("several codewords per launch: GF(2^32), m >= 2, at most 16383");
     if constexpr (F::BITS == 32) {
         if (ell > 0) {
-            fft_quarters32(msg, mat, m, n, tw, s, obs, tune);
+            fft_quarters32(msg, mat, m, n, tw, s, obs, tune, codewords);
             return;
         }
     } else {
@@ -738,7 +756,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
     constexpr int L1 = 1 << K1, L2 = 1 << K2;
     __shared__ __attribute__((aligned(256))) uint32_t tabs[NT * 128];
     __shared__ uint32_t xch[K2 > 0 ? R * 64 : 1];
-    const uint32_t q = q_arg >= 0 ? (uint32_t)q_arg : blockIdx.y;
+    // blockIdx.y runs over the quarters of one or more codewords laid out
+    // one after another ([codeword][quarter][m][n]); quarter qq is quarter
+    // qq & 3 of codeword qq >> 2, whose message (when shared) is the
+    // (qq >> 2)-th m x n block of src
+    const uint32_t qq = q_arg >= 0 ? (uint32_t)q_arg : blockIdx.y;
+    const uint32_t q = qq & 3u;
     const int cc = n < 64 ? (int)n : 64;
     const int lgcc = 31 - __clz(cc);
     const uint32_t nchunks = (uint32_t)((n + 63) >> 6);  // a power of two
@@ -751,8 +774,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
     const uint32_t rs4 = (S * (uint32_t)n) << 2;  // byte stride between tile rows
     const uint32_t lane4 = (g * (S * (uint32_t)n) + c) << 2;
     const uint32_t qbytes = (uint32_t)(m * n * 4);
-    const __amdgpu_buffer_rsrc_t rsrc = buf_rsrc(src + (src_shared ? (size_t)0 : (size_t)q * m * n), qbytes);
-    const __amdgpu_buffer_rsrc_t rdst = buf_rsrc(dst + (size_t)q * m * n, qbytes);
+    const __amdgpu_buffer_rsrc_t rsrc = buf_rsrc(src + (size_t)(src_shared ? (qq >> 2) : qq) * m * n, qbytes);
+    const __amdgpu_buffer_rsrc_t rdst = buf_rsrc(dst + (size_t)qq * m * n, qbytes);
     // tiles of one workgroup share B (consecutive lo), hence their tables
     const uint32_t first_job = blockIdx.x * (uint32_t)tiles_per_wg;
     const uint32_t B = (first_job >> lgnc) >> sh;
@@ -923,7 +946,7 @@ static void launch_pass32(int K, dim3 grid, int block, hipStream_t s, const uint
 // GF(2^32) encode passes, quarter by quarter so that a quarter (m x n x 4
 // bytes) stays resident in the last-level cache between its passes.
 static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t n, const uint32_t* tw, hipStream_t s,
-                           PassObserver* obs, PassTune tune) {
+                           PassObserver* obs, PassTune tune, int codewords) {
     if (m * n * sizeof(uint32_t) >= ((size_t)1 << 31)) throw std::invalid_argument("FFT quarter of 2 GiB or more");
     if (n == 0 || (n & (n - 1)) != 0) throw std::invalid_argument("column count must be a power of two");
     if (tune.kmax < 0 || tune.kmax == 1 || tune.kmax > 6 || tune.group < 0 || tune.group > 8 || (tune.group & (tune.group - 1)))
@@ -961,8 +984,8 @@ static void fft_quarters32(const uint32_t* msg, uint32_t* mat, size_t m, size_t 
         while (tpw < TPW && (size_t)(tpw * 2) <= S * nchunks && jobs / (tpw * 2) >= 512) tpw *= 2;
         const uint32_t* src = (p == 0 && msg) ? msg : mat;
         const int shared = (p == 0 && msg) ? 1 : 0;
-        dim3 grid((unsigned)(jobs / tpw), 4);
-        const size_t elems = m * n * 4;
+        dim3 grid((unsigned)(jobs / tpw), 4u * (unsigned)codewords);
+        const size_t elems = m * n * 4 * (size_t)codewords;
         if (obs) obs->before(K, elems);
         launch_pass32(K, grid, cc << K2, s, src, mat, pm, pn, pell, d0, -1, shared, tpw, tw);
         if (obs) obs->after(K, elems);
@@ -1073,10 +1096,13 @@ void launch_leaf_hash_var(const uint8_t* bytes, const uint64_t* offs, size_t n_l
 
 // Rows of NB whole 64-byte blocks (one chunk): each lane loads its whole row
 // first (all loads in flight together, so each row's cache lines are read
-// from HBM once), then chains the NB compressions from registers.
+// from HBM once), then chains the NB compressions from registers.  Rows
+// come in groups of 2^lg_per, one per tree: the leaf of row r goes to
+// layer0 + (r >> lg_per) * tree_stride + (r mod 2^lg_per) * 8.
 template <int NB>
 __global__ __launch_bounds__(128) void k_leaf_hash_blocks(const uint32_t* __restrict__ mat, size_t rows,
-                                                          uint32_t* __restrict__ layer0) {
+                                                          uint32_t* __restrict__ layer0, int lg_per, size_t tree_stride) {
+    const size_t per_mask = lg_per >= 63 ? ~(size_t)0 : ((size_t)1 << lg_per) - 1;
     for (size_t row = blockIdx.x * (size_t)blockDim.x + threadIdx.x; row < rows; row += (size_t)gridDim.x * blockDim.x) {
         const uint4* p = reinterpret_cast<const uint4*>(mat + row * 16 * NB);
         uint4 v[4 * NB];
@@ -1098,7 +1124,8 @@ __global__ __launch_bounds__(128) void k_leaf_hash_blocks(const uint32_t* __rest
             const uint32_t fl = (b == 0 ? b3::CHUNK_START : 0u) | (b == NB - 1 ? (b3::CHUNK_END | b3::ROOT) : 0u);
             b3::compress_cv(cv, m, 0, 64, fl, cv);
         }
-        uint4* o = reinterpret_cast<uint4*>(layer0 + row * 8);
+        const size_t tr = lg_per >= 63 ? 0 : row >> lg_per;
+        uint4* o = reinterpret_cast<uint4*>(layer0 + tr * tree_stride + (row & per_mask) * 8);
         o[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
         o[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
     }
@@ -1174,8 +1201,29 @@ size_t leaf_hash_scratch_words(size_t rows, size_t row_bytes) {
 }
 
 template <class F>
-void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, uint32_t* scratch, hipStream_t s) {
+void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, uint32_t* scratch, hipStream_t s,
+                      int trees, size_t tree_stride) {
     const size_t words = n * (sizeof(typename F::T) / 4);
+    if (trees != 1) {
+        if (trees < 1 || rows % (size_t)trees != 0) throw std::invalid_argument("leaf hash: rows must split evenly over the trees");
+        const size_t per = rows / (size_t)trees;
+        if (per & (per - 1)) throw std::invalid_argument("leaf hash: rows per tree must be a power of two");
+        int lg = 0;
+        while (((size_t)1 << lg) < per) lg++;
+        if (words == 16 || words == 32 || words == 64) {
+            const dim3 grid(grid_for(rows, 128)), block(128);
+            const uint32_t* m = reinterpret_cast<const uint32_t*>(mat);
+            if (words == 16) hipLaunchKernelGGL(k_leaf_hash_blocks<1>, grid, block, 0, s, m, rows, layer0, lg, tree_stride);
+            else if (words == 32) hipLaunchKernelGGL(k_leaf_hash_blocks<2>, grid, block, 0, s, m, rows, layer0, lg, tree_stride);
+            else hipLaunchKernelGGL(k_leaf_hash_blocks<4>, grid, block, 0, s, m, rows, layer0, lg, tree_stride);
+            return;
+        }
+        // other leaf shapes: one tree per launch (the scratch is reused in
+        // stream order)
+        for (int t = 0; t < trees; t++)
+            launch_leaf_hash<F>(mat + (size_t)t * per * n, per, n, layer0 + (size_t)t * tree_stride, scratch, s, 1, 0);
+        return;
+    }
     const dim3 grid(grid_for(rows, 128)), block(128);
     const uint32_t* m = reinterpret_cast<const uint32_t*>(mat);
     if (scratch && leaf_hash_scratch_words(rows, words * 4)) {
@@ -1186,16 +1234,18 @@ void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t*
         return;
     }
     switch (words) {
-        case 16: hipLaunchKernelGGL(k_leaf_hash_blocks<1>, grid, block, 0, s, m, rows, layer0); return;
-        case 32: hipLaunchKernelGGL(k_leaf_hash_blocks<2>, grid, block, 0, s, m, rows, layer0); return;
-        case 64: hipLaunchKernelGGL(k_leaf_hash_blocks<4>, grid, block, 0, s, m, rows, layer0); return;
+        case 16: hipLaunchKernelGGL(k_leaf_hash_blocks<1>, grid, block, 0, s, m, rows, layer0, 63, (size_t)0); return;
+        case 32: hipLaunchKernelGGL(k_leaf_hash_blocks<2>, grid, block, 0, s, m, rows, layer0, 63, (size_t)0); return;
+        case 64: hipLaunchKernelGGL(k_leaf_hash_blocks<4>, grid, block, 0, s, m, rows, layer0, 63, (size_t)0); return;
         default: hipLaunchKernelGGL(k_leaf_hash<F>, grid, block, 0, s, mat, rows, n, layer0); return;
     }
 }
 
 // ----------------------------------------------------------- Merkle levels --
 // One tree level: each thread hashes one pair of children.
-__global__ __launch_bounds__(256) void k_merkle_level(uint32_t* __restrict__ tree, size_t in_off, size_t n_out) {
+__global__ __launch_bounds__(256) void k_merkle_level(uint32_t* __restrict__ tree, size_t in_off, size_t n_out,
+                                                      size_t tree_stride) {
+    tree += blockIdx.z * tree_stride;  // one tree per blockIdx.z
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_out; i += (size_t)gridDim.x * blockDim.x) {
         const uint4* src = reinterpret_cast<const uint4*>(tree + (in_off + 2 * i) * 8);
         uint32_t msg[16];
@@ -1256,8 +1306,11 @@ __device__ __forceinline__ uint32_t rotr(uint32_t x, int n) { return (x >> n) | 
 // seq there (host memory; system scope, after the nodes and the root).
 __global__ __launch_bounds__(1024) void k_merkle_subtree4(uint32_t* __restrict__ tree, size_t in_off, size_t n_in, int g,
                                                           uint32_t* __restrict__ root_out, uint32_t* __restrict__ flag,
-                                                          uint32_t seq) {
+                                                          uint32_t seq, size_t tree_stride) {
     using namespace b3x4;
+    // one tree per blockIdx.z (a flag is only given for a single tree)
+    tree += blockIdx.z * tree_stride;
+    if (root_out) root_out += blockIdx.z * 8;
     extern __shared__ uint32_t sub4_lds[];
     uint32_t* lv[2] = {sub4_lds, sub4_lds + ((size_t)8 << g)};
     const int k = threadIdx.x & 3, quad = threadIdx.x >> 2, nquads = blockDim.x >> 2;
@@ -1330,7 +1383,9 @@ __global__ __launch_bounds__(1024) void k_merkle_subtree4(uint32_t* __restrict__
 // the layer at in_off into their 4 parents, 2 grandparents and 1
 // great-grandparent (7 compressions, no idle lanes, one launch instead of
 // three).  n_out = number of great-grandparents = n_in / 8.
-__global__ __launch_bounds__(256) void k_merkle_block3(uint32_t* __restrict__ tree, size_t in_off, size_t n_in) {
+__global__ __launch_bounds__(256) void k_merkle_block3(uint32_t* __restrict__ tree, size_t in_off, size_t n_in,
+                                                       size_t tree_stride) {
+    tree += blockIdx.z * tree_stride;  // one tree per blockIdx.z
     const size_t n_out = n_in / 8;
     const size_t l1 = in_off + n_in, l2 = l1 + n_in / 2, l3 = l2 + n_in / 4;  // first node of each layer
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n_out; t += (size_t)gridDim.x * blockDim.x) {
@@ -1381,16 +1436,20 @@ constexpr size_t MERKLE_WIDE = (size_t)1 << 17;
 // it, 256-thread workgroups of depth SUB4_G.
 constexpr int SUB4_G = 7, SUB4_TOP = 10;
 
-void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out, uint32_t* flag, uint32_t seq) {
+void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out, uint32_t* flag, uint32_t seq,
+                          int trees, size_t tree_stride) {
+    if (trees < 1 || trees > 65535 || (trees > 1 && flag)) throw std::invalid_argument("merkle levels: trees 1..65535, no flag for several");
+    const unsigned z = (unsigned)trees;
     size_t in_off = 0, nodes = n_leaves;
     while (nodes / 8 >= MERKLE_WIDE) {
-        hipLaunchKernelGGL(k_merkle_block3, dim3(grid_for(nodes / 8, 256)), dim3(256), 0, s, tree, in_off, nodes);
+        hipLaunchKernelGGL(k_merkle_block3, dim3(grid_for(nodes / 8, 256), 1, z), dim3(256), 0, s, tree, in_off, nodes, tree_stride);
         in_off += nodes + nodes / 2 + nodes / 4;
         nodes /= 8;
     }
     while (nodes / 2 >= MERKLE_WIDE) {
         const size_t n_out = nodes / 2;
-        hipLaunchKernelGGL(k_merkle_level, dim3((unsigned)((n_out + 255) / 256)), dim3(256), 0, s, tree, in_off, n_out);
+        hipLaunchKernelGGL(k_merkle_level, dim3((unsigned)((n_out + 255) / 256), 1, z), dim3(256), 0, s, tree, in_off, n_out,
+                           tree_stride);
         in_off += nodes;
         nodes = n_out;
     }
@@ -1401,8 +1460,8 @@ void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32
         const int g = top ? lg : SUB4_G;
         const size_t groups = nodes >> g;
         const size_t lds = (size_t)64 << g;  // two layers of 2^g nodes
-        hipLaunchKernelGGL(k_merkle_subtree4, dim3((unsigned)groups), dim3(top ? 1024 : 256), lds, s, tree, in_off, nodes, g,
-                           root_out, top ? flag : nullptr, seq);
+        hipLaunchKernelGGL(k_merkle_subtree4, dim3((unsigned)groups, 1, z), dim3(top ? 1024 : 256), lds, s, tree, in_off, nodes, g,
+                           root_out, top ? flag : nullptr, seq, tree_stride);
         for (int l = 0; l < g; l++) {
             in_off += nodes;
             nodes >>= 1;
@@ -2393,13 +2452,13 @@ double bench_rate(int which, uint32_t* scratch_word, hipStream_t s) {
 #define LIG_INST(F)                                                                                                         \
     template void launch_build_tables<F>(const F::T*, size_t, F::T*, hipStream_t);                                          \
     template void launch_transpose_scale<F>(const F::T*, size_t, size_t, const F::T*, const F::T*, const F::T*, F::T*,      \
-                                            hipStream_t);                                                                   \
+                                            hipStream_t, const F::T* const*, int);                                          \
     template void launch_scale_rows<F>(F::T*, size_t, size_t, const F::T*, hipStream_t);                                    \
     template void launch_transpose<F>(const F::T*, size_t, size_t, F::T*, hipStream_t);                                     \
     template void launch_fft_stage<F>(F::T*, size_t, size_t, int, const F::T*, int, hipStream_t);                           \
     template void launch_fft_quarters<F>(const F::T*, F::T*, size_t, size_t, const F::T*, const F::T*, hipStream_t,         \
-                                         PassObserver*, PassTune);                                                           \
-    template void launch_leaf_hash<F>(const F::T*, size_t, size_t, uint32_t*, uint32_t*, hipStream_t);                                 \
+                                         PassObserver*, PassTune, int);                                                      \
+    template void launch_leaf_hash<F>(const F::T*, size_t, size_t, uint32_t*, uint32_t*, hipStream_t, int, size_t);                   \
     template void launch_fold<F>(const F::T*, size_t, int, const F::T*, F::T*, hipStream_t);                                \
     template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, F::T*, hipStream_t);                  \
     template int launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t,                     \

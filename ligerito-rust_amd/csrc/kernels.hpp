@@ -51,9 +51,13 @@ template <class F> void launch_build_tables(const typename F::T* lams, size_t co
 // matrices (GF(2^32): fewer than 4 columns or 64 rows, GF(2^128): fewer
 // than 64 columns) use pis[i] = pis[i & 63] * pis[i & ~63] with lo_tabs
 // (min(m,64) tables) and hi_tabs (max(m/64,1) tables) instead.
+// With inputs > 1 (GF(2^32), m >= 64, n >= 4): that many inputs at once,
+// polys a device array of their pointers, msg receiving their messages one
+// after another ([input][m][n]); poly is then unused.
 template <class F>
 void launch_transpose_scale(const typename F::T* poly, size_t m, size_t n, const typename F::T* pis,
-                            const typename F::T* lo_tabs, const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s);
+                            const typename F::T* lo_tabs, const typename F::T* hi_tabs, typename F::T* msg, hipStream_t s,
+                            const typename F::T* const* polys = nullptr, int inputs = 1);
 // buf[i*n + c] *= pis[i] for i < rows (general multiply).
 template <class F> void launch_scale_rows(typename F::T* buf, size_t rows, size_t n, const typename F::T* pis, hipStream_t s);
 // dst[c*rows + r] = src[r*cols + c] (src row-major rows x cols; dst != src).
@@ -93,18 +97,26 @@ struct PassTune {
     int kmax = 0;
     int group = 0;
 };
+// With codewords > 1 (GF(2^32), m >= 2): that many codewords of the same
+// plan at once, mat holding them one after another ([codeword][4m][n]) and
+// msg (if given) their messages ([codeword][m][n]).
 template <class F>
 void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m, size_t n, const typename F::T* tw,
-                         const typename F::T* tw_tabs, hipStream_t s, PassObserver* obs = nullptr, PassTune tune = {});
+                         const typename F::T* tw_tabs, hipStream_t s, PassObserver* obs = nullptr, PassTune tune = {},
+                         int codewords = 1);
 
 // -------------------------------------------------------------- Merkle ----
 // layer0[row*8 .. +8] = BLAKE3(row bytes) for every row of the [rows][n] matrix.
 // Leaves of whole 1 KiB chunks (a power of two of them, 2..256) hash chunk-
 // parallel when scratch is given: leaf_hash_scratch_words(rows, row bytes)
 // words (0 when the leaf shape does not qualify).
+// With trees > 1 the rows are that many equal matrices one after another,
+// the leaves of matrix t going to layer0 + t * tree_stride (words); scratch
+// (if the leaf shape needs it) is sized for one matrix and reused.
 size_t leaf_hash_scratch_words(size_t rows, size_t row_bytes);
 template <class F>
-void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, uint32_t* scratch, hipStream_t s);
+void launch_leaf_hash(const typename F::T* mat, size_t rows, size_t n, uint32_t* layer0, uint32_t* scratch, hipStream_t s,
+                      int trees = 1, size_t tree_stride = 0);
 // layer0[i*8 .. +8] = BLAKE3(bytes[offs[i] .. offs[i+1])) for leaves of any
 // length up to 256 KiB (offs: n_leaves + 1 non-decreasing byte offsets).
 void launch_leaf_hash_var(const uint8_t* bytes, const uint64_t* offs, size_t n_leaves, uint32_t* layer0, hipStream_t s);
@@ -113,8 +125,11 @@ void launch_leaf_hash_var(const uint8_t* bytes, const uint64_t* offs, size_t n_l
 // If root_out is given, the root's 8 words are also written there (for
 // instance to mapped host memory); if flag is given, the last launch stores
 // seq there after every node and the root (for a host polling mapped memory).
+// With trees > 1: that many trees of n_leaves leaves, tree_stride words
+// apart, their roots to root_out + 8t (if given); no flag (launch_signal
+// after them instead).
 void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out = nullptr,
-                          uint32_t* flag = nullptr, uint32_t seq = 0);
+                          uint32_t* flag = nullptr, uint32_t seq = 0, int trees = 1, size_t tree_stride = 0);
 // A few words copied by the first threads of a launch (for instance a
 // section's last triple, as raw accumulators, to mapped host memory).
 struct WordCopy {
