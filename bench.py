@@ -219,7 +219,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "first-commit Reed-Solomon encode: k_transpose_scale32<6> and three k_fft_pass32<3,3> "
+            "kernel": "first-commit Reed-Solomon encode: k_transpose_scale32<6,5> and three k_fft_pass32<3,3> "
                       "launches (4 x 2^18 x 64 codeword)",
             "achieved": enc_gbps,
             "peak": HBM_PEAK_GBS,
