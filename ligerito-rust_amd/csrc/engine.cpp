@@ -348,7 +348,7 @@ void commit_group(RsPlan<H>& rs, const typename H::T* const* d_polys, int count,
     LIG_HIP(hipMemcpyAsync(d_ptrs.p, pin + PIN_PTRS, (size_t)count * sizeof(const T*), hipMemcpyHostToDevice, s));
     {
         PBuf<T> msg(&ctx->pool, (size_t)count * m * n);
-        launch_transpose_scale<F>(nullptr, m, n, rs.d_pis.p, rs.d_pis_lo.p, rs.d_pis_hi.p, msg.p, s, d_ptrs.p, count);
+        launch_transpose_scale<F>(d_polys[0], m, n, rs.d_pis.p, rs.d_pis_lo.p, rs.d_pis_hi.p, msg.p, s, d_ptrs.p, count);
         launch_fft_quarters<F>(msg.p, g->mat.p, m, n, rs.d_tw.p, rs.d_tw_tabs.p, s, nullptr, tune, count);
         PBuf<uint32_t> scratch(&ctx->pool, leaf_hash_scratch_words(M, n * sizeof(T)));
         launch_leaf_hash<F>(g->mat.p, (size_t)count * M, n, g->tree.p, scratch.n ? scratch.p : nullptr, s, count, tstride);
