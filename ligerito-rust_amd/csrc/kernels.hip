@@ -781,43 +781,64 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
     const uint32_t B = (first_job >> lgnc) >> sh;
     uint32_t zt = 0;  // bit j: phase-1 stage j's first table is zero
 
-    for (int tj = 0; tj < tiles_per_wg; tj++) {
+    // byte offset of tile tj's first element
+    auto tile_off = [&](int tj) -> uint32_t {
         const uint32_t job = first_job + tj;
         const uint32_t tile_id = job >> lgnc;
         const uint32_t c0 = (job & (nchunks - 1)) << 6;
         const uint32_t lo = tile_id & (S - 1);
         const uint32_t rbase = (B << (ell - d0)) + lo;
-        const uint32_t toff = (rbase * (uint32_t)n + c0) << 2;  // tile's first element, bytes
-        uint32_t v[L1];
+        return (rbase * (uint32_t)n + c0) << 2;
+    };
+    // Twiddle tables of the workgroup's tiles: table (j, t') multiplies by
+    // the heap twiddle of layer d0+j, block B*2^j + t'.  Work item (table,
+    // nibble position k) builds entries v = 0..15 from c_b = lambda *
+    // x^(4k+b): entry v = XOR of c_b over the bits of v.
+    auto lam_of = [&](int idx) -> uint32_t {
+        const int tt = idx >> 3;
+        const int j = 31 - __clz(tt + 1);
+        const uint32_t tp = (uint32_t)(tt + 1) - (1u << j);
+        const int d = d0 + j;
+        const size_t first = ((size_t)1 << (d + 2)) - 1;
+        return tw[first + ((size_t)q << d) + ((size_t)B << j) + tp];
+    };
+    // A thread's first twiddle is loaded before tile 0's words, so the table
+    // build waits for it alone while the tile's loads are in flight.  Tiles
+    // then alternate between the two register sets: the next tile's words
+    // are loaded while this tile computes (one set per role, no copies).
+    uint32_t va[L1], vb[L1];
+    const uint32_t lam0 = tid < NT * 8 ? lam_of(tid) : 0u;
+    {
+        const uint32_t toff0 = tile_off(0);
 #pragma unroll
-        for (int a = 0; a < L1; a++) v[a] = buf_ld(rsrc, lane4, toff + (uint32_t)(a * L2) * rs4);
-        if (tj == 0) {
-            // Twiddle tables of the tile: table (j, t') multiplies by the
-            // heap twiddle of layer d0+j, block B*2^j + t'.  Work item
-            // (table, nibble position k) builds entries v = 0..15 from
-            // c_b = lambda * x^(4k+b): entry v = XOR of c_b over the bits of v.
-            for (int idx = tid; idx < NT * 8; idx += blockDim.x) {
-                const int tt = idx >> 3, k = idx & 7;
-                const int j = 31 - __clz(tt + 1);
-                const uint32_t tp = (uint32_t)(tt + 1) - (1u << j);
-                const int d = d0 + j;
-                const size_t first = ((size_t)1 << (d + 2)) - 1;
-                const uint32_t lam = tw[first + ((size_t)q << d) + ((size_t)B << j) + tp];
-                gf32_tab_build<true>(tabs + tt * 128, lam, k);
-            }
-        } else {
-            LIG_LDS_BARRIER();  // previous tile's exchange reads are done
-        }
-        LIG_LDS_BARRIER();  // tables complete (this tile's loads may still be in flight)
-        if (tj == 0) {
-            // Stage j's first table (block B*2^j) is all zero when its twiddle
-            // is: entry 1 of nibble 0 is the twiddle itself.  The heap's
-            // first twiddle of every layer is zero, so quarter 0's leading
-            // tiles skip those multiplies (a quarter of the first stage's
-            // work and less after it).  Same for every tile of the workgroup.
+        for (int a = 0; a < L1; a++) va[a] = buf_ld(rsrc, lane4, toff0 + (uint32_t)(a * L2) * rs4);
+    }
+    if (tid < NT * 8) gf32_tab_build<true>(tabs + (tid >> 3) * 128, lam0, tid & 7);
+    for (int idx = tid + (int)blockDim.x; idx < NT * 8; idx += blockDim.x)
+        gf32_tab_build<true>(tabs + (idx >> 3) * 128, lam_of(idx), idx & 7);
+    LIG_LDS_BARRIER();
+    // Stage j's first table (block B*2^j) is all zero when its twiddle is:
+    // entry 1 of nibble 0 is the twiddle itself.  The heap's first twiddle
+    // of every layer is zero, so quarter 0's leading tiles skip those
+    // multiplies (a quarter of the first stage's work and less after it).
+    // Same for every tile of the workgroup.
 #pragma unroll
-            for (int j = 0; j < K1; j++) zt |= (tabs[((1 << j) - 1) * 128 + 1] == 0u ? 1u : 0u) << j;
-            zt = __builtin_amdgcn_readfirstlane(zt);
+    for (int j = 0; j < K1; j++) zt |= (tabs[((1 << j) - 1) * 128 + 1] == 0u ? 1u : 0u) << j;
+    zt = __builtin_amdgcn_readfirstlane(zt);
+
+    auto tile = [&](uint32_t(&v)[L1], uint32_t(&vn)[L1], int tj) {
+        const uint32_t toff = tile_off(tj);
+        if (tj > 0) LIG_LDS_BARRIER();  // the previous tile's exchange reads are done
+        {
+            // issued for every tile (a conditional load here makes the
+            // compiler wait for all loads in flight): after the last tile
+            // the lane offset lies past the quarter, so the buffer loads
+            // return zeros without touching memory (offsets stay below 2^32:
+            // quarter offsets are below 2^31)
+            const uint32_t toffn = tile_off(tj + 1);
+            const uint32_t lanen = tj + 1 < tiles_per_wg ? lane4 : 0x80000000u;
+#pragma unroll
+            for (int a = 0; a < L1; a++) vn[a] = buf_ld(rsrc, lanen, toffn + (uint32_t)(a * L2) * rs4);
         }
 
         auto stage1 = [&](auto J) {
@@ -923,6 +944,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k
                     buf_st(rdst, lane_st, toff + (uint32_t)(L2 * L2 * i + e) * rs4, v[i * L2 + e]);
             }
         }
+    };
+    // tile 0 first, so that every entry to the loop has the same memory
+    // operations in flight (the prefetched words, then the stores), which
+    // lets each tile wait for its own words only
+    tile(va, vb, 0);
+    for (int tj = 1;; tj += 2) {
+        if (tj >= tiles_per_wg) break;
+        tile(vb, va, tj);
+        if (tj + 1 >= tiles_per_wg) break;
+        tile(va, vb, tj + 1);
     }
 }
 
