@@ -1342,8 +1342,11 @@ __global__ __launch_bounds__(1024) void k_merkle_subtree4(uint32_t* __restrict__
     // one tree per blockIdx.z (a flag is only given for a single tree)
     tree += blockIdx.z * tree_stride;
     if (root_out) root_out += blockIdx.z * 8;
+    // the two layers live at word 0 and word 8 << g; picked by offset (a
+    // pointer array indexed at run time would turn the LDS accesses into
+    // flat ones)
     extern __shared__ uint32_t sub4_lds[];
-    uint32_t* lv[2] = {sub4_lds, sub4_lds + ((size_t)8 << g)};
+    const uint32_t lv_words = 8u << g;
     const int k = threadIdx.x & 3, quad = threadIdx.x >> 2, nquads = blockDim.x >> 2;
     const size_t base = blockIdx.x * ((size_t)1 << g);  // first input node of this workgroup
     // message word index per (round, slot) for this lane
@@ -1358,15 +1361,15 @@ __global__ __launch_bounds__(1024) void k_merkle_subtree4(uint32_t* __restrict__
     const uint32_t d_init = k == 2 ? 64u : k == 3 ? (uint32_t)(b3::CHUNK_START | b3::CHUNK_END | b3::ROOT) : 0u;
     {
         const uint4* src = reinterpret_cast<const uint4*>(tree + (in_off + base) * 8);
-        uint4* dst = reinterpret_cast<uint4*>(lv[0]);
+        uint4* dst = reinterpret_cast<uint4*>(sub4_lds);
         for (int i = threadIdx.x; i < (2 << g); i += blockDim.x) dst[i] = src[i];
     }
     __syncthreads();
     size_t lay_off = in_off, lay_n = n_in;
     for (int l = 1; l <= g; l++) {
         const int cnt = 1 << (g - l);
-        const uint32_t* in = lv[(l - 1) & 1];
-        uint32_t* out = lv[l & 1];
+        const uint32_t* in = sub4_lds + ((l - 1) & 1) * lv_words;
+        uint32_t* out = sub4_lds + (l & 1) * lv_words;
         const size_t out_off = lay_off + lay_n;
         const size_t out_base = base >> l;
         for (int p = quad; p < cnt; p += nquads) {
