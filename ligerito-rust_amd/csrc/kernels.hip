@@ -178,6 +178,9 @@ __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __res
         poly = polys[blockIdx.y];
         msg += (size_t)blockIdx.y * m * n;
     }
+    // a pointer read from memory is generic to the compiler: name its
+    // address space so the tile loads are global loads, not flat ones
+    const __attribute__((address_space(1))) uint32_t* gpoly = (const __attribute__((address_space(1))) uint32_t*)poly;
     constexpr int CT = 1 << LCT, TR = 1 << LTR, KPT = (TR * CT) / 256;  // elements per thread
     constexpr int NH = (TR * 8 + 255) / 256;                            // table positions per thread
     static_assert(KPT >= 1, "at least 256 elements per tile");
@@ -191,7 +194,7 @@ __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __res
 #pragma unroll
         for (int k = 0; k < KPT; k++) {
             const int idx = tid + 256 * k;
-            tv[k] = poly[(c0 + (idx >> LTR)) * m + i0 + (idx & (TR - 1))];
+            tv[k] = gpoly[(c0 + (idx >> LTR)) * m + i0 + (idx & (TR - 1))];
         }
 #pragma unroll
         for (int h = 0; h < NH; h++) lam[h] = pis[i0 + ((tid + 256 * h) >> 3)];
@@ -2178,6 +2181,17 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
                                                       0, 0, 0);
             asm volatile("" ::"v"(d.v[0]));
         }
+        if (step.enf && blockIdx.x == 0 && threadIdx.x >= 128 && threadIdx.x < 192) {
+            // the enforced-sum check (ligerito.rs:35,52) on wave 2 of the
+            // first workgroup, off the transcript's path: its loads would
+            // otherwise hold up the compressions of every workgroup
+            A e = Ops<F>::acc_zero();
+            for (int i = threadIdx.x - 128; i < step.n_enf; i += 64) e ^= step.enf[i];
+            e = wave_xor(e);
+            const T t0 = Ops<F>::acc_reduce(dfs::uniform_acc(step.res_in[0]));
+            const T t1 = Ops<F>::acc_reduce(dfs::uniform_acc(step.res_in[1]));
+            if (threadIdx.x == 128 && !Ops<F>::eq(t0 ^ t1, Ops<F>::acc_reduce(dfs::uniform_acc(e)))) *step.err = 1;
+        }
         if (threadIdx.x < 64) {
             dfs::State st;
             if (step.st_in)
@@ -2187,13 +2201,6 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
             T t[3];
 #pragma unroll
             for (int k = 0; k < 3; k++) t[k] = Ops<F>::acc_reduce(dfs::uniform_acc(step.res_in[k]));
-            uint32_t bad = 0;
-            if (step.enf) {
-                A e = Ops<F>::acc_zero();
-                for (int i = threadIdx.x; i < step.n_enf; i += 64) e ^= step.enf[i];
-                e = wave_xor(e);
-                bad = Ops<F>::eq(t[0] ^ t[1], Ops<F>::acc_reduce(dfs::uniform_acc(e))) ? 0u : 1u;
-            }
 #pragma unroll
             for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -2214,7 +2221,6 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
                     for (int k = 0; k < 3; k++) step.tlog[k] = t[k];
                     step.rlog[0] = rv0;
                     if (step.n_squeeze == 2) step.rlog[1] = rv1;
-                    if (bad) *step.err = 1;
                 }
             }
         }
