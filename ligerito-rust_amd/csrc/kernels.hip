@@ -224,7 +224,8 @@ __global__ __launch_bounds__(256) void k_transpose_scale32(const uint32_t* __res
     }
 }
 
-constexpr int TS128_ROWS = 4;  // message rows per k_transpose_scale128 tile
+constexpr int TS128_ROWS = 8;      // message rows per k_transpose_scale128 tile
+constexpr int TS128_TAB_ROWS = 4;  // rows whose tables share LDS at a time (32 KB)
 // log2 rows per k_transpose_scale32 tile: 32-row tiles (24.8 KB of LDS) let
 // five workgroups share a CU against three of 64 rows (49.4 KB), which hides
 // more of the column reads' latency (2^24 encode: 429.7 -> 426.6 us)
@@ -482,16 +483,31 @@ void launch_fft_quarters(const typename F::T* msg, typename F::T* mat, size_t m,
 constexpr int FFT128_MAXJ = 16;  // jobs (64-thread column chunks) per workgroup
 // Multiply by the table at t (32 nibble positions x 16 entries, position k
 // of w at entries 16k..16k+15; the lo word holds positions 0..15).
+// Each step folds its two entries into the sums with one three-input XOR
+// per word (v_bitop3): 64 XOR instructions per product, not 128.  Entry
+// byte offsets: per 32-bit word of w, (w << 4) & 0xF0F0F0F0 holds 16 times
+// the even nibbles in its bytes and w & 0xF0F0F0F0 the odd ones, so each
+// lookup's lane offset is one byte extract; the table and nibble-position
+// offsets are constants the compiler folds into the reads' immediates.
+__device__ __forceinline__ uint32_t byte_of(uint32_t q, int j) { return (q >> (8 * j)) & 0xFFu; }
 __device__ __forceinline__ u128v mul_tab128_lds(const uint4* __restrict__ t, u128v w) {
+    const char* tb = reinterpret_cast<const char*>(t);
+    const uint32_t wd[4] = {(uint32_t)w.lo, (uint32_t)(w.lo >> 32), (uint32_t)w.hi, (uint32_t)(w.hi >> 32)};
     uint4 acc = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        const uint4 a = t[16 * k + (int)((w.lo >> (4 * k)) & 15u)];
-        const uint4 b = t[256 + 16 * k + (int)((w.hi >> (4 * k)) & 15u)];
-        acc.x ^= a.x ^ b.x;
-        acc.y ^= a.y ^ b.y;
-        acc.z ^= a.z ^ b.z;
-        acc.w ^= a.w ^ b.w;
+    for (int i = 0; i < 4; i++) {
+        // nibble positions 8i .. 8i+7 (entries at 4096 * (i >> 1) + 256 * (8 * (i & 1) + nibble index))
+        const uint32_t qa = (wd[i] << 4) & 0xF0F0F0F0u, qb = wd[i] & 0xF0F0F0F0u;
+        const int base = 4096 * (i >> 1) + 2048 * (i & 1);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint4 a = *reinterpret_cast<const uint4*>(tb + base + 256 * (2 * j) + byte_of(qa, j));
+            const uint4 b = *reinterpret_cast<const uint4*>(tb + base + 256 * (2 * j + 1) + byte_of(qb, j));
+            acc.x = __builtin_amdgcn_bitop3_b32(acc.x, a.x, b.x, 0x96);
+            acc.y = __builtin_amdgcn_bitop3_b32(acc.y, a.y, b.y, 0x96);
+            acc.z = __builtin_amdgcn_bitop3_b32(acc.z, a.z, b.z, 0x96);
+            acc.w = __builtin_amdgcn_bitop3_b32(acc.w, a.w, b.w, 0x96);
+        }
     }
     return {(uint64_t)acc.x | ((uint64_t)acc.y << 32), (uint64_t)acc.z | ((uint64_t)acc.w << 32)};
 }
@@ -539,26 +555,79 @@ __device__ __forceinline__ void gf128_tab_build(uint4* t, u128v lam, int k) {
     }
 }
 
-// GF(2^128) transpose-and-scale: tiles of TR message rows x (64 * blockDim/64)
-// columns.  A thread reads TR consecutive elements of its column (one
-// stretch of the column-major input), the workgroup builds one multiply
-// table per row from pis in LDS, and each wave writes rows of 64 consecutive
-// columns, every lane multiplying by its row's table (wave-uniform, so the
-// reads of a nibble position hit 16 entries).
+// A whole multiply-by-lam table built by one wave: lane pair (2k, 2k+1)
+// holds nibble position k, lane 2k + h writes entries 8h..8h+7.  Called
+// with a wave-uniform lam (a scalar load), so the build waits for no vector
+// load in flight.
+__device__ __forceinline__ void gf128_tab_build_wave(uint4* t, u128v lam, int lane) {
+    const int k = lane >> 1, h = lane & 1;
+    // c[0] = lam * x^(4k) as k steps of a multiply by x^4 (the four bits
+    // shifted out fold back as t * (x^7 + x^2 + x + 1)): few registers live
+    // beside the caller's in-flight loads
+    u128v c[4];
+    c[0] = lam;
+    for (int i = 0; i < k; i++) {
+        const uint64_t top = c[0].hi >> 60;
+        c[0].hi = (c[0].hi << 4) | (c[0].lo >> 60);
+        c[0].lo = (c[0].lo << 4) ^ (top << 7) ^ (top << 2) ^ (top << 1) ^ top;
+    }
+#pragma unroll
+    for (int b = 1; b < 4; b++) {
+        const uint64_t top = c[b - 1].hi >> 63;
+        c[b] = {(c[b - 1].lo << 1) ^ ((0ull - top) & GF128_POLY), (c[b - 1].hi << 1) | (c[b - 1].lo >> 63)};
+    }
+    // entries 8h + v: c[3] when h, then the XOR of c[0..2] over the bits of
+    // v, written in Gray-code order from one running value (few registers
+    // live beside the caller's in-flight loads)
+    u128v e = h ? c[3] : u128v{0ull, 0ull};
+    uint4* tdst = t + (k < 16 ? 16 * k : 256 + 16 * (k - 16)) + 8 * h;
+    tdst[0] = u128_to4(e);
+    e = e ^ c[0];
+    tdst[1] = u128_to4(e);
+    e = e ^ c[1];
+    tdst[3] = u128_to4(e);
+    e = e ^ c[0];
+    tdst[2] = u128_to4(e);
+    e = e ^ c[2];
+    tdst[6] = u128_to4(e);
+    e = e ^ c[0];
+    tdst[7] = u128_to4(e);
+    e = e ^ c[1];
+    tdst[5] = u128_to4(e);
+    e = e ^ c[0];
+    tdst[4] = u128_to4(e);
+}
+
+// GF(2^128) transpose-and-scale: tiles of TR message rows x blockDim
+// columns.  A thread reads TR consecutive elements of its column (128 bytes,
+// one whole cache line of the column-major input; with half a line per
+// thread every line was fetched twice), the workgroup builds one multiply
+// table per row from pis in LDS, TB rows at a time, and each wave writes rows
+// of 64 consecutive columns, every lane multiplying by its row's table
+// (wave-uniform, so the reads of a nibble position hit 16 entries).
 __global__ __launch_bounds__(256) void k_transpose_scale128(const u128v* __restrict__ poly, size_t m, size_t n,
                                                             const u128v* __restrict__ pis, u128v* __restrict__ msg) {
-    constexpr int TR = TS128_ROWS;
-    __shared__ __attribute__((aligned(16))) uint4 tabs[TR * 512];
+    constexpr int TR = TS128_ROWS, TB = TS128_TAB_ROWS;
+    static_assert(TR % TB == 0, "table rows divide the tile");
+    __shared__ __attribute__((aligned(16))) uint4 tabs[TB * 512];
     const size_t cw = blockDim.x;
     const size_t row_tiles = m / TR;
     const size_t i0 = (blockIdx.x % row_tiles) * TR, c = (blockIdx.x / row_tiles) * cw + threadIdx.x;
     u128v v[TR];
 #pragma unroll
     for (int r = 0; r < TR; r++) v[r] = poly[c * m + i0 + r];
-    for (int idx = threadIdx.x; idx < TR * 32; idx += blockDim.x) gf128_tab_build(tabs + (idx >> 5) * 512, pis[i0 + (idx >> 5)], idx & 31);
-    __syncthreads();
+    // the column reads stay in flight while a wave builds whole row tables
+    // from a scalar load of the row's pis entry
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
+    const int lane = (int)threadIdx.x & 63;
 #pragma unroll
-    for (int r = 0; r < TR; r++) msg[(i0 + r) * n + c] = mul_tab128_lds(tabs + r * 512, v[r]);
+    for (int h = 0; h < TR / TB; h++) {
+        if (h > 0) __syncthreads();  // the previous rows' tables are read
+        for (int r = wave; r < TB; r += nw) gf128_tab_build_wave(tabs + r * 512, pis[i0 + h * TB + r], lane);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < TB; r++) msg[(i0 + h * TB + r) * n + c] = mul_tab128_lds(tabs + r * 512, v[h * TB + r]);
+    }
 }
 
 // One launch = K stages (d0..d0+K-1) of every quarter's size-m FFT.  Each
@@ -567,8 +636,10 @@ __global__ __launch_bounds__(256) void k_transpose_scale128(const u128v* __restr
 // (column chunks / tiles) that share B, hence the 2^K - 1 twiddle tables it
 // builds in LDS from the plain twiddles (entry v of position k = lambda *
 // v * x^(4k), from c_b = lambda * x^(4k+b)).
+// Two 1024-thread workgroups per CU (their 56 KB of tables fit twice in
+// LDS) need eight waves per SIMD, so at most 64 registers.
 template <int K>
-__global__ __launch_bounds__(1024) void k_fft_pass128(const u128v* __restrict__ src, u128v* __restrict__ dst, size_t m,
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_fft_pass128(const u128v* __restrict__ src, u128v* __restrict__ dst, size_t m,
                                                       size_t n, int ell, int d0, int q_arg, int src_shared,
                                                       const u128v* __restrict__ tw) {
     constexpr int R = 1 << K, NT = R - 1;
@@ -582,25 +653,34 @@ __global__ __launch_bounds__(1024) void k_fft_pass128(const u128v* __restrict__ 
     const int J = (int)(blockDim.x >> 6), sub = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
     const size_t job0 = (size_t)blockIdx.x * J;
     const size_t B = (job0 >> lgnc) >> sh;
-    for (int idx = threadIdx.x; idx < NT * 32; idx += blockDim.x) {
-        const int tt = idx >> 5, k = idx & 31;
-        const int j = 31 - __clz(tt + 1);
-        const size_t tp = (size_t)(tt + 1) - ((size_t)1 << j);
-        const int d = d0 + j;
-        const size_t first = ((size_t)1 << (d + 2)) - 1;
-        gf128_tab_build(tabs + tt * 512, tw[first + (q << d) + (B << j) + tp], k);
-    }
-    __syncthreads();
     const size_t job = job0 + sub;
     const size_t tile_id = job >> lgnc;
     const size_t c0 = (job & (nchunks - 1)) << 6;
     const size_t lo = tile_id & (S - 1);
     const size_t rbase = (B << (ell - d0)) + lo;
-    if (lane >= cc) return;
+    const bool active = lane < cc;
+    // twiddle of table tt (layer d0+j, block B*2^j + t')
+    auto lam_of = [&](int tt) -> u128v {
+        const int j = 31 - __clz(tt + 1);
+        const size_t tp = (size_t)(tt + 1) - ((size_t)1 << j);
+        const int d = d0 + j;
+        const size_t first = ((size_t)1 << (d + 2)) - 1;
+        return tw[first + (q << d) + (B << j) + tp];
+    };
+    // The tile's loads are issued first and stay in flight while the
+    // tables are built: a wave builds whole tables from a scalar load of
+    // their twiddle.  The loads are unconditional (a conditional load makes
+    // the compiler wait for every load in flight): lanes past the matrix's
+    // columns re-read column c0.
     const size_t qoff = src_shared ? 0 : q * m;
+    const size_t col = c0 + (active ? (size_t)lane : 0);
     u128v v[R];
 #pragma unroll
-    for (int a = 0; a < R; a++) v[a] = src[(qoff + rbase + a * S) * n + c0 + lane];
+    for (int a = 0; a < R; a++) v[a] = src[(qoff + rbase + a * S) * n + col];
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    for (int tt = wave; tt < NT; tt += J) gf128_tab_build_wave(tabs + tt * 512, lam_of(tt), lane);
+    __syncthreads();
+    if (!active) return;
 #pragma unroll
     for (int j = 0; j < K; j++) {
         const int h = R >> (j + 1);
@@ -614,7 +694,7 @@ __global__ __launch_bounds__(1024) void k_fft_pass128(const u128v* __restrict__ 
         }
     }
 #pragma unroll
-    for (int a = 0; a < R; a++) dst[(q * m + rbase + a * S) * n + c0 + lane] = v[a];
+    for (int a = 0; a < R; a++) dst[(q * m + rbase + a * S) * n + col] = v[a];
 }
 
 static void fft_quarters128(const u128v* msg, u128v* mat, size_t m, size_t n, const u128v* tw, hipStream_t s,
