@@ -1642,30 +1642,49 @@ __global__ void k_fold(const typename F::T* __restrict__ poly, size_t m, int kk,
     }
 }
 
-// One output row per thread.  Its 2^kk column values are loaded 16 at a
-// time with buffer loads (lane offset in one register, column offset in a
-// scalar register), the next 16 issued before the current 16 are
-// multiplied; the first 16 are in flight while the workgroup builds its
-// 2^kk multiply-by-weight tables in LDS from the weights (a kernel
-// argument, copied to LDS by one lane).
+// R consecutive output rows per thread (R = 2: one 8-byte load per column,
+// 512 bytes per wave and load instruction).  The thread's 2^kk column values
+// are loaded 16 / R columns at a time with buffer loads (lane offset in one
+// register, column offset in a scalar register), the next columns issued
+// before the current ones are multiplied; the first are in flight while the
+// workgroup builds its 2^kk multiply-by-weight tables in LDS from the
+// weights (a kernel argument, copied to LDS by one lane).
 struct FoldWeights32 {
     uint32_t w[1 << FOLD32_MAXK];
 };
+template <int R>
 __global__ __launch_bounds__(256) void k_fold32(const uint32_t* __restrict__ poly, size_t m, int kk, FoldWeights32 gr,
                                                  uint64_t* __restrict__ zero, int n_zero,
                                                  uint32_t* __restrict__ out) {
-    constexpr int NC = 1 << FOLD32_MAXK, BATCH = 16;
+    static_assert(R == 1 || R == 2, "one or two rows per thread");
+    constexpr int NC = 1 << FOLD32_MAXK, BATCH = 16 / R;
     __shared__ __attribute__((aligned(256))) uint32_t tabs[NC * 128];
     __shared__ uint32_t wsh[NC];
     if (blockIdx.x == 0 && (int)threadIdx.x < n_zero) zero[threadIdx.x] = 0ull;
     const int nc = 1 << kk;
     const uint32_t col_bytes = (uint32_t)(m * 4);
     const __amdgpu_buffer_rsrc_t rsrc = buf_rsrc(poly, col_bytes * (uint32_t)nc);
-    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * R;
     const uint32_t roff = (uint32_t)(i * 4);  // rows past m read other columns or 0; never stored
-    uint32_t v[BATCH];
+    auto ld = [&](int col, uint32_t (&dst)[R]) {
+        if constexpr (R == 1) {
+            dst[0] = buf_ld(rsrc, roff, (uint32_t)col * col_bytes);
+        } else {
+            const auto x = __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)roff, (int)((uint32_t)col * col_bytes), 0);
+            dst[0] = x[0];
+            dst[1] = x[1];
+        }
+    };
+    uint32_t v[BATCH][R];
 #pragma unroll
-    for (int t = 0; t < BATCH; t++) v[t] = t < nc ? buf_ld(rsrc, roff, (uint32_t)t * col_bytes) : 0u;
+    for (int t = 0; t < BATCH; t++) {
+        if (t < nc) {
+            ld(t, v[t]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[t][r] = 0u;
+        }
+    }
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int j = 0; j < NC; j++) wsh[j] = gr.w[j];
@@ -1673,21 +1692,36 @@ __global__ __launch_bounds__(256) void k_fold32(const uint32_t* __restrict__ pol
     LIG_LDS_BARRIER();
     for (int idx = threadIdx.x; idx < nc * 8; idx += blockDim.x) gf32_tab_build<true>(tabs + (idx >> 3) * 128, wsh[idx >> 3], idx & 7);
     LIG_LDS_BARRIER();
-    uint32_t acc = 0;
-    for (int j0 = 0; j0 < nc; j0 += BATCH) {
-        uint32_t w[BATCH];
+    uint32_t acc[R];
 #pragma unroll
-        for (int t = 0; t < BATCH; t++) w[t] = v[t];
+    for (int r = 0; r < R; r++) acc[r] = 0u;
+    for (int j0 = 0; j0 < nc; j0 += BATCH) {
+        uint32_t w[BATCH][R];
+#pragma unroll
+        for (int t = 0; t < BATCH; t++)
+#pragma unroll
+            for (int r = 0; r < R; r++) w[t][r] = v[t][r];
         if (j0 + BATCH < nc) {
 #pragma unroll
-            for (int t = 0; t < BATCH; t++)
-                v[t] = j0 + BATCH + t < nc ? buf_ld(rsrc, roff, (uint32_t)(j0 + BATCH + t) * col_bytes) : 0u;
+            for (int t = 0; t < BATCH; t++) {
+                if (j0 + BATCH + t < nc) {
+                    ld(j0 + BATCH + t, v[t]);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; r++) v[t][r] = 0u;
+                }
+            }
         }
 #pragma unroll
         for (int t = 0; t < BATCH; t++)
-            if (j0 + t < nc) acc ^= mul_tab_lds(tabs, (uint32_t)((j0 + t) * 512), w[t]);
+            if (j0 + t < nc) {
+#pragma unroll
+                for (int r = 0; r < R; r++) acc[r] ^= mul_tab_lds(tabs, (uint32_t)((j0 + t) * 512), w[t][r]);
+            }
     }
-    if (i < m) out[i] = acc;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        if (i + r < m) out[i + r] = acc[r];
 }
 
 bool launch_fold32_weights(const uint32_t* poly, size_t m, int kk, const uint32_t* gr_host, uint32_t* out, hipStream_t s,
@@ -1696,7 +1730,12 @@ bool launch_fold32_weights(const uint32_t* poly, size_t m, int kk, const uint32_
     if (n_zero < 0 || n_zero > 256 || (n_zero && !zero)) throw std::invalid_argument("launch_fold32_weights: zero range");
     FoldWeights32 gr{};
     for (int j = 0; j < (1 << kk); j++) gr.w[j] = gr_host[j];
-    hipLaunchKernelGGL(k_fold32, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, poly, m, kk, gr, zero, n_zero, out);
+    // two rows per thread when m is even (a thread's pair never straddles m)
+    // and the grid still holds a workgroup per CU or more
+    if (m % 2 == 0 && m >= ((size_t)1 << 17))
+        hipLaunchKernelGGL(k_fold32<2>, dim3((unsigned)((m + 511) / 512)), dim3(256), 0, s, poly, m, kk, gr, zero, n_zero, out);
+    else
+        hipLaunchKernelGGL(k_fold32<1>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, poly, m, kk, gr, zero, n_zero, out);
     return true;
 }
 
