@@ -1549,9 +1549,16 @@ __global__ __launch_bounds__(256) void k_merkle_block3(uint32_t* __restrict__ tr
 // once a level has fewer than MERKLE_WIDE parents, k_merkle_subtree4
 // finishes the top with four lanes per parent (latency-bound).
 constexpr size_t MERKLE_WIDE = (size_t)1 << 17;
-// The top (at most 2^SUB4_TOP nodes) in one 1024-thread workgroup; below
-// it, 256-thread workgroups of depth SUB4_G.
-constexpr int SUB4_G = 7, SUB4_TOP = 10;
+// Those levels go in as few launches as the workgroup depth allows (at most
+// SUB4_MAXG levels: two layers of 2^g nodes in LDS), of equal depth, the
+// last one a single workgroup; a workgroup of depth g has 2^(g-1) quads
+// (one per parent of its first level), at most 1024 threads.  Every level
+// costs one compression latency (about 0.75 us with a quad per parent), so
+// the first launch's workgroups should not be deeper than the second:
+// measured on MI355X for 2^17 nodes, depths (9, 8) 22.2 us against (7, 10)
+// 28.7 us; 2^16 nodes (8, 8) 19.6 against (7, 9) 21.7; 2^12 nodes (6, 6)
+// 14.2 against (7, 5) 14.5.
+constexpr int SUB4_MAXG = 10;
 
 void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out, uint32_t* flag, uint32_t seq,
                           int trees, size_t tree_stride) {
@@ -1570,14 +1577,19 @@ void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32
         in_off += nodes;
         nodes = n_out;
     }
-    while (nodes > 1) {
-        int lg = 0;
-        while (((size_t)1 << lg) < nodes) lg++;
-        const bool top = lg <= SUB4_TOP;
-        const int g = top ? lg : SUB4_G;
+    int lg = 0;
+    while (((size_t)1 << lg) < nodes) lg++;
+    const int launches = lg < 4 ? 1 : (lg + SUB4_MAXG - 1) / SUB4_MAXG < 2 ? 2 : (lg + SUB4_MAXG - 1) / SUB4_MAXG;
+    for (int li = 0; nodes > 1; li++) {
+        const int left = launches - li;  // launches still to make, this one included
+        int lgn = 0;
+        while (((size_t)1 << lgn) < nodes) lgn++;
+        const int g = (lgn + left - 1) / left;  // deeper launches first
+        const bool top = g == lgn;
         const size_t groups = nodes >> g;
         const size_t lds = (size_t)64 << g;  // two layers of 2^g nodes
-        hipLaunchKernelGGL(k_merkle_subtree4, dim3((unsigned)groups, 1, z), dim3(top ? 1024 : 256), lds, s, tree, in_off, nodes, g,
+        const unsigned thr = g >= 9 ? 1024u : g >= 5 ? (2u << g) : 64u;
+        hipLaunchKernelGGL(k_merkle_subtree4, dim3((unsigned)groups, 1, z), dim3(thr), lds, s, tree, in_off, nodes, g,
                            root_out, top ? flag : nullptr, seq, tree_stride);
         for (int l = 0; l < g; l++) {
             in_off += nodes;
