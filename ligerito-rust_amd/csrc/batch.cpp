@@ -48,21 +48,19 @@ template <class H> Config<H>& Worker<H>::config(int log_n) {
     return ref;
 }
 
-template <class H>
-void Worker<H>::prove_one(int log_n, const void* poly, bool on_device, uint32_t seed, Proof<H>& out, Witness<H>* first) {
+template <class H> void Worker<H>::prove_one(int log_n, const void* poly, bool on_device, uint32_t seed, Proof<H>& out) {
     using T = typename H::T;
     LIG_HIP(hipSetDevice(ctx.device));
     Config<H>& cfg = config(log_n);
     const size_t n = (size_t)1 << log_n;
     PBuf<T> tmp;
     const T* d = (const T*)poly;
-    LIG_REQUIRE(on_device || !first, LIG_E_INVALID, "a given first commit needs a device input");
     if (!on_device) {
         tmp = PBuf<T>(&ctx.pool, n);
         LIG_HIP(hipMemcpyAsync(tmp.p, poly, n * sizeof(T), hipMemcpyHostToDevice, ctx.stream));
         d = tmp.p;
     }
-    prove<H>(cfg, d, seed, out, first);
+    prove<H>(cfg, d, seed, out);
 }
 
 template <class H> size_t Worker<H>::retained() const {
@@ -96,24 +94,54 @@ template <class H> Batch<H>::Batch(int dev, int n_streams) : device(dev) {
     for (int i = 0; i < n_streams; i++) workers.emplace_back(new Worker<H>(dev));
 }
 
-// Grouped first commits: sizes up to 2^GROUP_MAX_LOG coefficients, groups
-// of up to 2^GROUP_MAX_ELEMS_LOG input elements (config D, 12 workers:
-// 2^16 jobs 49.0 -> 39.5 us each, 2^18 49.8 -> 42.9, 2^20 unchanged, 2^22
-// 189 -> 202 in groups of 64).
-constexpr int GROUP_MAX_LOG = 19, GROUP_MAX_ELEMS_LOG = 24;
+// Gangs: same-size GF(2^32) jobs whose configuration has one recursive step
+// (2^23 coefficients or fewer) are proved in lockstep by one worker
+// (prove_gang), every kernel launched once for the gang; at most
+// GANG_BATCH_MAX jobs and 2^GANG_MAX_ELEMS_LOG input elements per gang.
+// Measured on MI355X, 36 jobs of one size on 12 workers: 2^16 jobs 47.5 ->
+// 24.5 us each, 2^19 60.3 -> 40.4, 2^20 81.2 -> 61.8, 2^22 unchanged (190
+// us: bound by the encode); config D 21.0 -> 16.4 ms.  Gangs of 8 or 16
+// jobs, or of 2^23 / 2^25 / 2^26 elements, measured the same within noise:
+// a gang's host steps (queries, node lists) run on its one worker thread,
+// so larger gangs trade fewer launches for longer host phases.
+constexpr int GANG_BATCH_MAX = 4, GANG_MAX_ELEMS_LOG = 24;
 
-static bool group_commits_enabled() {
+static bool gangs_enabled() {
     static const bool on = [] {
-        const char* e = getenv("LIG_BATCH_GROUP");
+        const char* e = getenv("LIG_BATCH_GANG");
         return !(e && e[0] == '0');
     }();
     return on;
 }
 
 template <class H>
+void Worker<H>::prove_gang_jobs(int log_n, const void* const* polys, bool on_device, const uint32_t* seeds, int count,
+                                Proof<H>* out) {
+    using T = typename H::T;
+    if constexpr (std::is_same<H, H32>::value) {
+        LIG_HIP(hipSetDevice(ctx.device));
+        Config<H>& cfg = config(log_n);
+        const size_t n = (size_t)1 << log_n;
+        PBuf<T> tmp;
+        std::vector<const T*> d(count);
+        if (!on_device) {
+            tmp = PBuf<T>(&ctx.pool, n * (size_t)count);
+            for (int j = 0; j < count; j++) {
+                LIG_HIP(hipMemcpyAsync(tmp.p + j * n, polys[j], n * sizeof(T), hipMemcpyHostToDevice, ctx.stream));
+                d[j] = tmp.p + j * n;
+            }
+        } else {
+            for (int j = 0; j < count; j++) d[j] = (const T*)polys[j];
+        }
+        prove_gang<H>(cfg, d.data(), seeds, count, out);
+    } else {
+        throw Error(LIG_E_INVALID, "gang proves are GF(2^32)");
+    }
+}
+
+template <class H>
 void Batch<H>::run(size_t n_jobs, const int* log_n, const void* const* polys, bool on_device, const uint32_t* seeds,
                    std::vector<Proof<H>>& out) {
-    using T = typename H::T;
     for (size_t i = 0; i < n_jobs; i++) {
         LIG_REQUIRE(polys[i], LIG_E_INVALID, "NULL polynomial");
         LIG_REQUIRE(log_n[i] >= 16 && log_n[i] <= 27, LIG_E_INVALID, "log_n must be in [16, 27]");
@@ -124,49 +152,28 @@ void Batch<H>::run(size_t n_jobs, const int* log_n, const void* const* polys, bo
     for (size_t i = 0; i < n_jobs; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return log_n[a] > log_n[b]; });
 
-    // GF(2^32) jobs of a size that occurs more than once get their first
-    // commits made together (commit_group), largest size first, on the
-    // grouper's stream; their proves then start from those witnesses.  The
-    // other jobs are ready at once.  Workers take ready jobs in order.
-    struct Job {
-        Witness<H> first;
-        std::shared_ptr<void> keep;  // the group's buffers (and uploaded inputs)
-        const T* d_poly = nullptr;   // the device input the prove folds
-        bool grouped = false;
-    };
-    std::vector<Job> jobs(n_jobs);
-    std::vector<std::vector<size_t>> groups;  // job indices per size, largest first
-    if constexpr (std::is_same<H, H32>::value) {
-        if (group_commits_enabled()) {
-            for (size_t k = 0; k < n_jobs;) {
-                size_t e = k;
-                while (e < n_jobs && log_n[order[e]] == log_n[order[k]]) e++;
-                // small proves only (2^19 coefficients or fewer: their first
-                // commits are a few short launches each, which a group turns
-                // into one launch per kernel; larger ones gained nothing and
-                // made the workers wait for the group), at most 2^24 input
-                // elements per group
-                const int L = log_n[order[k]];
-                const size_t cap = L > GROUP_MAX_LOG ? 1 : ((size_t)1 << GROUP_MAX_ELEMS_LOG) >> L;
-                for (size_t c = k; c < e; c += cap) {
-                    const size_t ce = std::min(e, c + cap);
-                    if (ce - c >= 2) groups.emplace_back(order.begin() + (long)c, order.begin() + (long)ce);
-                }
-                k = e;
-            }
+    // work items in that order: a gang of same-size jobs or a single job
+    std::vector<std::vector<size_t>> items;
+    for (size_t k = 0; k < n_jobs;) {
+        size_t e = k;
+        while (e < n_jobs && log_n[order[e]] == log_n[order[k]]) e++;
+        const int L = log_n[order[k]];
+        size_t cap = 1;
+        if (std::is_same<H, H32>::value && gangs_enabled() && Shape::generated(L).recursive_steps == 1) {
+            cap = L >= GANG_MAX_ELEMS_LOG ? 1 : ((size_t)1 << GANG_MAX_ELEMS_LOG) >> L;
+            cap = std::min<size_t>(cap, GANG_BATCH_MAX);
         }
+        // gangs of about equal size
+        const size_t n_same = e - k, n_items = (n_same + cap - 1) / cap;
+        for (size_t g = 0, at = k; g < n_items; g++) {
+            const size_t sz = n_same / n_items + (g < n_same % n_items ? 1 : 0);
+            items.emplace_back(order.begin() + (long)at, order.begin() + (long)(at + sz));
+            at += sz;
+        }
+        k = e;
     }
-    std::mutex q_mu;
-    std::condition_variable q_cv;
-    std::deque<size_t> ready;
-    size_t groups_left = groups.size();
-    {
-        std::vector<char> in_group(n_jobs, 0);
-        for (const auto& g : groups)
-            for (size_t j : g) in_group[j] = 1;
-        for (size_t j : order)
-            if (!in_group[j]) ready.push_back(j);
-    }
+
+    std::atomic<size_t> next{0};
     std::atomic<bool> failed{false};
     std::mutex err_mu;
     int err_code = LIG_OK;
@@ -178,9 +185,27 @@ void Batch<H>::run(size_t n_jobs, const int* log_n, const void* const* polys, bo
             err_msg = msg;
         }
     };
-    auto guarded_run = [&](const std::function<void()>& fn) {
+    auto body = [&](Worker<H>* w) {
         try {
-            fn();
+            for (;;) {
+                if (failed.load()) return;
+                const size_t it = next.fetch_add(1);
+                if (it >= items.size()) return;
+                const std::vector<size_t>& g = items[it];
+                if (g.size() == 1) {
+                    w->prove_one(log_n[g[0]], polys[g[0]], on_device, seeds[g[0]], out[g[0]]);
+                    continue;
+                }
+                std::vector<const void*> ps(g.size());
+                std::vector<uint32_t> ss(g.size());
+                std::vector<Proof<H>> pr(g.size());
+                for (size_t i = 0; i < g.size(); i++) {
+                    ps[i] = polys[g[i]];
+                    ss[i] = seeds[g[i]];
+                }
+                w->prove_gang_jobs(log_n[g[0]], ps.data(), on_device, ss.data(), (int)g.size(), pr.data());
+                for (size_t i = 0; i < g.size(); i++) out[g[i]] = std::move(pr[i]);
+            }
         } catch (const Error& e) {
             fail(e.code, e.what());
         } catch (const std::bad_alloc&) {
@@ -189,92 +214,8 @@ void Batch<H>::run(size_t n_jobs, const int* log_n, const void* const* polys, bo
             fail(LIG_E_ASSERT, e.what());
         }
     };
-    auto grouper_body = [&]() {
-        for (const auto& g : groups) {
-            if (failed.load()) break;
-            bool made = false;
-            try {
-                if constexpr (std::is_same<H, H32>::value) {
-                    Worker<H>& gw = *grouper;
-                    LIG_HIP(hipSetDevice(gw.ctx.device));
-                    const int L = log_n[g[0]];
-                    Config<H>& cfg = gw.config(L);
-                    const size_t N = (size_t)1 << L;
-                    // host inputs are uploaded into one buffer the group keeps
-                    struct Inputs {
-                        PBuf<T> buf;
-                    };
-                    std::shared_ptr<Inputs> in;
-                    std::vector<const T*> ptrs(g.size());
-                    if (!on_device) {
-                        in = std::make_shared<Inputs>();
-                        in->buf = PBuf<T>(&gw.ctx.pool, g.size() * N);
-                        for (size_t i = 0; i < g.size(); i++) {
-                            ptrs[i] = in->buf.p + i * N;
-                            LIG_HIP(hipMemcpyAsync(in->buf.p + i * N, polys[g[i]], N * sizeof(T), hipMemcpyHostToDevice,
-                                                   gw.ctx.stream));
-                        }
-                    } else {
-                        for (size_t i = 0; i < g.size(); i++) ptrs[i] = (const T*)polys[g[i]];
-                    }
-                    std::vector<Witness<H>> ws;
-                    std::shared_ptr<void> keep;
-                    commit_group<H>(*cfg.initial_rs, ptrs.data(), (int)g.size(), cfg.initial_m, cfg.initial_n, ws, keep);
-                    std::shared_ptr<void> both = keep;
-                    if (in) both = std::make_shared<std::pair<std::shared_ptr<void>, std::shared_ptr<Inputs>>>(keep, in);
-                    std::lock_guard<std::mutex> lk(q_mu);
-                    for (size_t i = 0; i < g.size(); i++) {
-                        Job& jb = jobs[g[i]];
-                        jb.first = std::move(ws[i]);
-                        jb.keep = both;
-                        jb.d_poly = ptrs[i];
-                        jb.grouped = true;
-                        ready.push_back(g[i]);
-                    }
-                    groups_left--;
-                    made = true;
-                }
-            } catch (const std::exception&) {
-                // the group's jobs make their own first commits instead
-            }
-            if (!made) {
-                std::lock_guard<std::mutex> lk(q_mu);
-                for (size_t j : g) ready.push_back(j);
-                groups_left--;
-            }
-            q_cv.notify_all();
-        }
-        std::lock_guard<std::mutex> lk(q_mu);
-        groups_left = 0;
-        q_cv.notify_all();
-    };
-    auto body = [&](Worker<H>* w) {
-        guarded_run([&] {
-            for (;;) {
-                size_t j;
-                {
-                    std::unique_lock<std::mutex> lk(q_mu);
-                    q_cv.wait(lk, [&] { return failed.load() || !ready.empty() || groups_left == 0; });
-                    if (failed.load() || ready.empty()) return;
-                    j = ready.front();
-                    ready.pop_front();
-                }
-                Job& jb = jobs[j];
-                if (jb.grouped) {
-                    // the group's buffers stay held until this prove has returned
-                    std::shared_ptr<void> hold = std::move(jb.keep);
-                    w->prove_one(log_n[j], jb.d_poly, true, seeds[j], out[j], &jb.first);
-                } else {
-                    w->prove_one(log_n[j], polys[j], on_device, seeds[j], out[j]);
-                }
-            }
-        });
-        q_cv.notify_all();
-    };
-    if (!groups.empty() && !grouper) grouper.reset(new Worker<H>(device));
-    const size_t nw = std::min(workers.size(), std::max<size_t>(n_jobs, 1));
+    const size_t nw = std::min(workers.size(), std::max<size_t>(items.size(), 1));
     std::vector<std::thread> threads;
-    if (!groups.empty()) threads.emplace_back([&] { guarded_run(grouper_body); });
     for (size_t i = 1; i < nw; i++) threads.emplace_back(body, workers[i].get());
     body(workers[0].get());
     for (auto& t : threads) t.join();

@@ -21,9 +21,11 @@ template <class H> struct Worker {
     explicit Worker(int dev);
     ~Worker();
     Config<H>& config(int log_n);
-    // poly: 2^log_n elements in host memory, or device memory if on_device;
-    // first: its first commit, already made (device input only)
-    void prove_one(int log_n, const void* poly, bool on_device, uint32_t seed, Proof<H>& out, Witness<H>* first = nullptr);
+    // poly: 2^log_n elements in host memory, or device memory if on_device
+    void prove_one(int log_n, const void* poly, bool on_device, uint32_t seed, Proof<H>& out);
+    // count proves of 2^log_n elements each in lockstep (prove_gang;
+    // GF(2^32), one recursive step)
+    void prove_gang_jobs(int log_n, const void* const* polys, bool on_device, const uint32_t* seeds, int count, Proof<H>* out);
     // device bytes the worker holds between proves: its pool (cached
     // blocks) and the tables of its prover configurations
     size_t retained() const;
@@ -39,9 +41,6 @@ size_t prove_footprint(int field_bits, int log_n);
 template <class H> struct Batch {
     int device;
     std::vector<std::unique_ptr<Worker<H>>> workers;
-    // makes the first commits of same-size GF(2^32) jobs together (one launch
-    // per kernel for the whole group) on its own stream, beside the workers
-    std::unique_ptr<Worker<H>> grouper;
     Batch(int dev, int n_streams);
     void run(size_t n_jobs, const int* log_n, const void* const* polys, bool on_device, const uint32_t* seeds,
              std::vector<Proof<H>>& out);

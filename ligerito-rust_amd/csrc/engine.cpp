@@ -621,7 +621,7 @@ void induce_launch(Ctx* ctx, int ell, const Vec<H>& sks, const PendingOpen<H>& p
 }  // namespace
 
 template <class H>
-void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<H>& proof, Witness<H>* first) {
+void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<H>& proof) {
     using F = typename DevField<H>::F;
     using T = typename H::T;
     using A = typename F::Acc;
@@ -644,17 +644,7 @@ void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<
     const size_t final_M = 4 * cfg.dims_m.back(), final_mat_bytes = final_M * cfg.dims_n.back() * sizeof(T);
     ctx->pinned(PIN_FINAL + final_mat_bytes + (2 * final_M - 1) * 32);
     Witness<H> w0;
-    if (first) {
-        LIG_REQUIRE(first->m == cfg.initial_m && first->n == cfg.initial_n && first->M == 4 * cfg.initial_m && first->mat.p &&
-                        first->tree.p,
-                    LIG_E_INVALID, "prove: the given first commit has another shape");
-        w0 = std::move(*first);
-        // its openings run on this prove's stream, pool and pinned memory,
-        // not on those of the context that made the commit
-        w0.ctx = ctx;
-    } else {
-        commit<H>(*cfg.initial_rs, d_poly, cfg.initial_m, cfg.initial_n, w0, PH_ENCODE0);
-    }
+    commit<H>(*cfg.initial_rs, d_poly, cfg.initial_m, cfg.initial_n, w0, PH_ENCODE0);
     memcpy(proof.initial_root.data(), w0.root, 32);
     fs.absorb(w0.root, 32);
 
@@ -801,6 +791,7 @@ void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<
             if (j == 0) {
                 // the leading step: from the host's state, with the
                 // enforced-sum check, n0 challenges and (n0 == 2) the glue
+                step.lead = 1;
                 step.st_in = nullptr;
                 step.init = init;
                 step.enf = d_enf;
@@ -958,15 +949,317 @@ void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<
     throw Error(LIG_E_ASSERT, "Prover completed without entering final step (prover.rs:198)");
 }
 
+// ------------------------------------------------------------ gang prove --
+// `count` proves of one GF(2^32) configuration with one recursive step, in
+// lockstep: each kernel is launched once for all of them (job j =
+// blockIdx.y, or one table of pointers), and at each of the four
+// synchronisations the host runs every job's transcript steps.  A small
+// prove is a chain of short latency-bound kernels (transcript steps, Merkle
+// tops, small folds) that leave most of the GPU idle, and a device runs few
+// kernels at once (one per hardware queue); here a chain serves the whole
+// group.  The transcript schedule of each job is prove()'s (prover.rs:29-195),
+// so its proof bytes are the same.
+//
+// Pinned staging, from GANG_PIN (below it: the flag and commit_group's
+// region), per job at fixed strides: fold input pointer, fold weights,
+// query indices, induce constants, initial transcript state, section logs,
+// opened rows, yr; then the Merkle node indices and nodes of the first and
+// of the final openings, sized by the queries' worst case.
+template <class H>
+void prove_gang(Config<H>& cfg, const typename H::T* const* d_polys, const uint32_t* seeds, int count, Proof<H>* proofs) {
+    static_assert(std::is_same<H, H32>::value, "gang proves are GF(2^32)");
+    using F = F32;
+    using T = uint32_t;
+    using A = uint64_t;
+    LIG_REQUIRE(count >= 1 && count <= GANG_MAX, LIG_E_INVALID, "gang prove: 1..GANG_MAX proves");
+    LIG_REQUIRE(cfg.recursive_steps == 1, LIG_E_INVALID, "gang prove: configurations with one recursive step");
+    LIG_REQUIRE(cfg.initial_k >= 1 && cfg.initial_k <= FOLD32_MAXK && cfg.ks[0] >= 1 && cfg.ks[0] <= 8, LIG_E_INVALID,
+                "gang prove: fold widths");
+    for (int j = 0; j < count; j++) LIG_REQUIRE(d_polys[j], LIG_E_INVALID, "NULL polynomial");
+    Ctx* ctx = cfg.ctx;
+    hipStream_t s = ctx->stream;
+    const size_t S = 148;  // prover.rs:30
+    const size_t G = (size_t)count;
+    const size_t m0 = cfg.initial_m, n0 = cfg.initial_n, M0 = 4 * m0;
+    const size_t m1 = cfg.dims_m[0], n1 = cfg.dims_n[0], M1 = 4 * m1;
+    const int k0 = cfg.initial_k, k1 = cfg.ks[0];
+    const size_t flen0 = m0;  // N >> initial_k
+    const int ell = ilog2(flen0);
+    const size_t yr_len = flen0 >> k1;
+    const Vec<H>& sks = cfg.sks_by_ell.at(ell);
+    const size_t depth0 = (size_t)ilog2(M0), depth1 = (size_t)ilog2(M1);
+
+    // pinned layout (bytes)
+    constexpr size_t GANG_PIN = (size_t)64 << 10;
+    constexpr size_t CONST_W = 80;  // per job: 64 weights | alpha (induce), padded
+    constexpr size_t LOG_W = 64;    // per job: triples | challenges | check flag, raw last triple at LAST_W
+    constexpr size_t LAST_W = 40;
+    const size_t n_tlog = 3 * (size_t)(k1 + 1), n_r = (size_t)k1 + 2;
+    LIG_REQUIRE(n_tlog + n_r + 1 <= LAST_W && n0 <= 64, LIG_E_INVALID, "gang prove: section logs exceed their region");
+    size_t off = GANG_PIN;
+    auto region = [&](size_t bytes) {
+        const size_t at = off;
+        off = pin_round(off + bytes);
+        return at;
+    };
+    const size_t P_PTR = region(G * 8), P_W = region(G * 64 * 4), P_Q = region(G * S * 8),
+                 P_SKS = region((sks.size()) * 4), P_CONST = region(G * CONST_W * 4),
+                 P_INIT = region(G * 2 * sizeof(DevTranscript)), P_LOG = region(G * LOG_W * 4),
+                 P_ROWS = region(G * S * n0 * 4), P_YR = region(G * yr_len * 4), P_FQ = region(G * S * 8),
+                 P_NIDX0 = region(G * S * depth0 * 8), P_NODES0 = region(G * S * depth0 * 32),
+                 P_NIDX1 = region(G * S * depth1 * 8), P_NODES1 = region(G * S * depth1 * 32),
+                 P_FROWS = region(G * S * n1 * 4);
+    uint8_t* pin = ctx->pinned(off);
+    uint8_t* pdev = ctx->pin_dev;
+
+    // first commits of every job
+    std::vector<Witness<H>> w0, w1;
+    std::shared_ptr<void> keep0, keep1;
+    commit_group<H>(*cfg.initial_rs, d_polys, count, m0, n0, w0, keep0);
+
+    std::vector<Transcript> fs;
+    fs.reserve(G);
+    std::vector<Vec<H>> gr0(G);
+    T* h_w = (T*)(pin + P_W);
+    for (size_t j = 0; j < G; j++) {
+        proofs[j] = Proof<H>();
+        fs.emplace_back(seeds[j]);
+        memcpy(proofs[j].initial_root.data(), w0[j].root, 32);
+        fs[j].absorb(w0[j].root, 32);
+        Vec<H> pe0(k0);
+        for (auto& r : pe0) r = fs[j].get_field<H>();
+        gr0[j] = lagrange_basis<H>(pe0);
+        memcpy(h_w + j * 64, gr0[j].data(), gr0[j].size() * 4);
+        ((const T**)(pin + P_PTR))[j] = d_polys[j];
+    }
+
+    // accumulators per job: triples in rotation at 0, 3, 6, enforced-sum
+    // parts from 9
+    constexpr size_t ACC = 9 + ENFORCED_MAX_PARTS;
+    PBuf<A> d_acc(&ctx->pool, G * ACC);
+    size_t rc = 0, rn = 3, rf = 6;  // res_cur, res_next, res_free
+    auto rotate = [&]() {
+        const size_t t = rc;
+        rc = rn;
+        rn = rf;
+        rf = t;
+    };
+    // partial_eval of every job (its zeroing of res_next and res_free)
+    PBuf<T> d_f(&ctx->pool, G * flen0), d_f2(&ctx->pool, G * flen0 / 2);
+    {
+        FoldJobs fj;
+        fj.polys = (const T* const*)(pdev + P_PTR);
+        fj.w = (const T*)(pdev + P_W);
+        fj.out_stride = flen0;
+        fj.zero_stride = ACC;
+        launch_fold32_jobs(flen0, k0, fj, count, d_f.p, d_acc.p + rn, 6, s);
+        LIG_HIP(hipGetLastError());
+    }
+    // the recursive commit of every folded polynomial (synchronizes)
+    {
+        std::vector<const T*> fp(G);
+        for (size_t j = 0; j < G; j++) fp[j] = d_f.p + j * flen0;
+        commit_group<H>(*cfg.rs[0], fp.data(), count, m1, n1, w1, keep1);
+    }
+
+    // per job: root, queries, alpha; the openings' node lists
+    std::vector<std::vector<size_t>> q(G);
+    uint64_t* h_q = (uint64_t*)(pin + P_Q);
+    T* h_const = (T*)(pin + P_CONST);
+    memcpy(pin + P_SKS, sks.data(), sks.size() * 4);
+    uint64_t* h_nidx0 = (uint64_t*)(pin + P_NIDX0);
+    std::vector<size_t> nodes_at(G + 1, 0);
+    for (size_t j = 0; j < G; j++) {
+        proofs[j].rec_roots.emplace_back();
+        memcpy(proofs[j].rec_roots.back().data(), w1[j].root, 32);
+        fs[j].absorb(w1[j].root, 32);
+        q[j] = fs[j].get_distinct_queries(M0, S);
+        const T alpha = fs[j].get_field<H>();
+        LIG_REQUIRE(q[j].size() == S, LIG_E_ASSERT, "distinct queries");
+        for (size_t i = 0; i < S; i++) h_q[j * S + i] = q[j][i];
+        memcpy(h_const + j * CONST_W, gr0[j].data(), gr0[j].size() * 4);
+        h_const[j * CONST_W + 64] = alpha;
+        const std::vector<uint64_t> nd = merkle_proof_nodes(M0, q[j]);
+        LIG_REQUIRE(nd.size() <= S * depth0, LIG_E_ASSERT, "Merkle proof needs more nodes than queries x depth");
+        nodes_at[j + 1] = nodes_at[j] + nd.size();
+        for (size_t i = 0; i < nd.size(); i++) h_nidx0[nodes_at[j] + i] = nd[i] + j * (2 * M0 - 1);
+        // the section's leading step starts from this state
+        DevTranscript* init = (DevTranscript*)(pin + P_INIT) + 2 * j;
+        export_transcript(fs[j], *init);
+        ((uint32_t*)(pin + P_LOG))[j * LOG_W + n_tlog + n_r] = 0;  // check flag
+    }
+
+    // induce_sumcheck_poly of every job: opened rows, basis, enforced sum,
+    // and the 0/1/3 triple of (f, basis)
+    const int clo = induce_low_bits(ell);
+    PBuf<T> d_c(&ctx->pool, G * S * ell), d_ap(&ctx->pool, G * S);
+    PBuf<T> d_P(&ctx->pool, G * (S << clo)), d_K(&ctx->pool, G * (S << (ell - clo)));
+    PBuf<T> d_b(&ctx->pool, G * flen0), d_b2(&ctx->pool, G * flen0 / 2), d_rows(&ctx->pool, G * S * n0);
+    {
+        InduceJobs ij;
+        ij.q = S;
+        ij.alpha = CONST_W;
+        ij.mat = M0 * n0;
+        ij.rows2 = S * n0;
+        ij.gr = CONST_W;
+        ij.parts = ACC;
+        ij.f = flen0;
+        ij.res = ACC;
+        RowGather<F> g;
+        g.mat = w0[0].mat.p;  // the group's matrices, M0 x n0 apart
+        g.n = n0;
+        g.out = d_rows.p;
+        g.out2 = (T*)(pdev + P_ROWS);
+        const T* d_const = (const T*)(pdev + P_CONST);
+        launch_induce_prep<F>((const uint64_t*)(pdev + P_Q), (const T*)(pdev + P_SKS), d_const + 64, (int)S, ell, d_c.p, d_ap.p,
+                              s, &g, count, ij);
+        EnforcedIn<F> e;
+        e.rows = d_rows.p;
+        e.ncols = (int)n0;
+        e.gr = d_const;
+        e.parts = d_acc.p + 9;
+        Eval013In<F> ev;
+        ev.f = d_f.p;
+        ev.res = d_acc.p + rn;
+        ev.res_clear = d_acc.p + rf;
+        launch_induce<F>(d_c.p, d_ap.p, (int)S, ell, d_P.p, d_K.p, d_b.p, s, &e, &ev, count, ij);
+        LIG_HIP(hipGetLastError());
+        rotate();
+    }
+
+    // the sumcheck section of the step (prover.rs:96-102): k1 fold rounds,
+    // each running the preceding transcript step per job on the device
+    PBuf<DevTranscript> d_fs(&ctx->pool, 2 * G);
+    int fsi = 0;
+    size_t flen = flen0;
+    T* d_tlog = (T*)(pdev + P_LOG);
+    size_t sf = flen0, sf2 = flen0 / 2, sb = flen0, sb2 = flen0 / 2;
+    for (int r = 0; r < k1; r++) {
+        FoldStep<F> step{};
+        step.st_out = d_fs.p + (fsi ^ 1);
+        step.res_in = d_acc.p + rc;
+        if (r == 0) {
+            step.lead = 1;
+            step.st_in = (const DevTranscript*)(pdev + P_INIT);
+            step.enf = d_acc.p + 9;
+            step.n_enf = (int)S;
+            step.err = (uint32_t*)(d_tlog + n_tlog + n_r);
+            step.n_squeeze = 1;
+            step.tlog = d_tlog;
+            step.rlog = d_tlog + n_tlog;
+        } else {
+            step.st_in = d_fs.p + fsi;
+            step.n_squeeze = 1;
+            step.tlog = d_tlog + 3 * r;
+            step.rlog = d_tlog + n_tlog + 1 + (r - 1);
+        }
+        SumcheckJobs sj;
+        sj.f = sf;
+        sj.b = sb;
+        sj.f2 = sf2;
+        sj.b2 = sb2;
+        sj.acc = ACC;
+        sj.st = 2;
+        sj.log_elems = LOG_W;
+        sj.log_words = LOG_W;
+        launch_sumcheck_fold_step<F>(d_f.p, d_b.p, flen, step, d_f2.p, d_b2.p, d_acc.p + rn, d_acc.p + rf, s, count, sj);
+        fsi ^= 1;
+        rotate();
+        std::swap(d_f, d_f2);
+        std::swap(d_b, d_b2);
+        std::swap(sf, sf2);
+        std::swap(sb, sb2);
+        flen /= 2;
+    }
+    // the last triples (raw), yr, and the first openings' Merkle nodes
+    launch_copy_blocks((uint32_t*)(d_tlog + LAST_W), LOG_W, (const uint32_t*)(d_acc.p + rc), ACC * 2, 6, count, s);
+    launch_copy_blocks((uint32_t*)(pdev + P_YR), yr_len, d_f.p, sf, yr_len, count, s);
+    launch_gather_nodes(w0[0].tree.p, (const uint64_t*)(pdev + P_NIDX0), nodes_at[G], (uint32_t*)(pdev + P_NODES0), s);
+    {
+        const uint32_t seq = ctx->next_flag();
+        launch_signal(ctx->flag_dev(), seq, s);
+        LIG_HIP(hipGetLastError());
+        ctx->wait_flag(seq);
+    }
+
+    // per job: replay the section, yr, the final queries
+    uint64_t* h_fq = (uint64_t*)(pin + P_FQ);
+    uint64_t* h_nidx1 = (uint64_t*)(pin + P_NIDX1);
+    std::vector<size_t> nodes1_at(G + 1, 0);
+    std::vector<std::vector<size_t>> fq(G);
+    for (size_t j = 0; j < G; j++) {
+        Proof<H>& pr = proofs[j];
+        const T* tl = (const T*)(pin + P_LOG) + j * LOG_W;
+        const T* rr = tl + n_tlog;
+        LIG_REQUIRE(*(const uint32_t*)(rr + n_r) == 0, LIG_E_ASSERT, "sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
+        int ri = 0;
+        auto squeeze_check = [&]() {
+            const T v = fs[j].get_field<H>();
+            LIG_REQUIRE(H::eq(v, rr[ri]), LIG_E_ASSERT, "device transcript diverged from the host transcript");
+            ri++;
+        };
+        pr.tr.insert(pr.tr.end(), tl, tl + 3);
+        fs[j].absorb_elems<H>(tl, 3);
+        squeeze_check();
+        const A* last = (const A*)(tl + LAST_W);
+        for (int r = 0; r < k1; r++) {
+            const T* t3 = tl + 3 * (r + 1);
+            T last3[3];
+            if (r + 1 == k1) {
+                for (int i = 0; i < 3; i++) last3[i] = acc_reduce_host<H>(last[i]);
+                t3 = last3;
+            }
+            pr.tr.insert(pr.tr.end(), t3, t3 + 3);
+            fs[j].absorb_elems<H>(t3, 3);
+            if (r + 1 < k1) squeeze_check();
+        }
+        const T* yr = (const T*)(pin + P_YR) + j * yr_len;
+        pr.yr.assign(yr, yr + yr_len);
+        fs[j].absorb_elems<H>(pr.yr.data(), yr_len);
+        fq[j] = fs[j].get_distinct_queries(M1, S);
+        for (size_t i = 0; i < S; i++) h_fq[j * S + i] = fq[j][i] + j * M1;
+        const std::vector<uint64_t> nd = merkle_proof_nodes(M1, fq[j]);
+        LIG_REQUIRE(nd.size() <= S * depth1, LIG_E_ASSERT, "Merkle proof needs more nodes than queries x depth");
+        nodes1_at[j + 1] = nodes1_at[j] + nd.size();
+        for (size_t i = 0; i < nd.size(); i++) h_nidx1[nodes1_at[j] + i] = nd[i] + j * (2 * M1 - 1);
+        // the first opening (prover.rs:72-76)
+        pr.initial.n_rows = S;
+        pr.initial.n_cols = n0;
+        const T* rows = (const T*)(pin + P_ROWS) + j * S * n0;
+        pr.initial.rows.assign(rows, rows + S * n0);
+        const uint8_t* nodes = pin + P_NODES0 + nodes_at[j] * 32;
+        pr.initial.nodes.assign(nodes, nodes + (nodes_at[j + 1] - nodes_at[j]) * 32);
+    }
+    // the final openings (prover.rs:128-142) from the recursive witnesses
+    launch_gather_rows<F>(w1[0].mat.p, n1, (const uint64_t*)(pdev + P_FQ), G * S, (T*)(pdev + P_FROWS), nullptr, s);
+    launch_gather_nodes(w1[0].tree.p, (const uint64_t*)(pdev + P_NIDX1), nodes1_at[G], (uint32_t*)(pdev + P_NODES1), s);
+    {
+        const uint32_t seq = ctx->next_flag();
+        launch_signal(ctx->flag_dev(), seq, s);
+        LIG_HIP(hipGetLastError());
+        ctx->wait_flag(seq);
+    }
+    for (size_t j = 0; j < G; j++) {
+        Rlp<H>& fo = proofs[j].final_;
+        fo.n_rows = S;
+        fo.n_cols = n1;
+        const T* rows = (const T*)(pin + P_FROWS) + j * S * n1;
+        fo.rows.assign(rows, rows + S * n1);
+        const uint8_t* nodes = pin + P_NODES1 + nodes1_at[j] * 32;
+        fo.nodes.assign(nodes, nodes + (nodes1_at[j + 1] - nodes1_at[j]) * 32);
+    }
+}
+
 template struct RsPlan<H32>;
 template struct RsPlan<H128>;
 template struct Config<H32>;
 template struct Config<H128>;
 template void commit<H32>(RsPlan<H32>&, const uint32_t*, size_t, size_t, Witness<H32>&, int);
 template void commit<H128>(RsPlan<H128>&, const u128v*, size_t, size_t, Witness<H128>&, int);
-template void prove<H32>(Config<H32>&, const uint32_t*, uint32_t, Proof<H32>&, Witness<H32>*);
-template void prove<H128>(Config<H128>&, const u128v*, uint32_t, Proof<H128>&, Witness<H128>*);
+template void prove<H32>(Config<H32>&, const uint32_t*, uint32_t, Proof<H32>&);
+template void prove<H128>(Config<H128>&, const u128v*, uint32_t, Proof<H128>&);
 template void commit_group<H32>(RsPlan<H32>&, const uint32_t* const*, int, size_t, size_t, std::vector<Witness<H32>>&,
                                 std::shared_ptr<void>&);
+template void prove_gang<H32>(Config<H32>&, const uint32_t* const*, const uint32_t*, int, Proof<H32>*);
 
 }  // namespace lig

@@ -337,10 +337,13 @@ template <class H> struct Proof {
 };
 
 // prover (prover.rs:15-199) on a device-resident poly.
-// With first, the first commit of d_poly (from commit_group on the same
-// shape) is taken from it instead of being computed.
+template <class H> void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<H>& out);
+// prove() of count device polys of one GF(2^32) configuration with one
+// recursive step (H = H32), in lockstep: every kernel launch covers all of
+// them.  proofs[j] is prove(cfg, d_polys[j], seeds[j]).
+constexpr int GANG_MAX = 256;
 template <class H>
-void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<H>& out, Witness<H>* first = nullptr);
+void prove_gang(Config<H>& cfg, const typename H::T* const* d_polys, const uint32_t* seeds, int count, Proof<H>* proofs);
 // verifier (verifier.rs:10-164) on the host for a configuration's shape:
 // 1 accept, 0 reject, <0 at the reference's panic sites; with dev, the
 // basis inductions (verifier.rs:54-61,148-155) run on that context's GPU.

@@ -1624,6 +1624,23 @@ void launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t
     hipLaunchKernelGGL(k_copy_words, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, dst, src, n);
 }
 
+// n words of each of count blocks (block j = blockIdx.y at src + j *
+// src_stride, to dst + j * dst_stride).
+__global__ void k_copy_blocks(uint32_t* __restrict__ dst, size_t dst_stride, const uint32_t* __restrict__ src,
+                              size_t src_stride, size_t n) {
+    dst += blockIdx.y * dst_stride;
+    src += blockIdx.y * src_stride;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+void launch_copy_blocks(uint32_t* dst, size_t dst_stride, const uint32_t* src, size_t src_stride, size_t n, int count,
+                        hipStream_t s) {
+    if (!n || count < 1) return;
+    if (count > 65535) throw std::invalid_argument("launch_copy_blocks: at most 65535 blocks");
+    hipLaunchKernelGGL(k_copy_blocks, dim3(grid_for(n, 256, 1024), (unsigned)count), dim3(256), 0, s, dst, dst_stride, src,
+                       src_stride, n);
+}
+
 __global__ void k_gather_nodes(const uint32_t* __restrict__ tree, const uint64_t* __restrict__ idx, size_t count,
                                uint32_t* __restrict__ out, WordCopy extra) {
     if (blockIdx.x == 0 && (int)threadIdx.x < extra.n) extra.dst[threadIdx.x] = extra.src[threadIdx.x];
@@ -1667,11 +1684,17 @@ struct FoldWeights32 {
 template <int R>
 __global__ __launch_bounds__(256) void k_fold32(const uint32_t* __restrict__ poly, size_t m, int kk, FoldWeights32 gr,
                                                  uint64_t* __restrict__ zero, int n_zero,
-                                                 uint32_t* __restrict__ out) {
+                                                 uint32_t* __restrict__ out, FoldJobs jobs) {
     static_assert(R == 1 || R == 2, "one or two rows per thread");
     constexpr int NC = 1 << FOLD32_MAXK, BATCH = 16 / R;
     __shared__ __attribute__((aligned(256))) uint32_t tabs[NC * 128];
     __shared__ uint32_t wsh[NC];
+    // several proves at once: job blockIdx.y folds its own input with its
+    // own weights into its own output
+    const uint32_t job = blockIdx.y;
+    if (jobs.polys) poly = jobs.polys[job];
+    out += job * jobs.out_stride;
+    if (zero) zero += job * jobs.zero_stride;
     if (blockIdx.x == 0 && (int)threadIdx.x < n_zero) zero[threadIdx.x] = 0ull;
     const int nc = 1 << kk;
     const uint32_t col_bytes = (uint32_t)(m * 4);
@@ -1697,7 +1720,9 @@ __global__ __launch_bounds__(256) void k_fold32(const uint32_t* __restrict__ pol
             for (int r = 0; r < R; r++) v[t][r] = 0u;
         }
     }
-    if (threadIdx.x == 0) {
+    if (jobs.w) {
+        if ((int)threadIdx.x < NC) wsh[threadIdx.x] = jobs.w[(size_t)job * NC + threadIdx.x];
+    } else if (threadIdx.x == 0) {
 #pragma unroll
         for (int j = 0; j < NC; j++) wsh[j] = gr.w[j];
     }
@@ -1744,11 +1769,28 @@ bool launch_fold32_weights(const uint32_t* poly, size_t m, int kk, const uint32_
     for (int j = 0; j < (1 << kk); j++) gr.w[j] = gr_host[j];
     // two rows per thread when m is even (a thread's pair never straddles m)
     // and the grid still holds a workgroup per CU or more
+    const FoldJobs one{};
     if (m % 2 == 0 && m >= ((size_t)1 << 17))
-        hipLaunchKernelGGL(k_fold32<2>, dim3((unsigned)((m + 511) / 512)), dim3(256), 0, s, poly, m, kk, gr, zero, n_zero, out);
+        hipLaunchKernelGGL(k_fold32<2>, dim3((unsigned)((m + 511) / 512)), dim3(256), 0, s, poly, m, kk, gr, zero, n_zero, out,
+                           one);
     else
-        hipLaunchKernelGGL(k_fold32<1>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, poly, m, kk, gr, zero, n_zero, out);
+        hipLaunchKernelGGL(k_fold32<1>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, poly, m, kk, gr, zero, n_zero, out,
+                           one);
     return true;
+}
+
+void launch_fold32_jobs(size_t m, int kk, const FoldJobs& jobs, int count, uint32_t* out, uint64_t* zero, int n_zero,
+                        hipStream_t s) {
+    if (kk < 1 || kk > FOLD32_MAXK || (m << kk) * 4 >= ((size_t)1 << 31) || (m + 255) / 256 > 0x7fffffffu || !jobs.polys ||
+        !jobs.w || count < 1 || count > 65535 || n_zero < 0 || n_zero > 64 || (n_zero && !zero))
+        throw std::invalid_argument("launch_fold32_jobs: shape");
+    const FoldWeights32 none{};
+    if (m % 2 == 0 && m >= ((size_t)1 << 17))
+        hipLaunchKernelGGL(k_fold32<2>, dim3((unsigned)((m + 511) / 512), (unsigned)count), dim3(256), 0, s,
+                           (const uint32_t*)nullptr, m, kk, none, zero, n_zero, out, jobs);
+    else
+        hipLaunchKernelGGL(k_fold32<1>, dim3((unsigned)((m + 255) / 256), (unsigned)count), dim3(256), 0, s,
+                           (const uint32_t*)nullptr, m, kk, none, zero, n_zero, out, jobs);
 }
 
 template <class F>
@@ -1856,10 +1898,20 @@ template <class F> __device__ inline void block_xor3(typename F::Acc a0, typenam
 constexpr int ACC_QMAX = 148;
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_induce_accum32(
     const uint32_t* __restrict__ P, const uint32_t* __restrict__ Kt, int nq, int ell, int clo, uint32_t* __restrict__ basis,
-    const uint32_t* __restrict__ f, uint64_t* res, uint64_t* res_clear) {
+    const uint32_t* __restrict__ f, uint64_t* res, uint64_t* res_clear, InduceJobs jobs) {
     __shared__ __attribute__((aligned(256))) uint32_t tabs[(ACC_QMAX + 1) * 128];
     __shared__ uint32_t part[1024];
     const size_t np = (size_t)1 << clo, nk = (size_t)1 << (ell - clo);
+    {
+        // job blockIdx.y of several proves of one shape
+        const size_t jb = blockIdx.y;
+        P += jb * ((size_t)nq << clo);
+        Kt += jb * ((size_t)nq << (ell - clo));
+        basis += jb << ell;
+        if (f) f += jb * jobs.f;
+        if (res) res += jb * jobs.res;
+        if (res_clear) res_clear += jb * jobs.res;
+    }
     const size_t X = blockIdx.x;
     const int qs = (int)(blockDim.x >> clo);  // query slices (blockDim.x = qs * 2^clo)
     // the slice is uniform over a wave (2^clo >= 64); saying so keeps the
@@ -1970,11 +2022,23 @@ __device__ __forceinline__ uint32_t wave_xor32(uint32_t x);
 __global__ __launch_bounds__(256) void k_induce_tensor32(const uint32_t* __restrict__ cvals, const uint32_t* __restrict__ ap,
                                                          int ell, int clo, uint32_t* __restrict__ P, uint32_t* __restrict__ Kt,
                                                          const uint32_t* __restrict__ rows, int ncols,
-                                                         const uint32_t* __restrict__ gr, uint64_t* __restrict__ parts) {
+                                                         const uint32_t* __restrict__ gr, uint64_t* __restrict__ parts,
+                                                         InduceJobs jobs) {
     __shared__ __attribute__((aligned(256))) uint32_t tabs[PREP_MAXELL * 128];
     __shared__ uint32_t pb[1024];
     __shared__ uint32_t kb[TENSOR_MAXK];
     const int i = blockIdx.x;
+    {
+        // job blockIdx.y of several proves of one shape (gridDim.x queries each)
+        const size_t j = blockIdx.y, nq = gridDim.x;
+        cvals += j * nq * ell;
+        ap += j * nq;
+        P += j * (nq << clo);
+        Kt += j * (nq << (ell - clo));
+        if (rows) rows += j * nq * ncols;
+        if (gr) gr += j * jobs.gr;
+        if (parts) parts += j * jobs.parts;
+    }
     if (parts && threadIdx.x < 64) {
         uint32_t d = 0u;
         for (int j = threadIdx.x; j < ncols; j += 64) d ^= Ops<F32>::mul(rows[(size_t)i * ncols + j], gr[j]);
@@ -2009,20 +2073,23 @@ __global__ __launch_bounds__(256) void k_induce_tensor32(const uint32_t* __restr
 
 template <class F>
 int launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
-                  typename F::T* basis, hipStream_t s, const EnforcedIn<F>* e, const Eval013In<F>* ev) {
+                  typename F::T* basis, hipStream_t s, const EnforcedIn<F>* e, const Eval013In<F>* ev, int count,
+                  const InduceJobs& jobs) {
     const int clo = induce_low_bits(ell);
     size_t nk = (size_t)1 << (ell - clo);
     int n_parts = 0;
     bool fused = false;
+    if (count < 1 || count > 65535) throw std::invalid_argument("launch_induce: 1..65535 jobs");
     if constexpr (F::BITS == 32) {
         if (ell <= PREP_MAXELL && nk <= (size_t)TENSOR_MAXK && clo <= 10 && (!e || nq <= ENFORCED_MAX_PARTS)) {
-            hipLaunchKernelGGL(k_induce_tensor32, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K,
-                               e ? e->rows : nullptr, e ? e->ncols : 0, e ? e->gr : nullptr, e ? e->parts : nullptr);
+            hipLaunchKernelGGL(k_induce_tensor32, dim3(nq, (unsigned)count), dim3(256), 0, s, cvals, ap, ell, clo, P, K,
+                               e ? e->rows : nullptr, e ? e->ncols : 0, e ? e->gr : nullptr, e ? e->parts : nullptr, jobs);
             fused = true;
             if (e) n_parts = nq;
         }
     }
     if (!fused) {
+        if (count != 1) throw std::invalid_argument("launch_induce: several jobs need the GF(2^32) kernels");
         if (e) {
             launch_enforced<F>(e->rows, nq, e->ncols, e->gr, ap, e->parts, s);
             n_parts = enforced_parts(nq, e->ncols);
@@ -2033,11 +2100,12 @@ int launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, i
         // 1024 lanes per X row: 2^clo entries times 1024 / 2^clo query slices
         // the triple pairs rows X and X + nk/2; P is addressed with 32-bit offsets
         if (clo >= 6 && clo <= 10 && (!ev || nk >= 2) && ((size_t)nq << (clo + 2)) < ((size_t)1 << 31)) {
-            hipLaunchKernelGGL(k_induce_accum32, dim3((unsigned)nk), dim3(1024), 0, s, P, K, nq, ell, clo, basis,
-                               ev ? ev->f : nullptr, ev ? ev->res : nullptr, ev ? ev->res_clear : nullptr);
+            hipLaunchKernelGGL(k_induce_accum32, dim3((unsigned)nk, (unsigned)count), dim3(1024), 0, s, P, K, nq, ell, clo, basis,
+                               ev ? ev->f : nullptr, ev ? ev->res : nullptr, ev ? ev->res_clear : nullptr, jobs);
             return n_parts;
         }
     }
+    if (count != 1) throw std::invalid_argument("launch_induce: several jobs need the GF(2^32) kernels");
     hipLaunchKernelGGL(k_induce_accum<F>, dim3((unsigned)nk), dim3(256), 0, s, P, K, nq, ell, clo, basis);
     if (ev) launch_eval013<F>(ev->f, basis, (size_t)1 << ell, ev->res, ev->res_clear, s);
     return n_parts;
@@ -2078,10 +2146,21 @@ __global__ __launch_bounds__(256) void k_induce_prep32(const uint64_t* __restric
                                                        const uint32_t* __restrict__ alpha, int nq, int ell,
                                                        uint32_t* __restrict__ cvals, uint32_t* __restrict__ ap,
                                                        const uint32_t* __restrict__ mat, size_t n, uint32_t* __restrict__ out,
-                                                       uint32_t* __restrict__ out2) {
+                                                       uint32_t* __restrict__ out2, InduceJobs jobs) {
     // table 0: squaring; 1..ell-1: times sks[t-1]; ell..ell+7: times alpha^(2^k)
     __shared__ __attribute__((aligned(256))) uint32_t tabs[(PREP_MAXELL + 8) * 128];
     const int tid = threadIdx.x;
+    {
+        // job blockIdx.y of several proves of one shape (shared sks)
+        const size_t j = blockIdx.y;
+        q += j * jobs.q;
+        alpha += j * jobs.alpha;
+        cvals += j * (size_t)nq * ell;
+        ap += j * (size_t)nq;
+        if (mat) mat += j * jobs.mat;
+        if (out) out += j * (size_t)nq * n;
+        if (out2) out2 += j * jobs.rows2;
+    }
     if (blockIdx.x > 0) {
         const size_t total = (size_t)nq * n, stride = (size_t)(gridDim.x - 1) * blockDim.x;
         for (size_t i = (size_t)(blockIdx.x - 1) * blockDim.x + tid; i < total; i += stride) {
@@ -2143,16 +2222,20 @@ __global__ __launch_bounds__(256) void k_induce_prep32(const uint64_t* __restric
 
 template <class F>
 void launch_induce_prep(const uint64_t* q, const typename F::T* sks, const typename F::T* alpha, int nq, int ell,
-                        typename F::T* cvals, typename F::T* ap, hipStream_t s, const RowGather<F>* g) {
+                        typename F::T* cvals, typename F::T* ap, hipStream_t s, const RowGather<F>* g, int count,
+                        const InduceJobs& jobs) {
+    if (count < 1 || count > 65535) throw std::invalid_argument("launch_induce_prep: 1..65535 jobs");
     if constexpr (F::BITS == 32) {
         if (nq <= 256 && ell <= PREP_MAXELL) {
             // workgroup 0 the constants, the rest the row gather
             const unsigned ng = g && nq ? (unsigned)grid_for((size_t)nq * g->n, 256) : 0u;
-            hipLaunchKernelGGL(k_induce_prep32, dim3(1 + ng), dim3(256), 0, s, q, sks, alpha, nq, ell, cvals, ap,
-                               g ? g->mat : nullptr, g ? g->n : (size_t)0, g ? g->out : nullptr, g ? g->out2 : nullptr);
+            hipLaunchKernelGGL(k_induce_prep32, dim3(1 + ng, (unsigned)count), dim3(256), 0, s, q, sks, alpha, nq, ell, cvals,
+                               ap, g ? g->mat : nullptr, g ? g->n : (size_t)0, g ? g->out : nullptr, g ? g->out2 : nullptr,
+                               jobs);
             return;
         }
     }
+    if (count != 1) throw std::invalid_argument("launch_induce_prep: several jobs need the GF(2^32) kernel");
     if (g) launch_gather_rows<F>(g->mat, g->n, q, (size_t)nq, g->out, g->out2, s);
     hipLaunchKernelGGL(k_induce_prep<F>, dim3((nq + 63) / 64), dim3(64), 0, s, q, sks, alpha, nq, ell, cvals, ap);
 }
@@ -2283,9 +2366,30 @@ template <class F, bool STEP>
 __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __restrict__ f, const typename F::T* __restrict__ b,
                                                        size_t len, const typename F::T* __restrict__ rp, FoldStep<F> step,
                                                        typename F::T* __restrict__ f2, typename F::T* __restrict__ b2,
-                                                       typename F::Acc* res, typename F::Acc* res_clear) {
+                                                       typename F::Acc* res, typename F::Acc* res_clear, SumcheckJobs jobs) {
     using T = typename F::T;
     using A = typename F::Acc;
+    {
+        // job blockIdx.y of several proves of one shape: its own polys,
+        // accumulators, transcript state and logs
+        const size_t j = blockIdx.y;
+        f += j * jobs.f;
+        b += j * jobs.b;
+        f2 += j * jobs.f2;
+        b2 += j * jobs.b2;
+        res += j * jobs.acc;
+        res_clear += j * jobs.acc;
+        if constexpr (STEP) {
+            if (step.st_in) step.st_in += j * jobs.st;
+            step.st_out += j * jobs.st;
+            step.res_in += j * jobs.acc;
+            if (step.enf) step.enf += j * jobs.acc;
+            if (step.err) step.err += j * jobs.log_words;
+            if (step.bnew) step.bnew += j * jobs.b;
+            step.tlog += j * jobs.log_elems;
+            step.rlog += j * jobs.log_elems;
+        }
+    }
     if (blockIdx.x == 0 && threadIdx.x < 3) res_clear[threadIdx.x] = A{};
     const size_t L2 = len / 2, h2 = L2 / 2;
     const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
@@ -2303,7 +2407,7 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
     if constexpr (STEP) {
         // the preceding transcript step, computed by wave 0 of every workgroup
         __shared__ T r_sh[2];
-        if (!step.st_in && threadIdx.x >= 64 && threadIdx.x < 128) {
+        if (step.lead && threadIdx.x >= 64 && threadIdx.x < 128) {
             // a section's leading step: wave 1, idle until the challenge is
             // out, runs one compression on dummy words first, so the shared
             // compression code is in this CU's instruction cache by the time
@@ -2417,14 +2521,16 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
 template <class F>
 void launch_sumcheck_fold_step(const typename F::T* f, const typename F::T* b, size_t len, const FoldStep<F>& step,
                                typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
-                               hipStream_t s) {
+                               hipStream_t s, int count, const SumcheckJobs& jobs) {
     if (step.res_in == res || step.res_in == res_clear || step.st_in == step.st_out)
         throw std::invalid_argument("fold step: the triple or state read must not be written by the same kernel");
     if (step.n_squeeze < 1 || step.n_squeeze > 2 || (step.bnew && step.n_squeeze != 2))
         throw std::invalid_argument("fold step: one challenge, or two with glue");
+    if ((!step.lead && !step.st_in) || count < 1 || count > 65535 || (count > 1 && !step.st_in))
+        throw std::invalid_argument("fold step: a state to start from (per job for several jobs), 1..65535 jobs");
     size_t h2 = len / 4;
-    hipLaunchKernelGGL((k_sumcheck_fold<F, true>), dim3(grid_for(h2 ? h2 : 1, 256, 1024)), dim3(256), 0, s, f, b, len,
-                       (const typename F::T*)nullptr, step, f2, b2, res, res_clear);
+    hipLaunchKernelGGL((k_sumcheck_fold<F, true>), dim3(grid_for(h2 ? h2 : 1, 256, 1024), (unsigned)count), dim3(256), 0, s, f,
+                       b, len, (const typename F::T*)nullptr, step, f2, b2, res, res_clear, jobs);
 }
 
 // -------------------------------------------------------- microbenchmarks ---
@@ -2633,13 +2739,13 @@ double bench_rate(int which, uint32_t* scratch_word, hipStream_t s) {
     template void launch_fold<F>(const F::T*, size_t, int, const F::T*, F::T*, hipStream_t);                                \
     template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, F::T*, hipStream_t);                  \
     template int launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t,                     \
-                                  const EnforcedIn<F>*, const Eval013In<F>*);                                               \
+                                  const EnforcedIn<F>*, const Eval013In<F>*, int, const InduceJobs&);                       \
     template void launch_eval013<F>(const F::T*, const F::T*, size_t, F::Acc*, F::Acc*, hipStream_t);                       \
     template void launch_zero_acc<F>(F::Acc*, int, hipStream_t);                                                                 \
     template void launch_sumcheck_fold_step<F>(const F::T*, const F::T*, size_t, const FoldStep<F>&, F::T*, F::T*, F::Acc*, \
-                                               F::Acc*, hipStream_t);                                                       \
+                                               F::Acc*, hipStream_t, int, const SumcheckJobs&);                             \
     template void launch_induce_prep<F>(const uint64_t*, const F::T*, const F::T*, int, int, F::T*, F::T*, hipStream_t,    \
-                                        const RowGather<F>*);                                                               \
+                                        const RowGather<F>*, int, const InduceJobs&);                                       \
     template void launch_enforced<F>(const F::T*, int, int, const F::T*, const F::T*, F::Acc*, hipStream_t);
 
 LIG_INST(F32)

@@ -147,6 +147,9 @@ void launch_gather_nodes(const uint32_t* tree, const uint64_t* node_idx, size_t 
 
 // dst[i] = src[i] for i < n words (dst may be mapped pinned host memory).
 void launch_copy_words(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s);
+// n words from each of count blocks src + j * src_stride to dst + j * dst_stride
+void launch_copy_blocks(uint32_t* dst, size_t dst_stride, const uint32_t* src, size_t src_stride, size_t n, int count,
+                        hipStream_t s);
 
 // Microbenchmarks (best of five timed launches, synchronous): which = 0
 // GF(2^32) products per second by wave-uniform constants with nibble tables
@@ -169,6 +172,17 @@ constexpr int FOLD32_MAXK = 6;
 // It also zeroes zero[0..n_zero) (at most 256), if given.
 bool launch_fold32_weights(const uint32_t* poly, size_t m, int kk, const uint32_t* gr_host, uint32_t* out, hipStream_t s,
                            uint64_t* zero = nullptr, int n_zero = 0);
+// The same fold for `count` proves of one shape in one launch (job j =
+// blockIdx.y): input polys[j], the 2^kk weights at w + j * 64, output at
+// out + j * out_stride, accumulators to zero at zero + j * zero_stride.
+struct FoldJobs {
+    const uint32_t* const* polys = nullptr;
+    const uint32_t* w = nullptr;
+    size_t out_stride = 0;
+    size_t zero_stride = 0;
+};
+void launch_fold32_jobs(size_t m, int kk, const FoldJobs& jobs, int count, uint32_t* out, uint64_t* zero, int n_zero,
+                        hipStream_t s);
 // out (and out2, if given) receive the count rows listed in rows.
 template <class F>
 void launch_gather_rows(const typename F::T* mat, size_t n, const uint64_t* rows, size_t count, typename F::T* out,
@@ -200,9 +214,26 @@ template <class F> struct Eval013In {
 // 0/1/3 triple of (f, basis), inside the accumulation where the GF(2^32)
 // kernel applies, else by launch_eval013.  Returns the number of enforced
 // parts written (0 without e).
+// Several proves of one shape in one launch (job j = blockIdx.y): the
+// strides, in elements of each buffer, between the jobs' inputs and outputs
+// that are not laid out by the kernels themselves (those intermediate
+// buffers hold the jobs one after another: query constants nq * ell,
+// alpha powers nq, P nq * 2^clo, K nq * 2^(ell - clo), basis 2^ell, rows
+// nq * n).  Strides 0 with one job are the single prove.
+struct InduceJobs {
+    size_t q = 0;       // query indices
+    size_t alpha = 0;   // alpha
+    size_t mat = 0;     // the committed matrix the rows are gathered from
+    size_t rows2 = 0;   // second copy of the gathered rows
+    size_t gr = 0;      // row-combination weights
+    size_t parts = 0;   // enforced-sum parts
+    size_t f = 0;       // the folded polynomial of the triple
+    size_t res = 0;     // triple accumulators (res and res_clear)
+};
 template <class F>
 int launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
-                  typename F::T* basis, hipStream_t s, const EnforcedIn<F>* e = nullptr, const Eval013In<F>* ev = nullptr);
+                  typename F::T* basis, hipStream_t s, const EnforcedIn<F>* e = nullptr, const Eval013In<F>* ev = nullptr,
+                  int count = 1, const InduceJobs& jobs = InduceJobs());
 
 // The opened rows of a step: out (and out2, if given) receive the rows of
 // mat (row-major, n per row) listed in the query indices.
@@ -217,7 +248,8 @@ template <class F> struct RowGather {
 // the same queries (one launch for both where the GF(2^32) kernel applies).
 template <class F>
 void launch_induce_prep(const uint64_t* q, const typename F::T* sks, const typename F::T* alpha, int nq, int ell,
-                        typename F::T* cvals, typename F::T* ap, hipStream_t s, const RowGather<F>* g = nullptr);
+                        typename F::T* cvals, typename F::T* ap, hipStream_t s, const RowGather<F>* g = nullptr,
+                        int count = 1, const InduceJobs& jobs = InduceJobs());
 // The enforced sum of sumcheck_polys.rs:249-256, sum_i ap[i] * (row_i . gr)
 // (unreduced), as enforced_parts(nrows, ncols) parts whose XOR is the sum.
 constexpr int ENFORCED_MAX_PARTS = 256;
@@ -250,6 +282,7 @@ template <class F> void launch_zero_acc(typename F::Acc* res, int count, hipStre
 // first of two challenges.  res_in must differ from res and res_clear (all
 // workgroups read it while the kernel writes those).
 template <class F> struct FoldStep {
+    int lead;  // a section's leading step (enforced-sum check, one or two challenges)
     const DevTranscript* st_in;
     DevTranscript init;
     DevTranscript* st_out;
@@ -262,8 +295,18 @@ template <class F> struct FoldStep {
     typename F::T* tlog;
     typename F::T* rlog;
 };
+// Strides between the jobs of a fold step launched for several proves of
+// one shape (job j = blockIdx.y), in elements, accumulators, transcript
+// states and log words; zero for a single prove.
+struct SumcheckJobs {
+    size_t f = 0, b = 0, f2 = 0, b2 = 0;  // polys (b also for the glued basis)
+    size_t acc = 0;                       // res, res_clear, the triple read and the enforced-sum parts
+    size_t st = 0;                        // transcript states in and out
+    size_t log_elems = 0;                 // triple and challenge logs
+    size_t log_words = 0;                 // check flag
+};
 template <class F>
 void launch_sumcheck_fold_step(const typename F::T* f, const typename F::T* b, size_t len, const FoldStep<F>& step,
                                typename F::T* f2, typename F::T* b2, typename F::Acc* res, typename F::Acc* res_clear,
-                               hipStream_t s);
+                               hipStream_t s, int count = 1, const SumcheckJobs& jobs = SumcheckJobs());
 }  // namespace lig

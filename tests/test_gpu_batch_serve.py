@@ -57,6 +57,25 @@ def test_batch_equals_plain_and_oracle(O, ctx):
     b.close()
 
 
+def test_gang_proofs_equal_plain_and_oracle(O, ctx):
+    # same-size jobs are proved in lockstep (one launch per kernel for the
+    # gang): gangs of 9 x 2^16, 5 x 2^19, 3 x 2^21 and 6 x 2^22 (two gangs of
+    # three, the per-gang element cap), every proof equal to the plain path's
+    sizes = [16] * 9 + [19] * 5 + [21] * 3 + [22] * 6
+    say("gang gf32", sizes)
+    polys, seeds = jobs(O, sizes, 32, 700)
+    b = lh.Batch(0, 32, n_streams=2)
+    proofs = b.prove(sizes, polys, seeds)
+    cfgs = {}
+    for i, (L, poly, seed, p) in enumerate(zip(sizes, polys, seeds, proofs)):
+        say("gang job", i, "log", L)
+        assert p.to_bytes() == plain_bytes(ctx, cfgs, L, 32, poly, seed)
+    for i in (0, 8, 9):
+        assert proofs[i].to_bytes() == O.prove(polys[i], sizes[i], 32, fs_seed=seeds[i])
+    assert lh.verifier(22, proofs[-1], seeds[-1]) == 1
+    b.close()
+
+
 def test_batch_device_inputs(O, ctx):
     # device buffers from the HIP runtime the library itself loaded
     hip = C.CDLL("libamdhip64.so.7")
