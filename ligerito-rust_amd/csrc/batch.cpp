@@ -94,17 +94,17 @@ template <class H> Batch<H>::Batch(int dev, int n_streams) : device(dev) {
     for (int i = 0; i < n_streams; i++) workers.emplace_back(new Worker<H>(dev));
 }
 
-// Gangs: same-size GF(2^32) jobs whose configuration has one recursive step
-// (2^23 coefficients or fewer) are proved in lockstep by one worker
+// Gangs: same-size GF(2^32) jobs are proved in lockstep by one worker
 // (prove_gang), every kernel launched once for the gang; at most
 // GANG_BATCH_MAX jobs and 2^GANG_MAX_ELEMS_LOG input elements per gang.
 // Measured on MI355X, 36 jobs of one size on 12 workers: 2^16 jobs 47.5 ->
 // 24.5 us each, 2^19 60.3 -> 40.4, 2^20 81.2 -> 61.8, 2^22 unchanged (190
 // us: bound by the encode); config D 21.0 -> 16.4 ms.  Gangs of 8 or 16
-// jobs, or of 2^23 / 2^25 / 2^26 elements, measured the same within noise:
-// a gang's host steps (queries, node lists) run on its one worker thread,
-// so larger gangs trade fewer launches for longer host phases.
-constexpr int GANG_BATCH_MAX = 4, GANG_MAX_ELEMS_LOG = 24;
+// jobs, or of 2^23 / 2^26 elements, measured the same within noise: a
+// gang's host steps (queries, node lists) run on its one worker thread, so
+// larger gangs trade fewer launches for longer host phases.  Eight 2^24
+// proves: 6.7 ms without gangs, 6.4 ms in gangs of two (means of five runs).
+constexpr int GANG_BATCH_MAX = 4, GANG_MAX_ELEMS_LOG = 25;
 
 static bool gangs_enabled() {
     static const bool on = [] {
@@ -159,7 +159,7 @@ void Batch<H>::run(size_t n_jobs, const int* log_n, const void* const* polys, bo
         while (e < n_jobs && log_n[order[e]] == log_n[order[k]]) e++;
         const int L = log_n[order[k]];
         size_t cap = 1;
-        if (std::is_same<H, H32>::value && gangs_enabled() && Shape::generated(L).recursive_steps == 1) {
+        if (std::is_same<H, H32>::value && gangs_enabled()) {
             cap = L >= GANG_MAX_ELEMS_LOG ? 1 : ((size_t)1 << GANG_MAX_ELEMS_LOG) >> L;
             cap = std::min<size_t>(cap, GANG_BATCH_MAX);
         }

@@ -950,52 +950,60 @@ void prove(Config<H>& cfg, const typename H::T* d_poly, uint32_t fs_seed, Proof<
 }
 
 // ------------------------------------------------------------ gang prove --
-// `count` proves of one GF(2^32) configuration with one recursive step, in
-// lockstep: each kernel is launched once for all of them (job j =
-// blockIdx.y, or one table of pointers), and at each of the four
-// synchronisations the host runs every job's transcript steps.  A small
-// prove is a chain of short latency-bound kernels (transcript steps, Merkle
-// tops, small folds) that leave most of the GPU idle, and a device runs few
-// kernels at once (one per hardware queue); here a chain serves the whole
-// group.  The transcript schedule of each job is prove()'s (prover.rs:29-195),
-// so its proof bytes are the same.
+// `count` proves of one GF(2^32) configuration in lockstep: each kernel is
+// launched once for all of them (job j = blockIdx.y, or one table of
+// pointers), and at each synchronisation the host runs every job's
+// transcript steps.  A small prove is a chain of short latency-bound
+// kernels (transcript steps, Merkle tops, small folds) that leave most of
+// the GPU idle, and a device runs few kernels at once (one per hardware
+// queue); here a chain serves the whole gang.  Each job follows prove()'s
+// transcript schedule (prover.rs:29-195), so its proof bytes are the same.
 //
-// Pinned staging, from GANG_PIN (below it: the flag and commit_group's
-// region), per job at fixed strides: fold input pointer, fold weights,
-// query indices, induce constants, initial transcript state, section logs,
-// opened rows, yr; then the Merkle node indices and nodes of the first and
-// of the final openings, sized by the queries' worst case.
+// Witness levels: 0 is the first commit, l = i + 1 the commit of step i;
+// level l is opened at queries over its 4 m rows (level 0 as the proof's
+// first opening, levels 1..R-1 as its recursive openings, level R as the
+// final one).  Pinned staging, from GANG_PIN (below it: the flag and
+// commit_group's region), per job at fixed strides: fold input pointer,
+// fold weights, query indices, induce constants, the section's initial
+// transcript state and logs, yr, and per level the opened rows, the Merkle
+// node indices and the nodes (sized by the queries' worst case).
 template <class H>
 void prove_gang(Config<H>& cfg, const typename H::T* const* d_polys, const uint32_t* seeds, int count, Proof<H>* proofs) {
     static_assert(std::is_same<H, H32>::value, "gang proves are GF(2^32)");
     using F = F32;
     using T = uint32_t;
     using A = uint64_t;
+    const int R = cfg.recursive_steps;
     LIG_REQUIRE(count >= 1 && count <= GANG_MAX, LIG_E_INVALID, "gang prove: 1..GANG_MAX proves");
-    LIG_REQUIRE(cfg.recursive_steps == 1, LIG_E_INVALID, "gang prove: configurations with one recursive step");
-    LIG_REQUIRE(cfg.initial_k >= 1 && cfg.initial_k <= FOLD32_MAXK && cfg.ks[0] >= 1 && cfg.ks[0] <= 8, LIG_E_INVALID,
-                "gang prove: fold widths");
+    LIG_REQUIRE(R >= 1 && cfg.initial_k >= 1 && cfg.initial_k <= FOLD32_MAXK && cfg.initial_n <= 64, LIG_E_INVALID,
+                "gang prove: recursive steps, fold width");
+    int maxk = 0;
+    for (int k : cfg.ks) maxk = std::max(maxk, k);
+    LIG_REQUIRE(maxk >= 1 && maxk <= 8, LIG_E_INVALID, "gang prove: fold rounds per step");
     for (int j = 0; j < count; j++) LIG_REQUIRE(d_polys[j], LIG_E_INVALID, "NULL polynomial");
     Ctx* ctx = cfg.ctx;
     hipStream_t s = ctx->stream;
     const size_t S = 148;  // prover.rs:30
     const size_t G = (size_t)count;
-    const size_t m0 = cfg.initial_m, n0 = cfg.initial_n, M0 = 4 * m0;
-    const size_t m1 = cfg.dims_m[0], n1 = cfg.dims_n[0], M1 = 4 * m1;
-    const int k0 = cfg.initial_k, k1 = cfg.ks[0];
-    const size_t flen0 = m0;  // N >> initial_k
-    const int ell = ilog2(flen0);
-    const size_t yr_len = flen0 >> k1;
-    const Vec<H>& sks = cfg.sks_by_ell.at(ell);
-    const size_t depth0 = (size_t)ilog2(M0), depth1 = (size_t)ilog2(M1);
+    std::vector<size_t> Mw(R + 1), nw(R + 1), depth(R + 1);
+    Mw[0] = 4 * cfg.initial_m;
+    nw[0] = cfg.initial_n;
+    for (int i = 0; i < R; i++) {
+        Mw[i + 1] = 4 * cfg.dims_m[i];
+        nw[i + 1] = cfg.dims_n[i];
+    }
+    for (int l = 0; l <= R; l++) depth[l] = (size_t)ilog2(Mw[l]);
+    const size_t flen0 = cfg.initial_m;  // N >> initial_k
+    const size_t yr_len = cfg.dims_m.back();
 
     // pinned layout (bytes)
     constexpr size_t GANG_PIN = (size_t)64 << 10;
-    constexpr size_t CONST_W = 80;  // per job: 64 weights | alpha (induce), padded
-    constexpr size_t LOG_W = 64;    // per job: triples | challenges | check flag, raw last triple at LAST_W
+    constexpr size_t CONST_W = 80;  // per job: up to 64 row weights | alpha at word 64
+    constexpr size_t LOG_W = 64;    // per job: triples | challenges | check flag; raw last triple at LAST_W
     constexpr size_t LAST_W = 40;
-    const size_t n_tlog = 3 * (size_t)(k1 + 1), n_r = (size_t)k1 + 2;
-    LIG_REQUIRE(n_tlog + n_r + 1 <= LAST_W && n0 <= 64, LIG_E_INVALID, "gang prove: section logs exceed their region");
+    const size_t n_tlog = 3 * (size_t)(maxk + 1), n_r = (size_t)maxk + 2;
+    LIG_REQUIRE(n_tlog + n_r + 1 <= LAST_W, LIG_E_INVALID, "gang prove: section logs exceed their region");
+    for (int l = 0; l < R; l++) LIG_REQUIRE(nw[l] <= 64, LIG_E_INVALID, "gang prove: rows of at most 64 elements");
     size_t off = GANG_PIN;
     auto region = [&](size_t bytes) {
         const size_t at = off;
@@ -1003,33 +1011,38 @@ void prove_gang(Config<H>& cfg, const typename H::T* const* d_polys, const uint3
         return at;
     };
     const size_t P_PTR = region(G * 8), P_W = region(G * 64 * 4), P_Q = region(G * S * 8),
-                 P_SKS = region((sks.size()) * 4), P_CONST = region(G * CONST_W * 4),
+                 P_SKS = region(((size_t)ilog2(flen0) + 1) * 4), P_CONST = region(G * CONST_W * 4),
                  P_INIT = region(G * 2 * sizeof(DevTranscript)), P_LOG = region(G * LOG_W * 4),
-                 P_ROWS = region(G * S * n0 * 4), P_YR = region(G * yr_len * 4), P_FQ = region(G * S * 8),
-                 P_NIDX0 = region(G * S * depth0 * 8), P_NODES0 = region(G * S * depth0 * 32),
-                 P_NIDX1 = region(G * S * depth1 * 8), P_NODES1 = region(G * S * depth1 * 32),
-                 P_FROWS = region(G * S * n1 * 4);
+                 P_YR = region(G * yr_len * 4), P_FQ = region(G * S * 8);
+    std::vector<size_t> P_ROWS(R + 1), P_NIDX(R + 1), P_NODES(R + 1);
+    for (int l = 0; l <= R; l++) {
+        P_ROWS[l] = region(G * S * nw[l] * 4);
+        P_NIDX[l] = region(G * S * depth[l] * 8);
+        P_NODES[l] = region(G * S * depth[l] * 32);
+    }
     uint8_t* pin = ctx->pinned(off);
     uint8_t* pdev = ctx->pin_dev;
+    T* h_const = (T*)(pin + P_CONST);
+    T* d_tlog = (T*)(pdev + P_LOG);
 
-    // first commits of every job
-    std::vector<Witness<H>> w0, w1;
-    std::shared_ptr<void> keep0, keep1;
-    commit_group<H>(*cfg.initial_rs, d_polys, count, m0, n0, w0, keep0);
+    // witnesses per level (each a view into its gang buffers, kept to the end)
+    std::vector<std::vector<Witness<H>>> wl(R + 1);
+    std::vector<std::shared_ptr<void>> keep(R + 1);
+    commit_group<H>(*cfg.initial_rs, d_polys, count, cfg.initial_m, cfg.initial_n, wl[0], keep[0]);
 
     std::vector<Transcript> fs;
     fs.reserve(G);
-    std::vector<Vec<H>> gr0(G);
+    std::vector<Vec<H>> gr(G);
     T* h_w = (T*)(pin + P_W);
     for (size_t j = 0; j < G; j++) {
         proofs[j] = Proof<H>();
         fs.emplace_back(seeds[j]);
-        memcpy(proofs[j].initial_root.data(), w0[j].root, 32);
-        fs[j].absorb(w0[j].root, 32);
-        Vec<H> pe0(k0);
+        memcpy(proofs[j].initial_root.data(), wl[0][j].root, 32);
+        fs[j].absorb(wl[0][j].root, 32);
+        Vec<H> pe0(cfg.initial_k);
         for (auto& r : pe0) r = fs[j].get_field<H>();
-        gr0[j] = lagrange_basis<H>(pe0);
-        memcpy(h_w + j * 64, gr0[j].data(), gr0[j].size() * 4);
+        gr[j] = lagrange_basis<H>(pe0);
+        memcpy(h_w + j * 64, gr[j].data(), gr[j].size() * 4);
         ((const T**)(pin + P_PTR))[j] = d_polys[j];
     }
 
@@ -1044,210 +1057,252 @@ void prove_gang(Config<H>& cfg, const typename H::T* const* d_polys, const uint3
         rn = rf;
         rf = t;
     };
-    // partial_eval of every job (its zeroing of res_next and res_free)
+    // partial_eval of every job (zeroing res_next and res_free)
     PBuf<T> d_f(&ctx->pool, G * flen0), d_f2(&ctx->pool, G * flen0 / 2);
+    PBuf<T> d_b(&ctx->pool, G * flen0), d_b2(&ctx->pool, G * flen0 / 2);
+    size_t sf = flen0, sf2 = flen0 / 2, sb = flen0, sb2 = flen0 / 2, flen = flen0;
     {
         FoldJobs fj;
         fj.polys = (const T* const*)(pdev + P_PTR);
         fj.w = (const T*)(pdev + P_W);
         fj.out_stride = flen0;
         fj.zero_stride = ACC;
-        launch_fold32_jobs(flen0, k0, fj, count, d_f.p, d_acc.p + rn, 6, s);
+        launch_fold32_jobs(flen0, cfg.initial_k, fj, count, d_f.p, d_acc.p + rn, 6, s);
         LIG_HIP(hipGetLastError());
     }
-    // the recursive commit of every folded polynomial (synchronizes)
-    {
+    // the commit of level l >= 1: every job's folded polynomial (synchronizes)
+    auto commit_level = [&](int l) {
         std::vector<const T*> fp(G);
-        for (size_t j = 0; j < G; j++) fp[j] = d_f.p + j * flen0;
-        commit_group<H>(*cfg.rs[0], fp.data(), count, m1, n1, w1, keep1);
-    }
+        for (size_t j = 0; j < G; j++) fp[j] = d_f.p + j * sf;
+        commit_group<H>(*cfg.rs[l - 1], fp.data(), count, cfg.dims_m[l - 1], cfg.dims_n[l - 1], wl[l], keep[l]);
+    };
+    commit_level(1);
 
-    // per job: root, queries, alpha; the openings' node lists
-    std::vector<std::vector<size_t>> q(G);
-    uint64_t* h_q = (uint64_t*)(pin + P_Q);
-    T* h_const = (T*)(pin + P_CONST);
-    memcpy(pin + P_SKS, sks.data(), sks.size() * 4);
-    uint64_t* h_nidx0 = (uint64_t*)(pin + P_NIDX0);
+    // after a commit of level l + 1: per job, absorb its root, draw level l's
+    // queries and alpha, list level l's Merkle proof nodes, and export the
+    // transcript the next section starts from; then the induction over
+    // level l's opened rows (the basis into d_basis)
     std::vector<size_t> nodes_at(G + 1, 0);
-    for (size_t j = 0; j < G; j++) {
-        proofs[j].rec_roots.emplace_back();
-        memcpy(proofs[j].rec_roots.back().data(), w1[j].root, 32);
-        fs[j].absorb(w1[j].root, 32);
-        q[j] = fs[j].get_distinct_queries(M0, S);
-        const T alpha = fs[j].get_field<H>();
-        LIG_REQUIRE(q[j].size() == S, LIG_E_ASSERT, "distinct queries");
-        for (size_t i = 0; i < S; i++) h_q[j * S + i] = q[j][i];
-        memcpy(h_const + j * CONST_W, gr0[j].data(), gr0[j].size() * 4);
-        h_const[j * CONST_W + 64] = alpha;
-        const std::vector<uint64_t> nd = merkle_proof_nodes(M0, q[j]);
-        LIG_REQUIRE(nd.size() <= S * depth0, LIG_E_ASSERT, "Merkle proof needs more nodes than queries x depth");
-        nodes_at[j + 1] = nodes_at[j] + nd.size();
-        for (size_t i = 0; i < nd.size(); i++) h_nidx0[nodes_at[j] + i] = nd[i] + j * (2 * M0 - 1);
-        // the section's leading step starts from this state
-        DevTranscript* init = (DevTranscript*)(pin + P_INIT) + 2 * j;
-        export_transcript(fs[j], *init);
-        ((uint32_t*)(pin + P_LOG))[j * LOG_W + n_tlog + n_r] = 0;  // check flag
-    }
-
-    // induce_sumcheck_poly of every job: opened rows, basis, enforced sum,
-    // and the 0/1/3 triple of (f, basis)
-    const int clo = induce_low_bits(ell);
-    PBuf<T> d_c(&ctx->pool, G * S * ell), d_ap(&ctx->pool, G * S);
-    PBuf<T> d_P(&ctx->pool, G * (S << clo)), d_K(&ctx->pool, G * (S << (ell - clo)));
-    PBuf<T> d_b(&ctx->pool, G * flen0), d_b2(&ctx->pool, G * flen0 / 2), d_rows(&ctx->pool, G * S * n0);
-    {
+    auto open_and_induce = [&](int l, PBuf<T>& d_basis) {
+        uint64_t* h_q = (uint64_t*)(pin + P_Q);
+        uint64_t* h_nidx = (uint64_t*)(pin + P_NIDX[l]);
+        const int ell = ilog2(flen);
+        const Vec<H>& sks = cfg.sks_by_ell.at(ell);
+        memcpy(pin + P_SKS, sks.data(), sks.size() * 4);
+        nodes_at.assign(G + 1, 0);
+        for (size_t j = 0; j < G; j++) {
+            proofs[j].rec_roots.emplace_back();
+            memcpy(proofs[j].rec_roots.back().data(), wl[l + 1][j].root, 32);
+            fs[j].absorb(wl[l + 1][j].root, 32);
+            const std::vector<size_t> q = fs[j].get_distinct_queries(Mw[l], S);
+            const T alpha = fs[j].get_field<H>();
+            LIG_REQUIRE(q.size() == S && gr[j].size() == nw[l], LIG_E_ASSERT, "queries or opened row length");
+            for (size_t i = 0; i < S; i++) h_q[j * S + i] = q[i];
+            memcpy(h_const + j * CONST_W, gr[j].data(), gr[j].size() * 4);
+            h_const[j * CONST_W + 64] = alpha;
+            const std::vector<uint64_t> nd = merkle_proof_nodes(Mw[l], q);
+            LIG_REQUIRE(nd.size() <= S * depth[l], LIG_E_ASSERT, "Merkle proof needs more nodes than queries x depth");
+            nodes_at[j + 1] = nodes_at[j] + nd.size();
+            for (size_t i = 0; i < nd.size(); i++) h_nidx[nodes_at[j] + i] = nd[i] + j * (2 * Mw[l] - 1);
+            export_transcript(fs[j], *((DevTranscript*)(pin + P_INIT) + 2 * j));
+            ((uint32_t*)(pin + P_LOG))[j * LOG_W + n_tlog + n_r] = 0;  // check flag
+        }
+        const int clo = induce_low_bits(ell);
+        PBuf<T> d_c(&ctx->pool, G * S * ell), d_ap(&ctx->pool, G * S);
+        PBuf<T> d_P(&ctx->pool, G * (S << clo)), d_K(&ctx->pool, G * (S << (ell - clo)));
+        PBuf<T> d_rows(&ctx->pool, G * S * nw[l]);
+        d_basis = PBuf<T>(&ctx->pool, G * flen);
         InduceJobs ij;
         ij.q = S;
         ij.alpha = CONST_W;
-        ij.mat = M0 * n0;
-        ij.rows2 = S * n0;
+        ij.mat = Mw[l] * nw[l];
+        ij.rows2 = S * nw[l];
         ij.gr = CONST_W;
         ij.parts = ACC;
-        ij.f = flen0;
+        ij.f = sf;
         ij.res = ACC;
         RowGather<F> g;
-        g.mat = w0[0].mat.p;  // the group's matrices, M0 x n0 apart
-        g.n = n0;
+        g.mat = wl[l][0].mat.p;  // the level's gang matrices, Mw[l] x nw[l] apart
+        g.n = nw[l];
         g.out = d_rows.p;
-        g.out2 = (T*)(pdev + P_ROWS);
+        g.out2 = (T*)(pdev + P_ROWS[l]);
         const T* d_const = (const T*)(pdev + P_CONST);
-        launch_induce_prep<F>((const uint64_t*)(pdev + P_Q), (const T*)(pdev + P_SKS), d_const + 64, (int)S, ell, d_c.p, d_ap.p,
-                              s, &g, count, ij);
+        launch_induce_prep<F>((const uint64_t*)(pdev + P_Q), (const T*)(pdev + P_SKS), d_const + 64, (int)S, ell, d_c.p,
+                              d_ap.p, s, &g, count, ij);
         EnforcedIn<F> e;
         e.rows = d_rows.p;
-        e.ncols = (int)n0;
+        e.ncols = (int)nw[l];
         e.gr = d_const;
         e.parts = d_acc.p + 9;
         Eval013In<F> ev;
         ev.f = d_f.p;
         ev.res = d_acc.p + rn;
         ev.res_clear = d_acc.p + rf;
-        launch_induce<F>(d_c.p, d_ap.p, (int)S, ell, d_P.p, d_K.p, d_b.p, s, &e, &ev, count, ij);
+        launch_induce<F>(d_c.p, d_ap.p, (int)S, ell, d_P.p, d_K.p, d_basis.p, s, &e, &ev, count, ij);
         LIG_HIP(hipGetLastError());
         rotate();
-    }
+    };
 
-    // the sumcheck section of the step (prover.rs:96-102): k1 fold rounds,
-    // each running the preceding transcript step per job on the device
+    // the sumcheck section of a step: k fold rounds, each running the
+    // preceding transcript step per job on the device (the leading step with
+    // n0 challenges and, with bnew, the glue)
     PBuf<DevTranscript> d_fs(&ctx->pool, 2 * G);
     int fsi = 0;
-    size_t flen = flen0;
-    T* d_tlog = (T*)(pdev + P_LOG);
-    size_t sf = flen0, sf2 = flen0 / 2, sb = flen0, sb2 = flen0 / 2;
-    for (int r = 0; r < k1; r++) {
-        FoldStep<F> step{};
-        step.st_out = d_fs.p + (fsi ^ 1);
-        step.res_in = d_acc.p + rc;
-        if (r == 0) {
-            step.lead = 1;
-            step.st_in = (const DevTranscript*)(pdev + P_INIT);
-            step.enf = d_acc.p + 9;
-            step.n_enf = (int)S;
-            step.err = (uint32_t*)(d_tlog + n_tlog + n_r);
-            step.n_squeeze = 1;
-            step.tlog = d_tlog;
-            step.rlog = d_tlog + n_tlog;
-        } else {
-            step.st_in = d_fs.p + fsi;
-            step.n_squeeze = 1;
-            step.tlog = d_tlog + 3 * r;
-            step.rlog = d_tlog + n_tlog + 1 + (r - 1);
-        }
-        SumcheckJobs sj;
-        sj.f = sf;
-        sj.b = sb;
-        sj.f2 = sf2;
-        sj.b2 = sb2;
-        sj.acc = ACC;
-        sj.st = 2;
-        sj.log_elems = LOG_W;
-        sj.log_words = LOG_W;
-        launch_sumcheck_fold_step<F>(d_f.p, d_b.p, flen, step, d_f2.p, d_b2.p, d_acc.p + rn, d_acc.p + rf, s, count, sj);
-        fsi ^= 1;
-        rotate();
-        std::swap(d_f, d_f2);
-        std::swap(d_b, d_b2);
-        std::swap(sf, sf2);
-        std::swap(sb, sb2);
-        flen /= 2;
-    }
-    // the last triples (raw), yr, and the first openings' Merkle nodes
-    launch_copy_blocks((uint32_t*)(d_tlog + LAST_W), LOG_W, (const uint32_t*)(d_acc.p + rc), ACC * 2, 6, count, s);
-    launch_copy_blocks((uint32_t*)(pdev + P_YR), yr_len, d_f.p, sf, yr_len, count, s);
-    launch_gather_nodes(w0[0].tree.p, (const uint64_t*)(pdev + P_NIDX0), nodes_at[G], (uint32_t*)(pdev + P_NODES0), s);
-    {
-        const uint32_t seq = ctx->next_flag();
-        launch_signal(ctx->flag_dev(), seq, s);
-        LIG_HIP(hipGetLastError());
-        ctx->wait_flag(seq);
-    }
-
-    // per job: replay the section, yr, the final queries
-    uint64_t* h_fq = (uint64_t*)(pin + P_FQ);
-    uint64_t* h_nidx1 = (uint64_t*)(pin + P_NIDX1);
-    std::vector<size_t> nodes1_at(G + 1, 0);
-    std::vector<std::vector<size_t>> fq(G);
-    for (size_t j = 0; j < G; j++) {
-        Proof<H>& pr = proofs[j];
-        const T* tl = (const T*)(pin + P_LOG) + j * LOG_W;
-        const T* rr = tl + n_tlog;
-        LIG_REQUIRE(*(const uint32_t*)(rr + n_r) == 0, LIG_E_ASSERT, "sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
-        int ri = 0;
-        auto squeeze_check = [&]() {
-            const T v = fs[j].get_field<H>();
-            LIG_REQUIRE(H::eq(v, rr[ri]), LIG_E_ASSERT, "device transcript diverged from the host transcript");
-            ri++;
-        };
-        pr.tr.insert(pr.tr.end(), tl, tl + 3);
-        fs[j].absorb_elems<H>(tl, 3);
-        squeeze_check();
-        const A* last = (const A*)(tl + LAST_W);
-        for (int r = 0; r < k1; r++) {
-            const T* t3 = tl + 3 * (r + 1);
-            T last3[3];
-            if (r + 1 == k1) {
-                for (int i = 0; i < 3; i++) last3[i] = acc_reduce_host<H>(last[i]);
-                t3 = last3;
+    auto section = [&](int n0, int k, const T* bnew, size_t bnew_stride) {
+        for (int r = 0; r < k; r++) {
+            FoldStep<F> step{};
+            step.st_out = d_fs.p + (fsi ^ 1);
+            step.res_in = d_acc.p + rc;
+            if (r == 0) {
+                step.lead = 1;
+                step.st_in = (const DevTranscript*)(pdev + P_INIT);
+                step.enf = d_acc.p + 9;
+                step.n_enf = (int)S;
+                step.err = (uint32_t*)(d_tlog + n_tlog + n_r);
+                step.n_squeeze = n0;
+                step.bnew = bnew;
+                step.tlog = d_tlog;
+                step.rlog = d_tlog + n_tlog;
+            } else {
+                step.st_in = d_fs.p + fsi;
+                step.n_squeeze = 1;
+                step.tlog = d_tlog + 3 * r;
+                step.rlog = d_tlog + n_tlog + n0 + (r - 1);
             }
-            pr.tr.insert(pr.tr.end(), t3, t3 + 3);
-            fs[j].absorb_elems<H>(t3, 3);
-            if (r + 1 < k1) squeeze_check();
+            SumcheckJobs sj;
+            sj.f = sf;
+            sj.b = sb;
+            sj.f2 = sf2;
+            sj.b2 = sb2;
+            sj.bnew = bnew_stride;
+            sj.acc = ACC;
+            sj.st = 2;
+            sj.log_elems = LOG_W;
+            sj.log_words = LOG_W;
+            launch_sumcheck_fold_step<F>(d_f.p, d_b.p, flen, step, d_f2.p, d_b2.p, d_acc.p + rn, d_acc.p + rf, s, count, sj);
+            fsi ^= 1;
+            rotate();
+            std::swap(d_f, d_f2);
+            std::swap(d_b, d_b2);
+            std::swap(sf, sf2);
+            std::swap(sb, sb2);
+            flen /= 2;
         }
-        const T* yr = (const T*)(pin + P_YR) + j * yr_len;
-        pr.yr.assign(yr, yr + yr_len);
-        fs[j].absorb_elems<H>(pr.yr.data(), yr_len);
-        fq[j] = fs[j].get_distinct_queries(M1, S);
-        for (size_t i = 0; i < S; i++) h_fq[j * S + i] = fq[j][i] + j * M1;
-        const std::vector<uint64_t> nd = merkle_proof_nodes(M1, fq[j]);
-        LIG_REQUIRE(nd.size() <= S * depth1, LIG_E_ASSERT, "Merkle proof needs more nodes than queries x depth");
-        nodes1_at[j + 1] = nodes1_at[j] + nd.size();
-        for (size_t i = 0; i < nd.size(); i++) h_nidx1[nodes1_at[j] + i] = nd[i] + j * (2 * M1 - 1);
-        // the first opening (prover.rs:72-76)
-        pr.initial.n_rows = S;
-        pr.initial.n_cols = n0;
-        const T* rows = (const T*)(pin + P_ROWS) + j * S * n0;
-        pr.initial.rows.assign(rows, rows + S * n0);
-        const uint8_t* nodes = pin + P_NODES0 + nodes_at[j] * 32;
-        pr.initial.nodes.assign(nodes, nodes + (nodes_at[j + 1] - nodes_at[j]) * 32);
+        // the last triple (raw accumulators) to each job's log
+        launch_copy_blocks((uint32_t*)(d_tlog + LAST_W), LOG_W, (const uint32_t*)(d_acc.p + rc), ACC * 2, 6, count, s);
+    };
+    // after a synchronisation: per job, replay the section from its logs
+    // (checking the device's challenges), keeping its fold challenges as the
+    // next row weights, and copy level l's opening into the proof
+    auto replay_and_open = [&](int n0, int k, int l) {
+        for (size_t j = 0; j < G; j++) {
+            Proof<H>& pr = proofs[j];
+            const T* tl = (const T*)(pin + P_LOG) + j * LOG_W;
+            const T* rr = tl + n_tlog;
+            LIG_REQUIRE(*(const uint32_t*)(rr + n_r) == 0, LIG_E_ASSERT,
+                        "sumcheck: s0 + s1 != enforced sum (ligerito.rs:35,52)");
+            int ri = 0;
+            Vec<H> fold_rs;
+            auto squeeze_check = [&]() {
+                const T v = fs[j].get_field<H>();
+                LIG_REQUIRE(H::eq(v, rr[ri]), LIG_E_ASSERT, "device transcript diverged from the host transcript");
+                if (ri >= n0 - 1) fold_rs.push_back(v);
+                ri++;
+            };
+            pr.tr.insert(pr.tr.end(), tl, tl + 3);
+            fs[j].absorb_elems<H>(tl, 3);
+            for (int i = 0; i < n0; i++) squeeze_check();
+            const A* last = (const A*)(tl + LAST_W);
+            for (int r = 0; r < k; r++) {
+                const T* t3 = tl + 3 * (r + 1);
+                T last3[3];
+                if (r + 1 == k) {
+                    for (int i = 0; i < 3; i++) last3[i] = acc_reduce_host<H>(last[i]);
+                    t3 = last3;
+                }
+                pr.tr.insert(pr.tr.end(), t3, t3 + 3);
+                fs[j].absorb_elems<H>(t3, 3);
+                if (r + 1 < k) squeeze_check();
+            }
+            gr[j] = lagrange_basis<H>(fold_rs);
+            Rlp<H>* o = &pr.initial;
+            if (l > 0) {
+                pr.rec_proofs.emplace_back();
+                o = &pr.rec_proofs.back();
+            }
+            o->n_rows = S;
+            o->n_cols = nw[l];
+            const T* rows = (const T*)(pin + P_ROWS[l]) + j * S * nw[l];
+            o->rows.assign(rows, rows + S * nw[l]);
+            const uint8_t* nodes = pin + P_NODES[l] + nodes_at[j] * 32;
+            o->nodes.assign(nodes, nodes + (nodes_at[j + 1] - nodes_at[j]) * 32);
+        }
+    };
+
+    // step 0: level 0's opening and induction, the first section
+    open_and_induce(0, d_b);
+    section(1, cfg.ks[0], nullptr, 0);
+    launch_gather_nodes(wl[0][0].tree.p, (const uint64_t*)(pdev + P_NIDX[0]), nodes_at[G], (uint32_t*)(pdev + P_NODES[0]), s);
+    for (int i = 0; i < R; i++) {
+        const int n0 = i == 0 ? 1 : 2;
+        if (i == R - 1) {
+            // final step (prover.rs:104-142): yr, queries, the last level's opening
+            launch_copy_blocks((uint32_t*)(pdev + P_YR), yr_len, d_f.p, sf, yr_len, count, s);
+            {
+                const uint32_t seq = ctx->next_flag();
+                launch_signal(ctx->flag_dev(), seq, s);
+                LIG_HIP(hipGetLastError());
+                ctx->wait_flag(seq);
+            }
+            LIG_REQUIRE(flen == yr_len, LIG_E_ASSERT, "final message length");
+            replay_and_open(n0, cfg.ks[i], i);
+            uint64_t* h_fq = (uint64_t*)(pin + P_FQ);
+            uint64_t* h_nidx = (uint64_t*)(pin + P_NIDX[R]);
+            nodes_at.assign(G + 1, 0);
+            for (size_t j = 0; j < G; j++) {
+                const T* yr = (const T*)(pin + P_YR) + j * yr_len;
+                proofs[j].yr.assign(yr, yr + yr_len);
+                fs[j].absorb_elems<H>(proofs[j].yr.data(), yr_len);
+                const std::vector<size_t> q = fs[j].get_distinct_queries(Mw[R], S);
+                for (size_t k = 0; k < S; k++) h_fq[j * S + k] = q[k] + j * Mw[R];
+                const std::vector<uint64_t> nd = merkle_proof_nodes(Mw[R], q);
+                LIG_REQUIRE(nd.size() <= S * depth[R], LIG_E_ASSERT, "Merkle proof needs more nodes than queries x depth");
+                nodes_at[j + 1] = nodes_at[j] + nd.size();
+                for (size_t k = 0; k < nd.size(); k++) h_nidx[nodes_at[j] + k] = nd[k] + j * (2 * Mw[R] - 1);
+            }
+            launch_gather_rows<F>(wl[R][0].mat.p, nw[R], (const uint64_t*)(pdev + P_FQ), G * S, (T*)(pdev + P_ROWS[R]),
+                                  nullptr, s);
+            launch_gather_nodes(wl[R][0].tree.p, (const uint64_t*)(pdev + P_NIDX[R]), nodes_at[G],
+                                (uint32_t*)(pdev + P_NODES[R]), s);
+            {
+                const uint32_t seq = ctx->next_flag();
+                launch_signal(ctx->flag_dev(), seq, s);
+                LIG_HIP(hipGetLastError());
+                ctx->wait_flag(seq);
+            }
+            for (size_t j = 0; j < G; j++) {
+                Rlp<H>& fo = proofs[j].final_;
+                fo.n_rows = S;
+                fo.n_cols = nw[R];
+                const T* rows = (const T*)(pin + P_ROWS[R]) + j * S * nw[R];
+                fo.rows.assign(rows, rows + S * nw[R]);
+                const uint8_t* nodes = pin + P_NODES[R] + nodes_at[j] * 32;
+                fo.nodes.assign(nodes, nodes + (nodes_at[j + 1] - nodes_at[j]) * 32);
+            }
+            return;
+        }
+        // step i + 1 (prover.rs:144-195): commit the folded polynomial, replay
+        // section i, open level i + 1, induce and glue, the next section
+        commit_level(i + 2);
+        replay_and_open(n0, cfg.ks[i], i);
+        PBuf<T> d_bnew;
+        open_and_induce(i + 1, d_bnew);
+        const size_t bnew_stride = flen;
+        section(2, cfg.ks[i + 1], d_bnew.p, bnew_stride);
+        launch_gather_nodes(wl[i + 1][0].tree.p, (const uint64_t*)(pdev + P_NIDX[i + 1]), nodes_at[G],
+                            (uint32_t*)(pdev + P_NODES[i + 1]), s);
     }
-    // the final openings (prover.rs:128-142) from the recursive witnesses
-    launch_gather_rows<F>(w1[0].mat.p, n1, (const uint64_t*)(pdev + P_FQ), G * S, (T*)(pdev + P_FROWS), nullptr, s);
-    launch_gather_nodes(w1[0].tree.p, (const uint64_t*)(pdev + P_NIDX1), nodes1_at[G], (uint32_t*)(pdev + P_NODES1), s);
-    {
-        const uint32_t seq = ctx->next_flag();
-        launch_signal(ctx->flag_dev(), seq, s);
-        LIG_HIP(hipGetLastError());
-        ctx->wait_flag(seq);
-    }
-    for (size_t j = 0; j < G; j++) {
-        Rlp<H>& fo = proofs[j].final_;
-        fo.n_rows = S;
-        fo.n_cols = n1;
-        const T* rows = (const T*)(pin + P_FROWS) + j * S * n1;
-        fo.rows.assign(rows, rows + S * n1);
-        const uint8_t* nodes = pin + P_NODES1 + nodes1_at[j] * 32;
-        fo.nodes.assign(nodes, nodes + (nodes1_at[j + 1] - nodes1_at[j]) * 32);
-    }
+    throw Error(LIG_E_ASSERT, "Prover completed without entering final step (prover.rs:198)");
 }
 
 template struct RsPlan<H32>;

@@ -2385,7 +2385,7 @@ __global__ __launch_bounds__(256) void k_sumcheck_fold(const typename F::T* __re
             step.res_in += j * jobs.acc;
             if (step.enf) step.enf += j * jobs.acc;
             if (step.err) step.err += j * jobs.log_words;
-            if (step.bnew) step.bnew += j * jobs.b;
+            if (step.bnew) step.bnew += j * jobs.bnew;
             step.tlog += j * jobs.log_elems;
             step.rlog += j * jobs.log_elems;
         }

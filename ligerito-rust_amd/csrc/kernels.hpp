@@ -299,7 +299,8 @@ template <class F> struct FoldStep {
 // one shape (job j = blockIdx.y), in elements, accumulators, transcript
 // states and log words; zero for a single prove.
 struct SumcheckJobs {
-    size_t f = 0, b = 0, f2 = 0, b2 = 0;  // polys (b also for the glued basis)
+    size_t f = 0, b = 0, f2 = 0, b2 = 0;  // polys
+    size_t bnew = 0;                      // the basis glued in by a leading step
     size_t acc = 0;                       // res, res_clear, the triple read and the enforced-sum parts
     size_t st = 0;                        // transcript states in and out
     size_t log_elems = 0;                 // triple and challenge logs

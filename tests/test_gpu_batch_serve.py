@@ -59,9 +59,10 @@ def test_batch_equals_plain_and_oracle(O, ctx):
 
 def test_gang_proofs_equal_plain_and_oracle(O, ctx):
     # same-size jobs are proved in lockstep (one launch per kernel for the
-    # gang): gangs of 9 x 2^16, 5 x 2^19, 3 x 2^21 and 6 x 2^22 (two gangs of
-    # three, the per-gang element cap), every proof equal to the plain path's
-    sizes = [16] * 9 + [19] * 5 + [21] * 3 + [22] * 6
+    # gang, at most four jobs): 9 x 2^16 (gangs of 3), 5 x 2^19, 3 x 2^21,
+    # 6 x 2^22 and 2 x 2^24 (two recursive steps: a second commit, and the
+    # glue in the second section), every proof equal to the plain path's
+    sizes = [16] * 9 + [19] * 5 + [21] * 3 + [22] * 6 + [24] * 2
     say("gang gf32", sizes)
     polys, seeds = jobs(O, sizes, 32, 700)
     b = lh.Batch(0, 32, n_streams=2)
@@ -72,7 +73,7 @@ def test_gang_proofs_equal_plain_and_oracle(O, ctx):
         assert p.to_bytes() == plain_bytes(ctx, cfgs, L, 32, poly, seed)
     for i in (0, 8, 9):
         assert proofs[i].to_bytes() == O.prove(polys[i], sizes[i], 32, fs_seed=seeds[i])
-    assert lh.verifier(22, proofs[-1], seeds[-1]) == 1
+    assert lh.verifier(24, proofs[-1], seeds[-1]) == 1
     b.close()
 
 
