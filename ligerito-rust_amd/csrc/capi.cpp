@@ -127,7 +127,7 @@ size_t elem_bytes(int bits) { return (size_t)bits / 8; }
 // Upload a host buffer into a pooled device buffer.
 template <class T> PBuf<T> upload(Ctx& c, const void* host, size_t count) {
     PBuf<T> b(&c.pool, count);
-    LIG_HIP(hipMemcpyAsync(b.p, host, count * sizeof(T), hipMemcpyHostToDevice, c.stream));
+    c.upload(b.p, host, count * sizeof(T));
     return b;
 }
 }  // namespace

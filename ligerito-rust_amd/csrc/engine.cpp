@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <exception>
+#include <thread>
 #include <tuple>
 
 namespace lig {
@@ -63,6 +65,55 @@ void Ctx::collect() {
         (void)hipEventDestroy(mk.b);
     }
     marks.clear();
+}
+
+// A plain hipMemcpyAsync from pageable memory moves about 30 GB/s on the
+// MI355X host (the runtime stages it); from pinned memory the DMA engine
+// moves 57 GB/s, and four host threads copy pageable into pinned memory at
+// 65-100 GB/s.  So chunk c of a large copy goes to slot c % UP_SLOTS of a
+// pinned ring, filled by thread c % UP_THREADS (UP_SLOTS is a multiple of
+// UP_THREADS, so a slot always belongs to one thread), which then queues
+// the slot's DMA and records the slot's event; a slot is refilled once its
+// event, recorded after its previous DMA, has completed.  Like every use of
+// a context, one host thread at a time.  Measured with a 2^24 prove from a
+// host-resident input: 3.29 ms per call with the plain copy, 2.62 ms with
+// four 4 MiB slots (2.69 with eight 2 MiB slots, 2.65 with four 8 MiB).
+void Ctx::upload(void* dst, const void* src, size_t bytes) {
+    static_assert(UP_SLOTS % UP_THREADS == 0, "a slot belongs to one thread");
+    if (bytes < 4 * UP_SLOT_BYTES) {
+        LIG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+        return;
+    }
+    if (!up_ring) {
+        LIG_HIP(hipHostMalloc((void**)&up_ring, UP_SLOTS * UP_SLOT_BYTES, hipHostMallocDefault));
+        for (auto& e : up_ev) LIG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    const size_t chunks = (bytes + UP_SLOT_BYTES - 1) / UP_SLOT_BYTES;
+    std::exception_ptr err;
+    std::mutex err_mu;
+    auto body = [&](int t) {
+        try {
+            for (size_t c = (size_t)t; c < chunks; c += UP_THREADS) {
+                const int slot = (int)(c % UP_SLOTS);
+                uint8_t* stage = up_ring + (size_t)slot * UP_SLOT_BYTES;
+                const size_t at = c * UP_SLOT_BYTES, len = std::min(UP_SLOT_BYTES, bytes - at);
+                // the slot's previous DMA (of this copy, or of an earlier
+                // one; a never recorded event counts as complete)
+                LIG_HIP(hipEventSynchronize(up_ev[slot]));
+                memcpy(stage, (const uint8_t*)src + at, len);
+                LIG_HIP(hipMemcpyAsync((uint8_t*)dst + at, stage, len, hipMemcpyHostToDevice, stream));
+                LIG_HIP(hipEventRecord(up_ev[slot], stream));
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> g(err_mu);
+            if (!err) err = std::current_exception();
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < UP_THREADS; t++) th.emplace_back(body, t);
+    body(0);
+    for (auto& x : th) x.join();
+    if (err) std::rethrow_exception(err);
 }
 
 // ---------------------------------------------------------------- RS plan --

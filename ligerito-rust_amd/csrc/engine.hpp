@@ -145,7 +145,23 @@ struct Ctx {
     }
     // Resolves recorded event pairs into per-phase totals.
     void collect();
+    // Copies bytes from pageable host memory to device memory, ordered on
+    // the context stream; the host buffer may change once it returns.
+    // Large copies go through a ring of pinned staging slots (allocated on
+    // first use) filled by several host threads while the DMA engine moves
+    // earlier slots.
+    void upload(void* dst, const void* src, size_t bytes);
+    static constexpr int UP_SLOTS = 4, UP_THREADS = 4;
+    static constexpr size_t UP_SLOT_BYTES = (size_t)4 << 20;
+    uint8_t* up_ring = nullptr;
+    hipEvent_t up_ev[UP_SLOTS] = {};
     ~Ctx() {
+        if (up_ring) {
+            if (stream) (void)hipStreamSynchronize(stream);
+            (void)hipHostFree(up_ring);
+        }
+        for (hipEvent_t e : up_ev)
+            if (e) (void)hipEventDestroy(e);
         if (pin) (void)hipHostFree(pin);
     }
 };
