@@ -80,8 +80,17 @@ void Ctx::collect() {
 // four 4 MiB slots (2.69 with eight 2 MiB slots, 2.65 with four 8 MiB).
 void Ctx::upload(void* dst, const void* src, size_t bytes) {
     static_assert(UP_SLOTS % UP_THREADS == 0, "a slot belongs to one thread");
-    if (bytes < 4 * UP_SLOT_BYTES) {
-        LIG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    bool pinned_src = false;
+    if (bytes >= 4 * UP_SLOT_BYTES) {
+        // pinned (or device) memory goes straight to the DMA engine
+        hipPointerAttribute_t at;
+        if (hipPointerGetAttributes(&at, src) == hipSuccess)
+            pinned_src = at.type == hipMemoryTypeHost || at.type == hipMemoryTypeDevice;
+        else
+            (void)hipGetLastError();  // pageable memory is not known to the runtime
+    }
+    if (bytes < 4 * UP_SLOT_BYTES || pinned_src) {
+        LIG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, stream));
         return;
     }
     if (!up_ring) {
