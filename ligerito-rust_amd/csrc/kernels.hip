@@ -516,10 +516,8 @@ __device__ __forceinline__ uint4 u128_to4(u128v v) {
     return make_uint4((uint32_t)v.lo, (uint32_t)(v.lo >> 32), (uint32_t)v.hi, (uint32_t)(v.hi >> 32));
 }
 
-// Nibble position k (0..31) of the multiply-by-lam table at t (the layout
-// mul_tab128_lds reads): entry v = lam * v * x^(4k), from c_b = lam * x^(4k+b).
-__device__ __forceinline__ void gf128_tab_build(uint4* t, u128v lam, int k) {
-    const int s4 = 4 * k;
+// lam * x^s (0 <= s < 128): the 256-bit shift of lam, reduced.
+__device__ __forceinline__ u128v gf128_mul_xpow(u128v lam, int s4) {
     uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
     if (s4 == 0) {
         w0 = lam.lo;
@@ -537,8 +535,14 @@ __device__ __forceinline__ void gf128_tab_build(uint4* t, u128v lam, int k) {
         w2 = (lam.hi << r) | (lam.lo >> (64 - r));
         w3 = lam.hi >> (64 - r);
     }
+    return gf128_reduce256(w0, w1, w2, w3);
+}
+
+// Nibble position k (0..31) of the multiply-by-lam table at t (the layout
+// mul_tab128_lds reads): entry v = lam * v * x^(4k), from c_b = lam * x^(4k+b).
+__device__ __forceinline__ void gf128_tab_build(uint4* t, u128v lam, int k) {
     u128v c[4];
-    c[0] = gf128_reduce256(w0, w1, w2, w3);
+    c[0] = gf128_mul_xpow(lam, 4 * k);
 #pragma unroll
     for (int b = 1; b < 4; b++) {
         const uint64_t top = c[b - 1].hi >> 63;
@@ -561,16 +565,10 @@ __device__ __forceinline__ void gf128_tab_build(uint4* t, u128v lam, int k) {
 // load in flight.
 __device__ __forceinline__ void gf128_tab_build_wave(uint4* t, u128v lam, int lane) {
     const int k = lane >> 1, h = lane & 1;
-    // c[0] = lam * x^(4k) as k steps of a multiply by x^4 (the four bits
-    // shifted out fold back as t * (x^7 + x^2 + x + 1)): few registers live
-    // beside the caller's in-flight loads
+    // c[0] = lam * x^(4k): one shift and reduction (a chain of k multiplies
+    // by x^4 kept the wave for up to 31 steps)
     u128v c[4];
-    c[0] = lam;
-    for (int i = 0; i < k; i++) {
-        const uint64_t top = c[0].hi >> 60;
-        c[0].hi = (c[0].hi << 4) | (c[0].lo >> 60);
-        c[0].lo = (c[0].lo << 4) ^ (top << 7) ^ (top << 2) ^ (top << 1) ^ top;
-    }
+    c[0] = gf128_mul_xpow(lam, 4 * k);
 #pragma unroll
     for (int b = 1; b < 4; b++) {
         const uint64_t top = c[b - 1].hi >> 63;
