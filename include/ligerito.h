@@ -262,7 +262,9 @@ int lig_verifier_with_config_dev(lig_ctx *ctx, int field_bits, size_t recursive_
  * many independent proves on one device.  A batch holds n_streams workers,
  * each with its own HIP stream, device memory pool, pinned staging and prover
  * configurations (built on first use of a size), so the proves of different
- * jobs overlap on the GPU. */
+ * jobs overlap on the GPU.  GF(2^32) jobs of one size are proved in lockstep
+ * in gangs of up to four (every kernel launched once for the gang);
+ * LIG_BATCH_GANG=0 in the environment proves each job on its own. */
 typedef struct lig_batch lig_batch;
 int lig_batch_create(int device, int field_bits, int n_streams, lig_batch **out);
 void lig_batch_destroy(lig_batch *b);
