@@ -36,6 +36,7 @@ template <class H> Worker<H>::~Worker() {
     cfgs.clear();
     ctx.pool.drain();
     (void)hipStreamDestroy(ctx.stream);
+    ctx.stream = nullptr;
 }
 
 template <class H> Config<H>& Worker<H>::config(int log_n) {
@@ -57,7 +58,7 @@ template <class H> void Worker<H>::prove_one(int log_n, const void* poly, bool o
     const T* d = (const T*)poly;
     if (!on_device) {
         tmp = PBuf<T>(&ctx.pool, n);
-        LIG_HIP(hipMemcpyAsync(tmp.p, poly, n * sizeof(T), hipMemcpyHostToDevice, ctx.stream));
+        ctx.upload(tmp.p, poly, n * sizeof(T));
         d = tmp.p;
     }
     prove<H>(cfg, d, seed, out);
@@ -127,7 +128,7 @@ void Worker<H>::prove_gang_jobs(int log_n, const void* const* polys, bool on_dev
         if (!on_device) {
             tmp = PBuf<T>(&ctx.pool, n * (size_t)count);
             for (int j = 0; j < count; j++) {
-                LIG_HIP(hipMemcpyAsync(tmp.p + j * n, polys[j], n * sizeof(T), hipMemcpyHostToDevice, ctx.stream));
+                ctx.upload(tmp.p + j * n, polys[j], n * sizeof(T));
                 d[j] = tmp.p + j * n;
             }
         } else {

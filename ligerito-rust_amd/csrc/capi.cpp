@@ -36,6 +36,7 @@ void ctx_release(lig_ctx* ctx) {
     }
     ctx->c.pool.drain();
     (void)hipStreamDestroy(ctx->c.stream);
+    ctx->c.stream = nullptr;
     delete ctx;
 }
 

@@ -155,11 +155,10 @@ struct Ctx {
     static constexpr size_t UP_SLOT_BYTES = (size_t)4 << 20;
     uint8_t* up_ring = nullptr;
     hipEvent_t up_ev[UP_SLOTS] = {};
+    // The owner synchronizes and destroys the stream first (no copy from the
+    // staging ring or kernel reading pinned memory is then in flight).
     ~Ctx() {
-        if (up_ring) {
-            if (stream) (void)hipStreamSynchronize(stream);
-            (void)hipHostFree(up_ring);
-        }
+        if (up_ring) (void)hipHostFree(up_ring);
         for (hipEvent_t e : up_ev)
             if (e) (void)hipEventDestroy(e);
         if (pin) (void)hipHostFree(pin);
