@@ -395,7 +395,7 @@ def _batched(lh, torch, np, cfg24, ptr24, n_jobs, streams=12):
     b.close()
     return {
         "workload": f"{n_jobs} GF(2^32) polynomials, log2 sizes uniform in 16..22 (seed 3), own transcript seeds, "
-                    f"inputs in HBM, {streams} streams",
+                    f"inputs in HBM, {streams} streams, same-size jobs proved in lockstep gangs of up to 4",
         "coeffs": total,
         "batched_coeffs_per_s": total / t_batch,
         "plain_coeffs_per_s": total / t_plain,
@@ -439,6 +439,9 @@ def _served(lh, torch, np, n_req, rate, slots=8, budget=64 << 30):
     total = sum(1 << L for L in sizes)
     lats = np.array(lats)
     log(f"served {n_req} requests at {rate}/s: p50 {np.percentile(lats, 50):.2f} ms, p99 {np.percentile(lats, 99):.2f} ms")
+    sz = np.array(sizes)
+    by_size = {str(L): {"n": int((sz == L).sum()), "p50_ms": float(np.percentile(lats[sz == L], 50)),
+                        "p90_ms": float(np.percentile(lats[sz == L], 90))} for L in sorted(set(sizes))}
     return {
         "workload": f"{n_req} requests, Poisson arrivals at {rate:g}/s (seed 4), log2 sizes uniform in 16..24, "
                     f"GF(2^32), inputs in pinned host memory, {slots} slots, HBM budget {budget >> 30} GiB",
@@ -447,6 +450,7 @@ def _served(lh, torch, np, n_req, rate, slots=8, budget=64 << 30):
         "p50_ms": float(np.percentile(lats, 50)),
         "p99_ms": float(np.percentile(lats, 99)),
         "mean_ms": float(lats.mean()),
+        "by_log_n": by_size,
     }
 
 
