@@ -666,14 +666,14 @@ void induce_launch(Ctx* ctx, int ell, const Vec<H>& sks, const PendingOpen<H>& p
     d_basis = PBuf<T>(&ctx->pool, (size_t)1 << ell);
     {
         PhaseTimer pt(ctx, PH_INDUCE);
-        launch_induce_prep<F>(po.d_q, d_vals, d_vals + nvals - 1, (int)nq, ell, d_c.p, d_ap.p, s, &po.gather);
         EnforcedIn<F> e;
         e.rows = po.d_rows.p;
         e.ncols = (int)n_cols;
         e.gr = d_vals + sks.size();
         e.parts = d_enf;
         LIG_REQUIRE(nq <= (size_t)ENFORCED_MAX_PARTS, LIG_E_INVALID, "more queries than enforced-sum parts");
-        *n_enf = launch_induce<F>(d_c.p, d_ap.p, (int)nq, ell, d_P.p, d_K.p, d_basis.p, s, &e, &ev);
+        *n_enf = launch_induce_all<F>(po.d_q, d_vals, d_vals + nvals - 1, (int)nq, ell, d_c.p, d_ap.p, po.gather, d_P.p,
+                                      d_K.p, d_basis.p, s, e, &ev);
     }
     LIG_HIP(hipGetLastError());
 }
@@ -1187,8 +1187,6 @@ void prove_gang(Config<H>& cfg, const typename H::T* const* d_polys, const uint3
         g.out = d_rows.p;
         g.out2 = (T*)(pdev + P_ROWS[l]);
         const T* d_const = (const T*)(pdev + P_CONST);
-        launch_induce_prep<F>((const uint64_t*)(pdev + P_Q), (const T*)(pdev + P_SKS), d_const + 64, (int)S, ell, d_c.p,
-                              d_ap.p, s, &g, count, ij);
         EnforcedIn<F> e;
         e.rows = d_rows.p;
         e.ncols = (int)nw[l];
@@ -1198,7 +1196,9 @@ void prove_gang(Config<H>& cfg, const typename H::T* const* d_polys, const uint3
         ev.f = d_f.p;
         ev.res = d_acc.p + rn;
         ev.res_clear = d_acc.p + rf;
-        launch_induce<F>(d_c.p, d_ap.p, (int)S, ell, d_P.p, d_K.p, d_basis.p, s, &e, &ev, count, ij);
+        const int n_parts = launch_induce_all<F>((const uint64_t*)(pdev + P_Q), (const T*)(pdev + P_SKS), d_const + 64, (int)S,
+                                                 ell, d_c.p, d_ap.p, g, d_P.p, d_K.p, d_basis.p, s, e, &ev, count, ij);
+        LIG_REQUIRE(n_parts == (int)S, LIG_E_ASSERT, "gang prove: one enforced-sum part per query");
         LIG_HIP(hipGetLastError());
         rotate();
     };

@@ -2070,6 +2070,10 @@ __global__ __launch_bounds__(256) void k_induce_tensor32(const uint32_t* __restr
 }
 
 template <class F>
+void launch_induce_accum(const typename F::T* P, const typename F::T* K, int nq, int ell, typename F::T* basis, hipStream_t s,
+                         const Eval013In<F>* ev, int count, const InduceJobs& jobs);
+
+template <class F>
 int launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, int ell, typename F::T* P, typename F::T* K,
                   typename F::T* basis, hipStream_t s, const EnforcedIn<F>* e, const Eval013In<F>* ev, int count,
                   const InduceJobs& jobs) {
@@ -2094,19 +2098,28 @@ int launch_induce(const typename F::T* cvals, const typename F::T* ap, int nq, i
         }
         hipLaunchKernelGGL(k_induce_tensor<F>, dim3(nq), dim3(256), 0, s, cvals, ap, ell, clo, P, K);
     }
+    launch_induce_accum<F>(P, K, nq, ell, basis, s, ev, count, jobs);
+    return n_parts;
+}
+
+// The basis from the tensors (and its 0/1/3 triple with ev->f).
+template <class F>
+void launch_induce_accum(const typename F::T* P, const typename F::T* K, int nq, int ell, typename F::T* basis, hipStream_t s,
+                         const Eval013In<F>* ev, int count, const InduceJobs& jobs) {
+    const int clo = induce_low_bits(ell);
+    const size_t nk = (size_t)1 << (ell - clo);
     if constexpr (F::BITS == 32) {
         // 1024 lanes per X row: 2^clo entries times 1024 / 2^clo query slices
         // the triple pairs rows X and X + nk/2; P is addressed with 32-bit offsets
         if (clo >= 6 && clo <= 10 && (!ev || nk >= 2) && ((size_t)nq << (clo + 2)) < ((size_t)1 << 31)) {
             hipLaunchKernelGGL(k_induce_accum32, dim3((unsigned)nk, (unsigned)count), dim3(1024), 0, s, P, K, nq, ell, clo, basis,
                                ev ? ev->f : nullptr, ev ? ev->res : nullptr, ev ? ev->res_clear : nullptr, jobs);
-            return n_parts;
+            return;
         }
     }
     if (count != 1) throw std::invalid_argument("launch_induce: several jobs need the GF(2^32) kernels");
     hipLaunchKernelGGL(k_induce_accum<F>, dim3((unsigned)nk), dim3(256), 0, s, P, K, nq, ell, clo, basis);
     if (ev) launch_eval013<F>(ev->f, basis, (size_t)1 << ell, ev->res, ev->res_clear, s);
-    return n_parts;
 }
 
 template <class F>
@@ -2236,6 +2249,128 @@ void launch_induce_prep(const uint64_t* q, const typename F::T* sks, const typen
     if (count != 1) throw std::invalid_argument("launch_induce_prep: several jobs need the GF(2^32) kernel");
     if (g) launch_gather_rows<F>(g->mat, g->n, q, (size_t)nq, g->out, g->out2, s);
     hipLaunchKernelGGL(k_induce_prep<F>, dim3((nq + 63) / 64), dim3(64), 0, s, q, sks, alpha, nq, ell, cvals, ap);
+}
+
+// GF(2^32): the query constants and the tensors of one query per
+// workgroup (k_induce_prep32 and k_induce_tensor32 in one launch): the
+// workgroup reads its query index, sks and alpha (host memory), builds the
+// squaring, sks and alpha^(2^b) tables, runs the query's next_s chain on
+// wave 0 and alpha^i on wave 1 while wave 2 gathers the opened row (to out
+// and out2) and its dot product with gr; then the part of the enforced sum,
+// the tables of the chain's values and the P / K tensors by doubling.
+__global__ __launch_bounds__(256) void k_induce_pt32(const uint64_t* __restrict__ q, const uint32_t* __restrict__ sks,
+                                                     const uint32_t* __restrict__ alpha, int ell, int clo,
+                                                     const uint32_t* __restrict__ mat, int n, uint32_t* __restrict__ out,
+                                                     uint32_t* __restrict__ out2, const uint32_t* __restrict__ gr,
+                                                     uint64_t* __restrict__ parts, uint32_t* __restrict__ P,
+                                                     uint32_t* __restrict__ Kt, InduceJobs jobs) {
+    // tables 0: squaring, 1..ell-1: times sks[t-1], ell..ell+7: times
+    // alpha^(2^b); later 0..ell-1: times the chain's values c_t
+    __shared__ __attribute__((aligned(256))) uint32_t tabs[(PREP_MAXELL + 8) * 128];
+    __shared__ uint32_t cbuf[PREP_MAXELL];
+    __shared__ uint32_t ap_sh;
+    __shared__ uint32_t pb[1024];
+    __shared__ uint32_t kb[TENSOR_MAXK];
+    const int i = blockIdx.x, tid = threadIdx.x, nq = (int)gridDim.x;
+    {
+        const size_t j = blockIdx.y;
+        q += j * jobs.q;
+        alpha += j * jobs.alpha;
+        mat += j * jobs.mat;
+        out += j * (size_t)nq * n;
+        out2 += j * jobs.rows2;
+        gr += j * jobs.gr;
+        parts += j * jobs.parts;
+        P += j * ((size_t)nq << clo);
+        Kt += j * ((size_t)nq << (ell - clo));
+    }
+    const uint32_t qi = (uint32_t)q[i];
+    const uint32_t a0 = alpha[0];
+    for (int task = tid; task < (ell + 8) * 8; task += blockDim.x) {
+        const int tbl = task >> 3, k = task & 7;
+        if (tbl == 0) {
+            gf32_tab_build_sqr(tabs, k);
+        } else if (tbl < ell) {
+            gf32_tab_build(tabs + tbl * 128, sks[tbl - 1], k);
+        } else {
+            uint32_t a = a0;
+            for (int b = 0; b < tbl - ell; b++) a = Ops<F32>::mul(a, a);
+            gf32_tab_build(tabs + tbl * 128, a, k);
+        }
+    }
+    // wave 2: the opened row (query index qi) and its dot product with gr
+    uint32_t dot = 0u;
+    if (tid >= 128 && tid < 192) {
+        for (int c = tid - 128; c < n; c += 64) {
+            const uint32_t v = mat[(size_t)qi * n + c];
+            out[(size_t)i * n + c] = v;
+            if (out2) out2[(size_t)i * n + c] = v;
+            if (parts) dot ^= Ops<F32>::mul(v, gr[c]);
+        }
+        dot = wave_xor32(dot);
+    }
+    __syncthreads();
+    if (tid < 64) {
+        // next_s (binary_fft.rs:100-107): c_t = c_{t-1}^2 + sks[t-1] * c_{t-1}
+        uint32_t x = qi;
+        if (tid == 0) cbuf[0] = x;
+        for (int t = 1; t < ell; t++) {
+            x = mul_tab_lds(tabs, 0u, x) ^ mul_tab_lds(tabs, (uint32_t)(t * 512), x);
+            if (tid == 0) cbuf[t] = x;
+        }
+    } else if (tid < 128) {
+        uint32_t r = 1u;
+        for (int b = 0; b < 8; b++)
+            if ((i >> b) & 1) r = mul_tab_lds(tabs, (uint32_t)((ell + b) * 512), r);
+        if (tid == 64) ap_sh = r;
+    }
+    __syncthreads();
+    const uint32_t api = ap_sh;
+    if (parts && tid == 128) {
+        uint64_t acc = 0;
+        Ops<F32>::acc_mul(acc, dot, api);
+        parts[i] = acc;
+    }
+    for (int task = tid; task < ell * 8; task += blockDim.x) gf32_tab_build(tabs + (task >> 3) * 128, cbuf[task >> 3], task & 7);
+    if (tid == 0) {
+        pb[0] = 1u;
+        kb[0] = api;
+    }
+    __syncthreads();
+    for (int t = 0; t < clo; t++) {
+        for (int x = tid; x < (1 << t); x += blockDim.x) pb[x + (1 << t)] = mul_tab_lds(tabs, (uint32_t)(t * 512), pb[x]);
+        __syncthreads();
+    }
+    const int hk = ell - clo;
+    for (int t = 0; t < hk; t++) {
+        for (int x = tid; x < (1 << t); x += blockDim.x) kb[x + (1 << t)] = mul_tab_lds(tabs, (uint32_t)((clo + t) * 512), kb[x]);
+        __syncthreads();
+    }
+    const size_t np = (size_t)1 << clo, nk = (size_t)1 << hk;
+    for (size_t x = tid; x < np; x += blockDim.x) P[(size_t)i * np + x] = pb[x];
+    for (size_t x = tid; x < nk; x += blockDim.x) Kt[(size_t)i * nk + x] = kb[x];
+}
+
+// The whole induction of one step (sumcheck_polys.rs:74-160): query
+// constants, opened rows, tensors, basis and its triple with f.
+template <class F>
+int launch_induce_all(const uint64_t* q, const typename F::T* sks, const typename F::T* alpha, int nq, int ell,
+                      typename F::T* cvals, typename F::T* ap, const RowGather<F>& g, typename F::T* P, typename F::T* K,
+                      typename F::T* basis, hipStream_t s, const EnforcedIn<F>& e, const Eval013In<F>* ev, int count,
+                      const InduceJobs& jobs) {
+    if constexpr (F::BITS == 32) {
+        const int clo = induce_low_bits(ell);
+        const size_t nk = (size_t)1 << (ell - clo);
+        if (nq >= 1 && nq <= 256 && nq <= ENFORCED_MAX_PARTS && ell <= PREP_MAXELL && nk <= (size_t)TENSOR_MAXK &&
+            clo <= 10 && g.n >= 1 && g.n <= 4096 && e.rows == g.out && e.ncols == (int)g.n && count >= 1 && count <= 65535) {
+            hipLaunchKernelGGL(k_induce_pt32, dim3((unsigned)nq, (unsigned)count), dim3(256), 0, s, q, sks, alpha, ell, clo,
+                               g.mat, (int)g.n, g.out, g.out2, e.gr, e.parts, P, K, jobs);
+            launch_induce_accum<F>(P, K, nq, ell, basis, s, ev, count, jobs);
+            return nq;
+        }
+    }
+    launch_induce_prep<F>(q, sks, alpha, nq, ell, cvals, ap, s, &g, count, jobs);
+    return launch_induce<F>(cvals, ap, nq, ell, P, K, basis, s, &e, ev, count, jobs);
 }
 
 // Partial enforced sums: workgroup g XOR-accumulates the products of its
@@ -2744,6 +2879,9 @@ double bench_rate(int which, uint32_t* scratch_word, hipStream_t s) {
                                                F::Acc*, hipStream_t, int, const SumcheckJobs&);                             \
     template void launch_induce_prep<F>(const uint64_t*, const F::T*, const F::T*, int, int, F::T*, F::T*, hipStream_t,    \
                                         const RowGather<F>*, int, const InduceJobs&);                                       \
+    template int launch_induce_all<F>(const uint64_t*, const F::T*, const F::T*, int, int, F::T*, F::T*,                  \
+                                      const RowGather<F>&, F::T*, F::T*, F::T*, hipStream_t, const EnforcedIn<F>&,         \
+                                      const Eval013In<F>*, int, const InduceJobs&);                                        \
     template void launch_enforced<F>(const F::T*, int, int, const F::T*, const F::T*, F::Acc*, hipStream_t);
 
 LIG_INST(F32)

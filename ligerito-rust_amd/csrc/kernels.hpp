@@ -252,6 +252,15 @@ void launch_induce_prep(const uint64_t* q, const typename F::T* sks, const typen
                         int count = 1, const InduceJobs& jobs = InduceJobs());
 // The enforced sum of sumcheck_polys.rs:249-256, sum_i ap[i] * (row_i . gr)
 // (unreduced), as enforced_parts(nrows, ncols) parts whose XOR is the sum.
+// The whole induction (launch_induce_prep then launch_induce, or for
+// GF(2^32) one launch for the query constants, opened rows, enforced-sum
+// parts and tensors, then the accumulation): the opened rows go to g.out
+// (and g.out2), which e.rows must name.  Returns the number of parts.
+template <class F>
+int launch_induce_all(const uint64_t* q, const typename F::T* sks, const typename F::T* alpha, int nq, int ell,
+                      typename F::T* cvals, typename F::T* ap, const RowGather<F>& g, typename F::T* P, typename F::T* K,
+                      typename F::T* basis, hipStream_t s, const EnforcedIn<F>& e, const Eval013In<F>* ev, int count = 1,
+                      const InduceJobs& jobs = InduceJobs());
 constexpr int ENFORCED_MAX_PARTS = 256;
 int enforced_parts(int nrows, int ncols);
 template <class F>
