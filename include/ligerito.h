@@ -294,6 +294,15 @@ int lig_serve_submit(lig_server *s, int log_n, const void *poly, uint32_t fs_see
 int lig_serve_poll(lig_server *s, uint64_t ticket, lig_proof **out, double *latency_ms);
 /* As lig_serve_poll, waiting for completion. */
 int lig_serve_wait(lig_server *s, uint64_t ticket, lig_proof **out, double *latency_ms);
+/* Prepares every slot for requests of the count sizes in log_n: builds its
+ * prover configuration and caches the device blocks one prove of each size
+ * takes (one prove of a zero polynomial per slot and size), so that the
+ * first requests of those sizes pay for neither.  Waits for the running
+ * requests to finish and admits no queued request until it returns; stops
+ * at the first slot and size whose memory would exceed hbm_budget.  Sets
+ * *warmed (may be NULL) to the number of (slot, size) pairs prepared.
+ * No counterpart in the reference (it has no server). */
+int lig_serve_warm(lig_server *s, const int *log_n, size_t count, size_t *warmed);
 /* Finishes the running requests, fails the queued ones, frees the server. */
 void lig_serve_stop(lig_server *s);
 #ifdef __cplusplus

@@ -6,6 +6,7 @@
 // device-memory pool, pinned staging and cached prover configurations, so
 // the kernels of different proofs overlap on the GPU while any one proof
 // waits on its host-side transcript steps.
+#include <algorithm>
 #include <atomic>
 #include <cstdlib>
 #include <chrono>
@@ -229,6 +230,7 @@ template <class H> Server<H>::Server(int dev, int n_slots, uint64_t budget) : de
     for (int i = 0; i < n_slots; i++) workers.emplace_back(new Worker<H>(dev));
     busy.assign(n_slots, 0);
     charge.assign(n_slots, 0);
+    warm_todo.assign(n_slots, 0);
     for (int i = 0; i < n_slots; i++) threads.emplace_back([this, i] { loop((size_t)i); });
 }
 
@@ -265,6 +267,110 @@ template <class H> uint64_t Server<H>::submit(int log_n, const void* poly, uint3
     queue.push_back(t);
     cv.notify_all();
     return t;
+}
+
+template <class H> size_t Server<H>::warm(const int* log_ns, size_t count) {
+    using T = typename H::T;
+    LIG_REQUIRE(log_ns || count == 0, LIG_E_INVALID, "NULL size list");
+    std::vector<int> sizes(log_ns, log_ns + count);
+    for (int L : sizes) LIG_REQUIRE(L >= 16 && L <= 27, LIG_E_INVALID, "log_n must be in [16, 27]");
+    if (sizes.empty()) return 0;
+    std::sort(sizes.begin(), sizes.end());
+    sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+    // one zero input of the largest size, in pinned host memory as the
+    // requests that matter most for latency bring theirs
+    const size_t bytes = ((size_t)1 << sizes.back()) * sizeof(T);
+    LIG_HIP(hipSetDevice(device));
+    void* input = nullptr;
+    LIG_HIP(hipHostMalloc(&input, bytes, hipHostMallocDefault));
+    memset(input, 0, bytes);
+    size_t warmed = 0;
+    int code = 0;
+    std::string msg;
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        if (stopping || warming) {
+            lk.unlock();
+            (void)hipHostFree(input);
+            LIG_REQUIRE(false, LIG_E_INVALID, stopping ? "server is stopping" : "server is already warming");
+        }
+        warming = true;
+        // running requests finish; queued ones wait until the warm-up ends
+        cv.wait(lk, [&] { return std::none_of(busy.begin(), busy.end(), [](char b) { return b != 0; }); });
+        warm_sizes = sizes;
+        warm_input = input;
+        warm_todo.assign(workers.size(), 1);
+        warm_left = workers.size();
+        warm_done = 0;
+        warm_full = false;
+        warm_code = 0;
+        warm_msg.clear();
+        cv.notify_all();
+        done_cv.wait(lk, [&] { return warm_left == 0; });
+        warming = false;
+        warm_input = nullptr;
+        warmed = warm_done;
+        code = warm_code;
+        msg = warm_msg;
+    }
+    cv.notify_all();
+    (void)hipHostFree(input);
+    if (code) throw Error(code, msg);
+    return warmed;
+}
+
+template <class H> void Server<H>::warm_worker(size_t wi) {
+    Worker<H>& w = *workers[wi];
+    size_t n = 0;
+    int code = 0;
+    std::string msg;
+    for (int L : warm_sizes) {
+        const size_t fp = prove_footprint(H::BYTES * 8, L);
+        {
+            std::lock_guard<std::mutex> g(mu);
+            if (warm_full || charged() + fp > hbm_budget) {
+                warm_full = true;
+                break;
+            }
+            charge[wi] = w.retained() + fp;
+            busy[wi] = 1;
+        }
+        try {
+            Proof<H> proof;
+            w.prove_one(L, warm_input, false, 0, proof);
+            n++;
+        } catch (const Error& e) {
+            code = e.code;
+            msg = e.what();
+        } catch (const std::bad_alloc&) {
+            code = LIG_E_OOM;
+            msg = "host allocation failed";
+        } catch (const std::exception& e) {
+            code = LIG_E_ASSERT;
+            msg = e.what();
+        }
+        {
+            std::lock_guard<std::mutex> g(mu);
+            busy[wi] = 0;
+            charge[wi] = 0;
+        }
+        if (code) break;
+    }
+    {
+        std::lock_guard<std::mutex> g(mu);
+        // what the worker keeps must fit beside what the others hold
+        if (charged() > hbm_budget) {
+            w.trim(-1);
+            warm_full = true;
+        }
+        warm_done += n;
+        if (code && !warm_code) {
+            warm_code = code;
+            warm_msg = msg;
+        }
+        warm_left--;
+    }
+    done_cv.notify_all();
 }
 
 template <class H> size_t Server<H>::charged() const {
@@ -321,7 +427,15 @@ template <class H> void Server<H>::loop(size_t wi) {
         {
             std::unique_lock<std::mutex> lk(mu);
             // first come, first admitted: the head waits until it fits
-            cv.wait(lk, [&] { return stopping || (!queue.empty() && admit(wi, reqs[queue.front()])); });
+            cv.wait(lk, [&] {
+                return stopping || warm_todo[wi] || (!warming && !queue.empty() && admit(wi, reqs[queue.front()]));
+            });
+            if (warm_todo[wi]) {
+                warm_todo[wi] = 0;
+                lk.unlock();
+                warm_worker(wi);
+                continue;
+            }
             if (stopping && queue.empty()) return;
             if (queue.empty()) continue;
             t = queue.front();

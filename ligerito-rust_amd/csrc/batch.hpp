@@ -66,6 +66,16 @@ template <class H> struct Server {
     std::unordered_map<uint64_t, Req> reqs;
     uint64_t last_ticket = 0;
     bool stopping = false;
+    // set while warm() runs: workers admit no request
+    bool warming = false;
+    // the warm-up's sizes, input, per-worker tasks and outcome (under mu)
+    std::vector<int> warm_sizes;
+    const void* warm_input = nullptr;
+    std::vector<char> warm_todo;
+    size_t warm_left = 0, warm_done = 0;
+    bool warm_full = false;
+    int warm_code = 0;
+    std::string warm_msg;
     std::vector<std::unique_ptr<Worker<H>>> workers;
     // per worker: running a request, and the device bytes charged for it
     // (what the worker held when the request started plus the request's
@@ -77,6 +87,12 @@ template <class H> struct Server {
     Server(int dev, int n_slots, uint64_t budget);
     ~Server();
     uint64_t submit(int log_n, const void* poly, uint32_t seed);
+    // Each worker, on its own thread, proves a zero input of each size as a
+    // request from pinned host memory would be proved (configuration, pool
+    // blocks, staging, upload path), while no request runs; a worker stops
+    // at the first size the budget cannot hold.  Returns the number of
+    // (worker, size) pairs warmed.
+    size_t warm(const int* log_ns, size_t count);
     // 1 with the proof when done, 0 while pending (wait == false), or the
     // request's negative status; a finished ticket is released.
     int take(uint64_t ticket, bool wait, Proof<H>& out, double* latency_ms, std::string& msg);
@@ -89,6 +105,7 @@ private:
     // idle workers' caches when that makes it fit; sets charge[wi].
     bool admit(size_t wi, const Req& r);
     void loop(size_t wi);
+    void warm_worker(size_t wi);
 };
 
 }  // namespace lig

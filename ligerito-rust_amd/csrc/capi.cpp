@@ -883,6 +883,15 @@ int lig_serve_wait(lig_server* s, uint64_t ticket, lig_proof** out, double* late
     return serve_take(s, ticket, true, out, latency_ms);
 }
 
+int lig_serve_warm(lig_server* s, const int* log_n, size_t count, size_t* warmed) {
+    return guarded([&] {
+        LIG_REQUIRE(s, LIG_E_INVALID, "NULL server");
+        const size_t w = s->bits == 32 ? s->s32->warm(log_n, count) : s->s128->warm(log_n, count);
+        if (warmed) *warmed = w;
+        return LIG_OK;
+    });
+}
+
 void lig_serve_stop(lig_server* s) { delete s; }
 
 }  // extern "C"

@@ -112,6 +112,7 @@ def lib():
             "lig_serve_submit": (I, [P, I, P, U32, C.POINTER(C.c_uint64)]),
             "lig_serve_poll": (I, [P, C.c_uint64, PP, C.POINTER(C.c_double)]),
             "lig_serve_wait": (I, [P, C.c_uint64, PP, C.POINTER(C.c_double)]),
+            "lig_serve_warm": (I, [P, C.POINTER(C.c_int), SZ, C.POINTER(SZ)]),
             "lig_serve_stop": (None, [P]),
         }
         for name, (res, args) in sig.items():
@@ -626,6 +627,16 @@ class Server(_Handle):
         _check(lib().lig_serve_submit(self._h, log_n, C.c_void_p(ptr), fs_seed, C.byref(t)))
         self._keep[t.value] = keep
         return t.value
+
+    def warm(self, log_ns) -> int:
+        """Prepares every slot for requests of these sizes (configurations and
+        cached device blocks, one zero-input prove per slot and size) while no
+        request runs; returns the number of (slot, size) pairs prepared."""
+        sizes = [int(x) for x in log_ns]
+        arr = (C.c_int * max(1, len(sizes)))(*sizes)
+        n = C.c_size_t(0)
+        _check(lib().lig_serve_warm(self._h, arr, len(sizes), C.byref(n)))
+        return n.value
 
     def _take(self, fn, ticket):
         h = C.c_void_p()

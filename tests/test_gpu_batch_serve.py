@@ -139,6 +139,32 @@ def test_served_results_same_alone_and_in_mix(O, ctx):
         assert p.to_bytes() == plain_bytes(ctx, cfgs, L, 32, poly, seed)
 
 
+def test_served_warm_keeps_results(O, ctx):
+    # warm-up called while requests run and wait: it lets the running ones
+    # finish, holds the queued ones back, prepares every slot for every size,
+    # and what the slots prove afterwards is the plain prover's proof
+    sizes = [18, 17, 18, 16, 17, 18]
+    say("serve warm", sizes)
+    polys, seeds = jobs(O, sizes, 32, 1100)
+    srv = lh.Server(0, 32, n_slots=3, hbm_budget=8 << 30)
+    early = [srv.submit(L, p, s) for L, p, s in zip(sizes, polys, seeds)]
+    assert srv.warm([18, 16, 17, 16]) == 3 * 3
+    late = [srv.submit(L, p, s) for L, p, s in zip(sizes, polys, seeds)]
+    results = [srv.wait(t) for t in early + late]
+    srv.stop()
+    cfgs = {}
+    for (L, poly, seed), (p, _) in zip(list(zip(sizes, polys, seeds)) * 2, results):
+        assert p.to_bytes() == plain_bytes(ctx, cfgs, L, 32, poly, seed)
+    # a budget that holds no prove warms nothing and still serves
+    srv = lh.Server(0, 32, n_slots=2, hbm_budget=1)
+    assert srv.warm([16]) == 0
+    p, _ = srv.wait(srv.submit(16, polys[3], seeds[3]))
+    srv.stop()
+    assert p.to_bytes() == plain_bytes(ctx, cfgs, 16, 32, polys[3], seeds[3])
+    with pytest.raises(lh.LigeritoError):
+        lh.Server(0, 32, n_slots=1).warm([15])
+
+
 def test_served_tiny_budget_runs_one_at_a_time(O, ctx):
     sizes = [16, 17, 16]
     say("serve tiny budget", sizes)
