@@ -300,7 +300,10 @@ int lig_serve_wait(lig_server *s, uint64_t ticket, lig_proof **out, double *late
  * first requests of those sizes pay for neither.  Waits for the running
  * requests to finish and admits no queued request until it returns; stops
  * at the first slot and size whose memory would exceed hbm_budget.  Sets
- * *warmed (may be NULL) to the number of (slot, size) pairs prepared.
+ * *warmed (may be NULL) to the number of (slot, size) pairs prepared.  The
+ * zero input is a pinned host buffer of the largest size, held for the
+ * call.  A second call while one runs fails with LIG_E_INVALID; a call
+ * must not overlap lig_serve_stop.
  * No counterpart in the reference (it has no server). */
 int lig_serve_warm(lig_server *s, const int *log_n, size_t count, size_t *warmed);
 /* Finishes the running requests, fails the queued ones, frees the server. */
