@@ -163,6 +163,13 @@ def test_served_warm_keeps_results(O, ctx):
     assert p.to_bytes() == plain_bytes(ctx, cfgs, 16, 32, polys[3], seeds[3])
     with pytest.raises(lh.LigeritoError):
         lh.Server(0, 32, n_slots=1).warm([15])
+    # GF(2^128) slots
+    p128, s128 = jobs(O, [16], 128, 1200)
+    srv = lh.Server(0, 128, n_slots=2, hbm_budget=8 << 30)
+    assert srv.warm([16]) == 2
+    p, _ = srv.wait(srv.submit(16, p128[0], s128[0]))
+    srv.stop()
+    assert p.to_bytes() == plain_bytes(ctx, cfgs, 16, 128, p128[0], s128[0])
 
 
 def test_served_tiny_budget_runs_one_at_a_time(O, ctx):

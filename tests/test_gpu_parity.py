@@ -178,6 +178,21 @@ def test_prover_bytes_match_oracle(O, ctx, bits, L):
         assert hashlib.sha256(got).hexdigest() == g["L20_seed1_gf32"]["sha256"]
 
 
+@pytest.mark.parametrize("L", [26, 27])
+def test_prover_largest_sizes(O, ctx, L):
+    # the largest generated configurations (2^27 coefficients is the
+    # library's limit): proof bytes equal the oracle's, both verifiers accept
+    say("prove largest", L)
+    cfg = lh.ProverConfig(ctx, L, 32)
+    poly = col(O, 3, 1 << L, 32)
+    p = lh.prover(cfg, poly)
+    got = p.to_bytes()
+    assert lh.verifier(L, p) == 1
+    assert lh.verifier_dev(ctx, L, p) == 1
+    say("prove largest", L, ": oracle")
+    assert got == O.prove(poly, L, 32)
+
+
 def test_prover_seeded_transcripts(O, ctx):
     cfg = lh.ProverConfig(ctx, 16, 32)
     poly = col(O, 9, 1 << 16, 32)
