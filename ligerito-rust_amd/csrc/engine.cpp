@@ -364,11 +364,11 @@ void commit(RsPlan<H>& rs, const typename H::T* d_poly, size_t m, size_t n, Witn
     {
         PhaseTimer pt(ctx, PH_MERKLE);
         PBuf<uint32_t> scratch(&ctx->pool, leaf_hash_scratch_words(M, n * sizeof(T)));
-        launch_leaf_hash<F>(w.mat.p, M, n, w.tree.p, scratch.n ? scratch.p : nullptr, s);
         uint8_t* pin = ctx->pinned(256);
         (void)pin;
         seq = ctx->next_flag();
-        launch_merkle_levels(w.tree.p, M, s, (uint32_t*)ctx->pin_dev, ctx->flag_dev(), seq);
+        launch_merkle_commit<F>(w.mat.p, M, n, w.tree.p, scratch.n ? scratch.p : nullptr, s, (uint32_t*)ctx->pin_dev,
+                                ctx->flag_dev(), seq);
     }
     LIG_HIP(hipGetLastError());
     ctx->wait_flag(seq);  // the last tree kernel wrote the root to pinned offset 0, then the flag
@@ -411,8 +411,8 @@ void commit_group(RsPlan<H>& rs, const typename H::T* const* d_polys, int count,
         launch_transpose_scale<F>(d_polys[0], m, n, rs.d_pis.p, rs.d_pis_lo.p, rs.d_pis_hi.p, msg.p, s, d_ptrs.p, count);
         launch_fft_quarters<F>(msg.p, g->mat.p, m, n, rs.d_tw.p, rs.d_tw_tabs.p, s, nullptr, tune, count);
         PBuf<uint32_t> scratch(&ctx->pool, leaf_hash_scratch_words(M, n * sizeof(T)));
-        launch_leaf_hash<F>(g->mat.p, (size_t)count * M, n, g->tree.p, scratch.n ? scratch.p : nullptr, s, count, tstride);
-        launch_merkle_levels(g->tree.p, M, s, roots_dev, nullptr, 0, count, tstride);
+        launch_merkle_commit<F>(g->mat.p, (size_t)count * M, n, g->tree.p, scratch.n ? scratch.p : nullptr, s, roots_dev, nullptr,
+                                0, count, tstride);
         const uint32_t seq = ctx->next_flag();
         launch_signal(ctx->flag_dev(), seq, s);
         LIG_HIP(hipGetLastError());

@@ -1416,12 +1416,18 @@ __device__ __forceinline__ uint32_t rotr(uint32_t x, int n) { return (x >> n) | 
 // levels does not wait for the global stores.  With a flag (the one
 // workgroup of the top launch), once every node is written the kernel stores
 // seq there (host memory; system scope, after the nodes and the root).
+// With rows, the input layer is a matrix of 64-byte rows read as pairs of
+// 32-byte nodes: its first level are the leaves (BLAKE3 of a 64-byte row is
+// one block with a parent's flags), written at node 0 of the tree (in_off is
+// then minus n_in, modulo 2^64).
 __global__ __launch_bounds__(1024) void k_merkle_subtree4(uint32_t* __restrict__ tree, size_t in_off, size_t n_in, int g,
                                                           uint32_t* __restrict__ root_out, uint32_t* __restrict__ flag,
-                                                          uint32_t seq, size_t tree_stride) {
+                                                          uint32_t seq, size_t tree_stride,
+                                                          const uint32_t* __restrict__ rows, size_t rows_stride) {
     using namespace b3x4;
     // one tree per blockIdx.z (a flag is only given for a single tree)
     tree += blockIdx.z * tree_stride;
+    if (rows) rows += blockIdx.z * rows_stride;
     if (root_out) root_out += blockIdx.z * 8;
     // the two layers live at word 0 and word 8 << g; picked by offset (a
     // pointer array indexed at run time would turn the LDS accesses into
@@ -1441,7 +1447,7 @@ __global__ __launch_bounds__(1024) void k_merkle_subtree4(uint32_t* __restrict__
     const uint32_t iv_hi = k == 0 ? b3::iv(4) : k == 1 ? b3::iv(5) : k == 2 ? b3::iv(6) : b3::iv(7);
     const uint32_t d_init = k == 2 ? 64u : k == 3 ? (uint32_t)(b3::CHUNK_START | b3::CHUNK_END | b3::ROOT) : 0u;
     {
-        const uint4* src = reinterpret_cast<const uint4*>(tree + (in_off + base) * 8);
+        const uint4* src = reinterpret_cast<const uint4*>(rows ? rows + base * 8 : tree + (in_off + base) * 8);
         uint4* dst = reinterpret_cast<uint4*>(sub4_lds);
         for (int i = threadIdx.x; i < (2 << g); i += blockDim.x) dst[i] = src[i];
     }
@@ -1558,6 +1564,9 @@ constexpr size_t MERKLE_WIDE = (size_t)1 << 17;
 // 14.2 against (7, 5) 14.5.
 constexpr int SUB4_MAXG = 10;
 
+static void merkle_tops(uint32_t* tree, size_t in_off, size_t nodes, hipStream_t s, uint32_t* root_out, uint32_t* flag,
+                        uint32_t seq, unsigned z, size_t tree_stride, const uint32_t* rows, size_t rows_stride);
+
 void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out, uint32_t* flag, uint32_t seq,
                           int trees, size_t tree_stride) {
     if (trees < 1 || trees > 65535 || (trees > 1 && flag)) throw std::invalid_argument("merkle levels: trees 1..65535, no flag for several");
@@ -1575,6 +1584,14 @@ void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32
         in_off += nodes;
         nodes = n_out;
     }
+    merkle_tops(tree, in_off, nodes, s, root_out, flag, seq, z, tree_stride, nullptr, 0);
+}
+
+// the levels from a layer of fewer than 2 * MERKLE_WIDE nodes up to the root
+// (k_merkle_subtree4 launches); with rows, the layer is the matrix's rows
+// (see the kernel)
+static void merkle_tops(uint32_t* tree, size_t in_off, size_t nodes, hipStream_t s, uint32_t* root_out, uint32_t* flag,
+                        uint32_t seq, unsigned z, size_t tree_stride, const uint32_t* rows, size_t rows_stride) {
     int lg = 0;
     while (((size_t)1 << lg) < nodes) lg++;
     const int launches = lg < 4 ? 1 : (lg + SUB4_MAXG - 1) / SUB4_MAXG < 2 ? 2 : (lg + SUB4_MAXG - 1) / SUB4_MAXG;
@@ -1588,12 +1605,28 @@ void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32
         const size_t lds = (size_t)64 << g;  // two layers of 2^g nodes
         const unsigned thr = g >= 9 ? 1024u : g >= 5 ? (2u << g) : 64u;
         hipLaunchKernelGGL(k_merkle_subtree4, dim3((unsigned)groups, 1, z), dim3(thr), lds, s, tree, in_off, nodes, g,
-                           root_out, top ? flag : nullptr, seq, tree_stride);
+                           root_out, top ? flag : nullptr, seq, tree_stride, li == 0 ? rows : nullptr, rows_stride);
         for (int l = 0; l < g; l++) {
             in_off += nodes;
             nodes >>= 1;
         }
     }
+}
+
+template <class F>
+void launch_merkle_commit(const typename F::T* mat, size_t rows, size_t n, uint32_t* tree, uint32_t* scratch, hipStream_t s,
+                          uint32_t* root_out, uint32_t* flag, uint32_t seq, int trees, size_t tree_stride) {
+    if (trees < 1 || trees > 65535 || (trees > 1 && flag) || rows % (size_t)trees != 0)
+        throw std::invalid_argument("merkle commit: trees 1..65535 splitting the rows evenly, no flag for several");
+    const size_t per = rows / (size_t)trees;
+    if (n * sizeof(typename F::T) == 64 && per >= 1 && per < MERKLE_WIDE && (per & (per - 1)) == 0) {
+        // 64-byte rows: the leaves are the first level of the top launches
+        merkle_tops(tree, (size_t)0 - 2 * per, 2 * per, s, root_out, flag, seq, (unsigned)trees, tree_stride,
+                    reinterpret_cast<const uint32_t*>(mat), per * 16);
+        return;
+    }
+    launch_leaf_hash<F>(mat, rows, n, tree, scratch, s, trees, tree_stride);
+    launch_merkle_levels(tree, per, s, root_out, flag, seq, trees, tree_stride);
 }
 
 // One wave: the first extra.n lanes copy extra words, then lane 0 stores
@@ -2869,6 +2902,8 @@ double bench_rate(int which, uint32_t* scratch_word, hipStream_t s) {
     template void launch_fft_quarters<F>(const F::T*, F::T*, size_t, size_t, const F::T*, const F::T*, hipStream_t,         \
                                          PassObserver*, PassTune, int);                                                      \
     template void launch_leaf_hash<F>(const F::T*, size_t, size_t, uint32_t*, uint32_t*, hipStream_t, int, size_t);                   \
+    template void launch_merkle_commit<F>(const F::T*, size_t, size_t, uint32_t*, uint32_t*, hipStream_t, uint32_t*, uint32_t*,     \
+                                          uint32_t, int, size_t);                                                                  \
     template void launch_fold<F>(const F::T*, size_t, int, const F::T*, F::T*, hipStream_t);                                \
     template void launch_gather_rows<F>(const F::T*, size_t, const uint64_t*, size_t, F::T*, F::T*, hipStream_t);                  \
     template int launch_induce<F>(const F::T*, const F::T*, int, int, F::T*, F::T*, F::T*, hipStream_t,                     \

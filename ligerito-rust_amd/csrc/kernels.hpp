@@ -130,6 +130,15 @@ void launch_leaf_hash_var(const uint8_t* bytes, const uint64_t* offs, size_t n_l
 // after them instead).
 void launch_merkle_levels(uint32_t* tree, size_t n_leaves, hipStream_t s, uint32_t* root_out = nullptr,
                           uint32_t* flag = nullptr, uint32_t seq = 0, int trees = 1, size_t tree_stride = 0);
+// The whole tree of a [rows][n] matrix (launch_leaf_hash, then
+// launch_merkle_levels with the same root_out, flag, trees and tree_stride;
+// rows per tree a power of two).  Rows of 64 bytes in trees of fewer than
+// 2^17 leaves skip the separate leaf launch: a 64-byte leaf is hashed like a
+// parent, so the first top launch reads the rows as its input layer.
+template <class F>
+void launch_merkle_commit(const typename F::T* mat, size_t rows, size_t n, uint32_t* tree, uint32_t* scratch, hipStream_t s,
+                          uint32_t* root_out = nullptr, uint32_t* flag = nullptr, uint32_t seq = 0, int trees = 1,
+                          size_t tree_stride = 0);
 // A few words copied by the first threads of a launch (for instance a
 // section's last triple, as raw accumulators, to mapped host memory).
 struct WordCopy {

@@ -96,11 +96,13 @@ def test_build_merkle_tree_wide_levels(O, ctx, log_leaves):
 # GF(2^32) shapes cover the transpose-and-scale tiles of 4, 8, 16, 32 and 64
 # columns, two column tiles (128), the generic path (m < 64 or n < 4), and
 # narrow matrices whose leading FFT passes run on the 64-column view (one
-# such pass at 2^8 x 16 and 2^7 x 8, two at 2^14 x 16 and 2^13 x 4)
+# such pass at 2^8 x 16 and 2^7 x 8, two at 2^14 x 16 and 2^13 x 4); rows
+# of 64 bytes (16 GF(2^32) or 4 GF(2^128) columns) hash their leaves inside
+# the tree's first top launch, from 64 leaves to 2^16
 @pytest.mark.parametrize("bits,m,n", [(32, 1 << 10, 64), (32, 1 << 8, 16), (32, 64, 4), (32, 16, 1),
                                       (32, 1 << 7, 8), (32, 1 << 6, 32), (32, 32, 8), (32, 1 << 8, 2),
                                       (128, 1 << 8, 16), (128, 64, 64), (32, 1 << 9, 128),
-                                      (32, 1 << 14, 16), (32, 1 << 13, 4)])
+                                      (32, 1 << 14, 16), (32, 1 << 13, 4), (32, 16, 16), (128, 64, 4)])
 def test_ligero_commit_matches_oracle(O, ctx, bits, m, n):
     say("commit", bits, m, n)
     rs = lh.ReedSolomonEncoding(ctx, m, 4 * m, bits)
