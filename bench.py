@@ -269,10 +269,11 @@ def _peaks(ctx):
     """Microbenchmarked peaks the rooflines are priced against (library
     kernels, best of five launches): GF(2^32) products/s with the FFT passes'
     LDS nibble tables, general GF(2^32) products/s, 32-bit VALU lane-ops/s."""
-    out = {k: ctx.bench_rate(k) for k in ("mul_tab", "mul_clmul", "valu", "mul_tab5", "lds128")}
+    out = {k: ctx.bench_rate(k) for k in ("mul_tab", "mul_clmul", "valu", "mul_tab5", "lds128", "mul_mask", "gf8_logexp")}
     log(f"peaks: GF(2^32) table products {out['mul_tab'] / 1e12:.2f} T/s (5-bit windows "
-        f"{out['mul_tab5'] / 1e12:.2f}), general products {out['mul_clmul'] / 1e12:.2f} T/s, VALU "
-        f"{out['valu'] / 1e12:.1f} T lane-ops/s, LDS 16-byte table reads {out['lds128'] / 1e12:.1f} TB/s")
+        f"{out['mul_tab5'] / 1e12:.2f}, register bit masks {out['mul_mask'] / 1e12:.2f}, GF(2^8) log/exp "
+        f"{out['gf8_logexp'] / 1e12:.2f} T subfield products/s), general products {out['mul_clmul'] / 1e12:.2f} T/s, "
+        f"VALU {out['valu'] / 1e12:.1f} T lane-ops/s, LDS 16-byte table reads {out['lds128'] / 1e12:.1f} TB/s")
     return out
 
 

@@ -69,8 +69,11 @@ int lig_ctx_profile_read(lig_ctx *ctx, const char *phase, double *total_ms, uint
  * second summed over lanes; 3: GF(2^32) products per second by constants
  * with 5-bit-window tables (seven 4-byte LDS reads per product instead of
  * eight); 4: LDS bytes read per second with the GF(2^128) table pattern
- * (16-byte reads, 16 distinct entries per wave instruction).  Best of five
- * launches on the context's GPU. */
+ * (16-byte reads, 16 distinct entries per wave instruction); 5: GF(2^32)
+ * products per second by a constant from its 32 multiples in registers
+ * (bit masks, no tables); 6: GF(2^8) products per second by a constant
+ * through log/exp tables in LDS (the subfield step of a tower
+ * representation).  Best of five launches on the context's GPU. */
 int lig_bench_rate(lig_ctx *ctx, int which, double *rate);
 /* The hipStream_t all work of this context is ordered on. */
 void *lig_ctx_stream(lig_ctx *ctx);

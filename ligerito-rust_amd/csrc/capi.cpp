@@ -200,7 +200,7 @@ int lig_ctx_profile_read(lig_ctx* ctx, const char* phase, double* total_ms, uint
 int lig_bench_rate(lig_ctx* ctx, int which, double* rate) {
     return guarded([&] {
         LIG_REQUIRE(ctx && rate, LIG_E_INVALID, "NULL argument");
-        LIG_REQUIRE(which >= 0 && which <= 4, LIG_E_INVALID, "unknown microbenchmark");
+        LIG_REQUIRE(which >= 0 && which <= 6, LIG_E_INVALID, "unknown microbenchmark");
         LIG_HIP(hipSetDevice(ctx->c.device));
         PBuf<uint32_t> word(&ctx->c.pool, 1);
         try {

@@ -2855,10 +2855,90 @@ __global__ __launch_bounds__(BENCH_THREADS) void k_bench_lds128(uint32_t seed, i
     if ((acc.x ^ acc.y ^ acc.z ^ acc.w) == seed) out[0] = acc.x;
 }
 
+// GF(2^32) products by a wave-uniform constant without tables: its 32
+// multiples c_j = lam * x^j sit in scalar registers and each set bit j of w
+// adds c_j, one sign-extended bit extract and one three-input AND-XOR per
+// bit (64 VALU instructions per product, no LDS).
+__global__ __launch_bounds__(BENCH_THREADS) void k_bench_mul_mask(uint32_t seed, int steps, uint32_t* out) {
+    constexpr int NB = BENCH_CHAINS;
+    uint32_t c[32];
+    c[0] = bench_mix(seed);
+#pragma unroll
+    for (int j = 1; j < 32; j++) c[j] = gf32_dbl(c[j - 1]);
+    uint32_t w[NB];
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < NB; i++) w[i] = bench_mix(id * NB + i + seed);
+    for (int s = 0; s < steps; s++) {
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            uint32_t u = 0;
+#pragma unroll
+            for (int j = 0; j < 32; j++) {
+                const uint32_t m = (uint32_t)((int32_t)(w[i] << (31 - j)) >> 31);  // bit j of w, sign-extended
+                u = __builtin_amdgcn_bitop3_b32(u, c[j], m, 0x78);                    // u ^ (c_j & m)
+            }
+            w[i] ^= u;
+        }
+    }
+    uint32_t x = 0;
+#pragma unroll
+    for (int i = 0; i < NB; i++) x ^= w[i];
+    if (x == seed) out[0] = x;
+}
+
+// GF(2^8) products by a constant through log/exp tables in LDS (the
+// subfield step of a tower representation GF(((2^8)^2)^2), whose GF(2^32)
+// product takes nine of them by Karatsuba): per byte a log read, an add of
+// the constant's log, an exp read (510-entry table, no reduction) and a
+// zero select; four bytes per word.  The rate counts GF(2^8) products.
+__global__ __launch_bounds__(BENCH_THREADS) void k_bench_gf8_logexp(uint32_t seed, int steps, uint32_t* out) {
+    constexpr int NB = BENCH_CHAINS;
+    __shared__ uint8_t lg[256];
+    __shared__ uint8_t ex[512];
+    if (threadIdx.x == 0) {
+        // GF(2^8) mod x^8 + x^4 + x^3 + x^2 + 1 (0x11D), where x (= 2) is primitive
+        uint32_t v = 1;
+        for (int e = 0; e < 255; e++) {
+            ex[e] = (uint8_t)v;
+            ex[e + 255] = (uint8_t)v;
+            lg[v] = (uint8_t)e;
+            v <<= 1;
+            if (v & 0x100u) v ^= 0x11Du;
+        }
+        ex[510] = ex[511] = 0;
+        lg[0] = 0;
+    }
+    __syncthreads();
+    const uint32_t lc = bench_mix(seed) % 255u;  // log of the constant
+    uint32_t w[NB];
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < NB; i++) w[i] = bench_mix(id * NB + i + seed);
+    for (int s = 0; s < steps; s++) {
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            uint32_t r = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t byte = (w[i] >> (8 * k)) & 255u;
+                const uint32_t p = ex[lg[byte] + lc];
+                r |= (byte ? p : 0u) << (8 * k);
+            }
+            w[i] ^= r;
+        }
+    }
+    uint32_t x = 0;
+#pragma unroll
+    for (int i = 0; i < NB; i++) x ^= w[i];
+    if (x == seed) out[0] = x;
+}
+
 double bench_rate(int which, uint32_t* scratch_word, hipStream_t s) {
-    static const int steps_of[5] = {1024, 256, 4096, 1024, 1024};
-    static const double per_step_of[5] = {BENCH_CHAINS, BENCH_CHAINS, 16.0, BENCH_CHAINS, BENCH_CHAINS * 16.0};
-    if (which < 0 || which > 4) throw std::invalid_argument("unknown microbenchmark");
+    static const int steps_of[7] = {1024, 256, 4096, 1024, 1024, 128, 512};
+    static const double per_step_of[7] = {BENCH_CHAINS, BENCH_CHAINS, 16.0, BENCH_CHAINS, BENCH_CHAINS * 16.0,
+                                          BENCH_CHAINS, BENCH_CHAINS * 4.0};
+    if (which < 0 || which > 6) throw std::invalid_argument("unknown microbenchmark");
     const int steps = steps_of[which];
     const double per_step = per_step_of[which];
     auto launch = [&](int st) {
@@ -2868,7 +2948,9 @@ double bench_rate(int which, uint32_t* scratch_word, hipStream_t s) {
             case 1: hipLaunchKernelGGL(k_bench_mul_clmul, g, b, 0, s, 12345u, st, scratch_word); break;
             case 2: hipLaunchKernelGGL(k_bench_valu, g, b, 0, s, 12345u, st, scratch_word); break;
             case 3: hipLaunchKernelGGL(k_bench_mul_tab5, g, b, 0, s, 12345u, st, scratch_word); break;
-            default: hipLaunchKernelGGL(k_bench_lds128, g, b, 0, s, 12345u, st, scratch_word); break;
+            case 4: hipLaunchKernelGGL(k_bench_lds128, g, b, 0, s, 12345u, st, scratch_word); break;
+            case 5: hipLaunchKernelGGL(k_bench_mul_mask, g, b, 0, s, 12345u, st, scratch_word); break;
+            default: hipLaunchKernelGGL(k_bench_gf8_logexp, g, b, 0, s, 12345u, st, scratch_word); break;
         }
     };
     hipEvent_t a, b;

@@ -186,10 +186,12 @@ class Context(_Handle):
     def bench_rate(self, which: str) -> float:
         """Microbenchmarked peak: "mul_tab" (GF(2^32) products/s, LDS nibble
         tables), "mul_clmul" (general GF(2^32) products/s), "valu" (32-bit
-        lane-ops/s), "mul_tab5" (GF(2^32) products/s, 5-bit-window tables)
-        or "lds128" (LDS bytes/s, 16-byte table reads)."""
+        lane-ops/s), "mul_tab5" (GF(2^32) products/s, 5-bit-window tables),
+        "lds128" (LDS bytes/s, 16-byte table reads), "mul_mask" (GF(2^32)
+        products/s, bit masks over a constant's multiples in registers) or
+        "gf8_logexp" (GF(2^8) products/s, log/exp tables in LDS)."""
         r = C.c_double(0)
-        idx = {"mul_tab": 0, "mul_clmul": 1, "valu": 2, "mul_tab5": 3, "lds128": 4}[which]
+        idx = {"mul_tab": 0, "mul_clmul": 1, "valu": 2, "mul_tab5": 3, "lds128": 4, "mul_mask": 5, "gf8_logexp": 6}[which]
         _check(lib().lig_bench_rate(self._h, idx, C.byref(r)))
         return r.value
 
