@@ -116,6 +116,39 @@ def test_ligero_commit_matches_oracle(O, ctx, bits, m, n):
         assert np.array_equal(w.extract_row(r), mat.reshape(n, 4 * m, -1)[:, r].reshape(w.extract_row(r).shape))
 
 
+def test_merkle_with_ligero_data(O, ctx):
+    # the reference's test_merkle_with_ligero_data (config.rs:143-203): the
+    # input poly[i] = i, hardcoded_config_20's first commit (2^14 x 64 ->
+    # 2^16 rows), the rows at queries [0, 1, 100, 1959, 50294, 50459] opened
+    # and their batched Merkle proof verified against the root; here the root,
+    # the tree, the rows and the proof also equal the oracle's
+    m, n = 1 << 14, 64
+    say("merkle with ligero data")
+    poly = np.arange(m * n, dtype=np.uint32)
+    rs = lh.ReedSolomonEncoding(ctx, m, 4 * m, 32)
+    w = lh.ligero_commit(poly, m, n, rs)
+    root, mat, layers = O.ligero_commit(poly, m, n, 32, want_matrix=True, want_layers=True)
+    assert w.root() == root
+    tree = w.tree()
+    assert tree == layers
+    queries = sorted([0, 1, 100, 50459, 50294, 1959])
+    rows = [w.extract_row(q) for q in queries]
+    for q, r in zip(queries, rows):
+        assert np.array_equal(r, mat.reshape(n, 4 * m)[:, q])
+    leaves = np.stack([np.ascontiguousarray(r).view(np.uint8) for r in rows])
+    proof = lh.prove(tree, queries)
+    depth = len(tree) - 1
+    assert lh.verify(tree[-1], proof, depth, leaves, queries)
+    rb = np.ascontiguousarray(mat.reshape(n, 4 * m).T).view(np.uint8)
+    row_bytes = [rb[r].tobytes() for r in range(4 * m)]
+    assert proof == O.merkle_prove(row_bytes, queries)
+    assert O.merkle_verify(root, proof, depth, [bytes(x) for x in leaves], queries)
+    # a changed row no longer verifies
+    bad = leaves.copy()
+    bad[2, 0] ^= 1
+    assert not lh.verify(tree[-1], proof, depth, bad, queries)
+
+
 def test_commit_tree_2e19_leaves(O, ctx):
     # 2^19 leaves: two single-level launches to 2^17 nodes, then subtree
     # launches of depth 7 and the one-workgroup top
